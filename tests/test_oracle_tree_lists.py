@@ -1,0 +1,141 @@
+"""CPU: pins the oracle's trees (quadrature_octree.cpp / calculation_point_octree.cpp
+restated) and interaction maps (fast_multipole_solver.cpp:105-184) by brute force."""
+import numpy as np
+import pytest
+
+from conftest import shell_sources
+
+CHILD_SIGN = np.array([(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (-1, -1, 1), (1, -1, 1), (1, 1, 1),
+                       (-1, 1, 1)])
+
+
+def _in_box(P, c, h):
+    return np.all((c - h <= P) & (P < c + h), axis=1)
+
+
+def _adversarial(rng, n):
+    """Points on dyadic planes of the [-3,3)^3 root, plus random ones."""
+    k = rng.integers(0, 5, (n, 3))
+    P = -3 + 6 * rng.integers(0, 2 ** k) / 2.0 ** k
+    mix = rng.random((n, 3)) < 0.5
+    return np.where(mix, P, rng.uniform(-3, 3, (n, 3)))
+
+
+@pytest.mark.parametrize("kind", ["shell", "adversarial", "clustered"])
+def test_source_tree_rules(orc, kind):
+    rng = np.random.default_rng(11)
+    if kind == "shell":
+        P = shell_sources(rng, 3000)
+    elif kind == "adversarial":
+        P = _adversarial(rng, 3000)
+        P[:5] = 3.0          # outside the half-open root: dropped (quadrature_octree.cpp:33-36)
+    else:
+        P = rng.normal(scale=0.05, size=(3000, 3)) + 0.3
+    cap = 16
+    T = orc.source_tree(P, cap)
+    inside = _in_box(P, np.zeros(3), np.full(3, 3.0))
+    # every point inside the root lands in exactly one leaf, once
+    assert sorted(T.slots.tolist()) == np.nonzero(inside)[0].tolist()
+    for b in range(len(T.level)):
+        cnt = int(_in_box(P, T.center[b], T.half[b]).sum())
+        sub = T.first_child[b] >= 0
+        # subdivided <=> population >= cap (source leaf holds <= cap-1, quadrature_octree.cpp:38);
+        # coincident points stop at the 21-level depth cap (the reference recurses without bound)
+        assert sub == (cnt >= cap and T.level[b] < 21), (b, cnt, sub)
+        assert T.end[b] - T.begin[b] == cnt
+        if not sub:
+            idx = T.slots[T.begin[b]:T.end[b]]
+            assert np.all(np.diff(idx) > 0)                 # insertion (input) order
+            assert np.all(_in_box(P[idx], T.center[b], T.half[b]))
+        else:
+            fc = T.first_child[b]
+            for o in range(8):
+                ch = fc + o
+                assert T.parent[ch] == b and T.level[ch] == T.level[b] + 1
+                # reference child order and half sizes (quadrature_octree.cpp:88-96)
+                assert np.array_equal(np.sign(T.center[ch] - T.center[b]), CHILD_SIGN[o])
+                assert np.allclose(T.half[ch], T.half[b] / 2, rtol=0, atol=0)
+
+
+def test_target_tree_capacity_and_bbox_root(orc):
+    rng = np.random.default_rng(12)
+    P = rng.uniform(-2, 2, (2500, 3))
+    cap = 20
+    T = orc.target_tree(P, cap)
+    c, h = orc.bounding_box(P)
+    # math.cpp:39-57
+    mn, mx = P.min(0), P.max(0)
+    assert np.array_equal(c, mn + ((mx - mn) / 2 + 1e-5))
+    assert np.array_equal(h, ((mx - mn) / 2 + 1e-5) * 1.1)
+    assert np.array_equal(T.center[0], c) and np.array_equal(T.half[0], h)
+    for b in range(len(T.level)):
+        if T.first_child[b] < 0:
+            assert T.npts[b] <= cap           # target leaf holds <= cap (calculation_point_octree.cpp:41)
+        else:
+            # children centres rounded through float (calculation_point_octree.cpp:76-96)
+            f = np.float32
+            fc = T.first_child[b]
+            for o in range(8):
+                exp = np.array([f(f(T.center[b][a]) + f(CHILD_SIGN[o][a]) * (f(T.half[b][a]) / f(2)))
+                                for a in range(3)], dtype=np.float64)
+                assert np.array_equal(T.center[fc + o], exp)
+    # population rule on nodes whose boxes tile their parent exactly is subdivided <=> count > cap
+    assert T.first_child[0] >= 0
+
+
+def _leaf_of(T, n):
+    leaf = np.full(n, -1)
+    for b in range(len(T.level)):
+        if T.first_child[b] < 0:
+            leaf[T.slots[T.begin[b]:T.end[b]]] = b
+    return leaf
+
+
+def _ancestors(T, b):
+    out = []
+    while b >= 0:
+        out.append(b)
+        b = T.parent[b]
+    return out
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_interaction_lists_cover_every_pair_once(orc, shared):
+    rng = np.random.default_rng(13)
+    src = shell_sources(rng, 1500)
+    tgt = rng.uniform(-2, 2, (400, 3))
+    S = orc.source_tree(src, 12)
+    root = (np.zeros(3), np.full(3, 3.0)) if shared else None
+    T = orc.target_tree(tgt, 10, root=root)
+    (ft, fs), (ct, cs) = orc.interaction_lists(S, T)
+    rad_s = np.linalg.norm(S.half, axis=1)
+    rad_t = np.linalg.norm(T.half, axis=1)
+
+    def far(s, t):
+        md = 2 * max(rad_s[s], rad_t[t]) + min(rad_s[s], rad_t[t])
+        d = S.center[s] - T.center[t]
+        return md * md < d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+
+    assert all(far(s, t) for t, s in zip(ft, fs))
+    assert not any(far(s, t) for t, s in zip(ct, cs))
+    # close pairs only at useful target leaves
+    assert np.all(T.npts[ct] > 0)
+    # coverage: every (target point, source point) pair counted exactly once
+    src_leaf = _leaf_of(S, len(src))
+    far_set = {}
+    for t, s in zip(ft, fs):
+        far_set.setdefault(int(t), []).append(int(s))
+    close_set = {}
+    for t, s in zip(ct, cs):
+        close_set.setdefault(int(t), []).append(int(s))
+    src_anc = [set(_ancestors(S, src_leaf[j])) for j in range(len(src))]
+    for b in range(len(T.level)):
+        if T.npts[b] == 0:
+            continue
+        hits = np.zeros(len(src), int)
+        for a in _ancestors(T, b):
+            for s in far_set.get(a, []):
+                hits += np.array([s in anc for anc in src_anc])
+        for s in close_set.get(b, []):
+            hits += np.array([s in anc for anc in src_anc])
+        assert np.all(hits == 1), (b, np.unique(hits))
