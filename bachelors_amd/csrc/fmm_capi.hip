@@ -1,0 +1,773 @@
+// C ABI and stage orchestration of the MI355X Biot-Savart FMM (include/fmm_capi.h).
+//
+// Stage order follows FastMultipoleSolver (fast_multipole_solver.cpp): trees, leaf
+// multipoles (P2M), upward pass (M2M), downward pass (interaction lists, M2L, L2L),
+// evaluation (L2P curl + near field).  Everything after the inputs is device resident:
+// the host only plans per-level pair lists and GEMM tiles from small node tables.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fmm_capi.h"
+#include "fmm_adjoint.hpp"
+#include "fmm_expand.hpp"
+#include "fmm_gemm.hpp"
+#include "fmm_lists.hpp"
+#include "fmm_prims.hpp"
+#include "fmm_tables.hpp"
+#include "fmm_tree.hpp"
+
+using namespace fmm;
+
+namespace {
+
+enum Stage { ST_EMPTY = 0, ST_TREES = 1, ST_LEAVES = 2, ST_UP = 3, ST_DOWN = 4, ST_EVAL = 5 };
+enum Ev { E_START, E_TREE, E_P2M, E_M2M, E_LISTS, E_TABLES, E_M2L, E_L2L, E_L2P, E_P2P, E_END, E_COUNT };
+
+struct Pass {               // one batched GEMM launch: tiles over (src, dst) pairs
+    int ntiles = 0, npairs = 0;
+    DevBuf tiles, src, dst;
+};
+
+struct OpTables {           // FP64 master + the precision actually applied
+    DevBuf f64, f32, hi, lo, specs;
+    int nops = 0;
+};
+
+}  // namespace
+
+struct fmm_ctx {
+    fmm_config cfg;
+    std::string err;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int p = 10, L = 121, Lp = 128;
+    Precision prec = Precision::F64;
+    // inputs (input order)
+    int ns = 0, nt = 0;
+    bool have_J = false;
+    DevBuf src_xyz, src_w, src_J, tgt_xyz;
+    // trees
+    TreeDev S, T;
+    TreeScratch tscr;
+    ScaleTable sigS, sigT;
+    DevBuf sxyz, sq, sJ, txyz;                   // slot order
+    std::vector<int> h_src_leaves, h_tgt_leaves;
+    DevBuf src_leaves, tgt_leaves;
+    // expansions [node][3][Lp]
+    DevBuf M, Lx;
+    // translations
+    OpTables up_ops, down_ops;
+    std::vector<Pass> m2m_pass, l2l_pass;
+    Pass m2l_pass;
+    Lists lists;
+    ListScratch lscr;
+    // evaluation
+    DevBuf Bfar, Bnear, Bout;
+    double current = 1.0;
+    // adjoint state
+    AdjointState adj;
+    int stage = ST_EMPTY;
+    hipEvent_t ev[E_COUNT];
+    bool ev_ok = false;
+    long long near_inter = -1;
+    bool lists_ready = false;
+
+    bool f64() const { return prec == Precision::F64; }
+    size_t esize() const { return f64() ? sizeof(double) : sizeof(float); }
+    void rec(int e) { if (ev_ok) FMM_HIP(hipEventRecord(ev[e], stream)); }
+};
+
+namespace {
+
+template <class F> int guard(fmm_ctx* ctx, F&& f) {
+    if (!ctx) return FMM_ERR_ARG;
+    try {
+        f();
+        ctx->err.clear();
+        return FMM_OK;
+    } catch (const Error& e) {
+        ctx->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return FMM_ERR_HIP;
+    }
+}
+
+inline int blocksN(long long n) { return (int)((n + 255) / 256); }
+
+void require(bool c, int code, const char* msg) {
+    if (!c) throw Error(code, msg);
+}
+
+bool nonempty_host(const TreeDev& T, int n) { return T.h_first_child[n] >= 0 || T.h_npts[n] > 0; }
+
+// reference child order (quadrature_octree.cpp:88-96)
+const int SIGN[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                        {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+
+void upload_pass(Pass& P, const std::vector<int>& src, const std::vector<int>& dst, const std::vector<GemmTile>& tiles,
+                 hipStream_t st) {
+    P.npairs = (int)src.size();
+    P.ntiles = (int)tiles.size();
+    if (P.npairs) {
+        FMM_HIP(hipMemcpyAsync(P.src.get<int>(P.npairs), src.data(), P.npairs * sizeof(int), hipMemcpyHostToDevice, st));
+        FMM_HIP(hipMemcpyAsync(P.dst.get<int>(P.npairs), dst.data(), P.npairs * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    if (P.ntiles)
+        FMM_HIP(hipMemcpyAsync(P.tiles.get<GemmTile>(P.ntiles), tiles.data(), P.ntiles * sizeof(GemmTile),
+                               hipMemcpyHostToDevice, st));
+}
+
+void finish_tables(fmm_ctx* c, OpTables& O) {
+    size_t n = (size_t)O.nops * c->Lp * c->Lp;
+    if (!n) return;
+    if (c->prec == Precision::F32) operators_to_f32(O.f64.as<double>(), O.f32.get<float>(n), n, c->stream);
+    if (c->prec == Precision::BF16X3)
+        operators_to_bf16x2(O.f64.as<double>(), O.hi.get<__bf16>(n), O.lo.get<__bf16>(n), n, c->stream);
+}
+
+const void* ops_ptr(const fmm_ctx* c, const OpTables& O) {
+    if (c->prec == Precision::F64) return O.f64.ptr;
+    if (c->prec == Precision::F32) return O.f32.ptr;
+    return O.hi.ptr;
+}
+
+void run_pass(fmm_ctx* c, const OpTables& O, const Pass& P, const void* src_exp, void* dst_exp, bool atomic) {
+    if (!P.ntiles) return;
+    class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, P.src.as<int>(), P.dst.as<int>(),
+               src_exp, dst_exp, c->Lp, atomic, c->stream);
+}
+
+void stage_trees(fmm_ctx* c) {
+    require(c->ns > 0 || c->nt > 0, FMM_ERR_STATE, "sources and targets are not set");
+    hipStream_t st = c->stream;
+    c->rec(E_START);
+    const fmm_config& g = c->cfg;
+    build_tree(c->S, c->tscr, c->src_xyz.as<double>(), c->ns, g.source_root_center, g.source_root_half,
+               g.source_leaf_capacity, g.max_depth, true, st);
+    double tc[3], th[3];
+    if (g.frame == FMM_FRAME_SHARED) {
+        for (int a = 0; a < 3; a++) { tc[a] = g.source_root_center[a]; th[a] = g.source_root_half[a]; }
+    } else {
+        // math::getBoundingBox (math.cpp:39-57) on the host copy of the targets
+        std::vector<double> P(3 * (size_t)c->nt);
+        FMM_HIP(hipMemcpyAsync(P.data(), c->tgt_xyz.ptr, P.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipStreamSynchronize(st));
+        for (int a = 0; a < 3; a++) {
+            double mx = -1e+16, mn = 1e+16;
+            for (int i = 0; i < c->nt; i++) {
+                double v = P[3 * (size_t)i + a];
+                if (v > mx) mx = v;
+                if (v < mn) mn = v;
+            }
+            double hd = (mx - mn) / 2 + 1e-5;
+            tc[a] = mn + hd;
+            th[a] = hd * 1.1;
+        }
+    }
+    build_tree(c->T, c->tscr, c->tgt_xyz.as<double>(), c->nt, tc, th, g.target_leaf_capacity + 1, g.max_depth,
+               g.frame == FMM_FRAME_SHARED, st);
+    for (int k = 0; k < MAX_LEVELS; k++) { c->sigS.s[k] = c->S.scale[k]; c->sigT.s[k] = c->T.scale[k]; }
+    // sources and targets in slot (leaf DFS) order
+    int nss = c->S.num_slots, nts = c->T.num_slots;
+    launch_charges(c->S.slots.as<int>(), nss, c->src_xyz.as<double>(), c->src_w.as<double>(),
+                   c->have_J ? c->src_J.as<double>() : nullptr, c->sxyz.get<double>(3 * (size_t)std::max(nss, 1)),
+                   c->sq.get<double>(3 * (size_t)std::max(nss, 1)), c->sJ.get<double>(3 * (size_t)std::max(nss, 1)), st);
+    launch_gather_xyz(c->T.slots.as<int>(), nts, c->tgt_xyz.as<double>(), c->txyz.get<double>(3 * (size_t)std::max(nts, 1)),
+                      st);
+    // useful leaves
+    c->h_src_leaves.clear();
+    c->h_tgt_leaves.clear();
+    for (int n = 0; n < c->S.num_nodes; n++)
+        if (c->S.h_first_child[n] < 0 && c->S.h_npts[n] > 0) c->h_src_leaves.push_back(n);
+    for (int n = 0; n < c->T.num_nodes; n++)
+        if (c->T.h_first_child[n] < 0 && c->T.h_npts[n] > 0) c->h_tgt_leaves.push_back(n);
+    if (!c->h_src_leaves.empty())
+        FMM_HIP(hipMemcpyAsync(c->src_leaves.get<int>(c->h_src_leaves.size()), c->h_src_leaves.data(),
+                               c->h_src_leaves.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    if (!c->h_tgt_leaves.empty())
+        FMM_HIP(hipMemcpyAsync(c->tgt_leaves.get<int>(c->h_tgt_leaves.size()), c->h_tgt_leaves.data(),
+                               c->h_tgt_leaves.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    c->near_inter = -1;
+    c->lists_ready = false;
+    c->rec(E_TREE);
+    c->stage = ST_TREES;
+}
+
+void stage_leaves(fmm_ctx* c) {
+    require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built");
+    size_t n = (size_t)c->S.num_nodes * 3 * c->Lp;
+    void* M = c->M.reserve(n * c->esize());
+    FMM_HIP(hipMemsetAsync(M, 0, n * c->esize(), c->stream));
+    launch_p2m(c->f64(), c->src_leaves.as<int>(), (int)c->h_src_leaves.size(), c->S, c->sigS, c->sxyz.as<double>(),
+               c->sq.as<double>(), c->p, c->Lp, M, c->stream);
+    c->rec(E_P2M);
+    c->stage = ST_LEAVES;
+}
+
+// M2M: per level, 8 orientation classes of (child -> parent) pairs (multipole_solver.cpp:341-423)
+void stage_upward(fmm_ctx* c) {
+    require(c->stage >= ST_LEAVES, FMM_ERR_STATE, "leaf multipoles are not ready");
+    const TreeDev& S = c->S;
+    int nlev = S.num_levels;
+    OpTables& O = c->up_ops;
+    O.nops = 8 * std::max(nlev - 1, 0);
+    std::vector<OpSpec> specs(O.nops);
+    for (int k = 1; k < nlev; k++)
+        for (int o = 0; o < 8; o++) {
+            OpSpec& s = specs[(k - 1) * 8 + o];
+            for (int a = 0; a < 3; a++) s.t[a] = SIGN[o][a] * S.half[k][a];   // child - parent
+            s.sig_in = S.scale[k];
+            s.sig_out = S.scale[k - 1];
+            s.kind = OPK_M2M;
+            s.m2l_full = 0;
+        }
+    if (O.nops) {
+        OpSpec* d = O.specs.get<OpSpec>(O.nops);
+        FMM_HIP(hipMemcpyAsync(d, specs.data(), O.nops * sizeof(OpSpec), hipMemcpyHostToDevice, c->stream));
+        make_operators(d, O.nops, c->p, c->Lp, O.f64.get<double>((size_t)O.nops * c->Lp * c->Lp), c->stream);
+        finish_tables(c, O);
+    }
+    c->m2m_pass.resize(MAX_LEVELS);
+    for (int k = nlev - 1; k >= 1; k--) {
+        std::vector<int> src[8], dst[8];
+        for (int n = S.level_begin[k - 1]; n < S.level_begin[k]; n++) {
+            int fc = S.h_first_child[n];
+            if (fc < 0) continue;
+            for (int o = 0; o < 8; o++)
+                if (nonempty_host(S, fc + o)) { src[o].push_back(fc + o); dst[o].push_back(n); }
+        }
+        std::vector<int> all_s, all_d, cb(1, 0), cop;
+        for (int o = 0; o < 8; o++) {
+            all_s.insert(all_s.end(), src[o].begin(), src[o].end());
+            all_d.insert(all_d.end(), dst[o].begin(), dst[o].end());
+            cb.push_back((int)all_s.size());
+            cop.push_back((k - 1) * 8 + o);
+        }
+        std::vector<GemmTile> tiles;
+        plan_tiles(cb, cop, tiles);
+        upload_pass(c->m2m_pass[k], all_s, all_d, tiles, c->stream);
+        run_pass(c, O, c->m2m_pass[k], c->M.ptr, c->M.ptr, true);
+    }
+    c->rec(E_M2M);
+    c->stage = ST_UP;
+}
+
+void stage_downward(fmm_ctx* c) {
+    require(c->stage >= ST_UP, FMM_ERR_STATE, "Multipoles are not ready!");
+    hipStream_t st = c->stream;
+    const TreeDev& S = c->S;
+    const TreeDev& T = c->T;
+    const bool shared = c->cfg.frame == FMM_FRAME_SHARED;
+    build_lists(c->lists, c->lscr, S, T, c->cfg.mac_scale, shared, st);
+    c->lists_ready = true;
+    require(shared, FMM_ERR_UNSUPPORTED,
+            "the M2L pass needs the shared frame (reference-frame trees and lists are available for parity)");
+    c->rec(E_LISTS);
+    // operator tables: L2L per (level, orientation), then one M2L operator per class
+    OpTables& O = c->down_ops;
+    int nlev = T.num_levels;
+    int n_l2l = 8 * std::max(nlev - 1, 0);
+    int nc = c->lists.num_classes;
+    O.nops = n_l2l + nc;
+    OpSpec* d = O.specs.get<OpSpec>(std::max(O.nops, 1));
+    std::vector<OpSpec> specs(n_l2l);
+    for (int k = 1; k < nlev; k++)
+        for (int o = 0; o < 8; o++) {
+            OpSpec& s = specs[(k - 1) * 8 + o];
+            for (int a = 0; a < 3; a++) s.t[a] = -SIGN[o][a] * T.half[k][a];   // parent - child
+            s.sig_in = T.scale[k - 1];
+            s.sig_out = T.scale[k];
+            s.kind = OPK_L2L;
+            s.m2l_full = 0;
+        }
+    if (n_l2l) FMM_HIP(hipMemcpyAsync(d, specs.data(), n_l2l * sizeof(OpSpec), hipMemcpyHostToDevice, st));
+    make_m2l_specs(c->lists.class_begin.as<int>(), nc, c->lists.far_t.as<int>(), c->lists.far_s.as<int>(),
+                   S.center.as<double>(), T.center.as<double>(), S.level.as<int>(), T.level.as<int>(), c->sigS, c->sigT,
+                   c->cfg.m2l_full, d + n_l2l, st);
+    if (O.nops) {
+        make_operators(d, O.nops, c->p, c->Lp, O.f64.get<double>((size_t)O.nops * c->Lp * c->Lp), st);
+        finish_tables(c, O);
+    }
+    size_t n = (size_t)T.num_nodes * 3 * c->Lp;
+    void* Lx = c->Lx.reserve(n * c->esize());
+    FMM_HIP(hipMemsetAsync(Lx, 0, n * c->esize(), st));
+    // M2L: all classes in one launch (fast_multipole_solver.cpp:258-276); the events
+    // around it bracket the GEMM alone
+    Pass& P = c->m2l_pass;
+    {
+        std::vector<int> cop(nc);
+        for (int i = 0; i < nc; i++) cop[i] = n_l2l + i;
+        std::vector<GemmTile> tiles;
+        plan_tiles(c->lists.h_class_begin, cop, tiles);
+        P.npairs = c->lists.num_far;
+        P.ntiles = (int)tiles.size();
+        if (P.ntiles)
+            FMM_HIP(hipMemcpyAsync(P.tiles.get<GemmTile>(P.ntiles), tiles.data(), P.ntiles * sizeof(GemmTile),
+                                   hipMemcpyHostToDevice, st));
+    }
+    c->rec(E_TABLES);
+    if (P.ntiles)
+        class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, c->lists.far_s.as<int>(),
+                   c->lists.far_t.as<int>(), c->M.ptr, Lx, c->Lp, true, st);
+    c->rec(E_M2L);
+    // L2L top-down (calculation_point_octree.cpp:277-299)
+    c->l2l_pass.resize(MAX_LEVELS);
+    for (int k = 1; k < nlev; k++) {
+        std::vector<int> src[8], dst[8];
+        for (int nd = T.level_begin[k - 1]; nd < T.level_begin[k]; nd++) {
+            int fc = T.h_first_child[nd];
+            if (fc < 0) continue;
+            for (int o = 0; o < 8; o++)
+                if (nonempty_host(T, fc + o)) { src[o].push_back(nd); dst[o].push_back(fc + o); }
+        }
+        std::vector<int> all_s, all_d, cb(1, 0), cop;
+        for (int o = 0; o < 8; o++) {
+            all_s.insert(all_s.end(), src[o].begin(), src[o].end());
+            all_d.insert(all_d.end(), dst[o].begin(), dst[o].end());
+            cb.push_back((int)all_s.size());
+            cop.push_back((k - 1) * 8 + o);
+        }
+        std::vector<GemmTile> tiles;
+        plan_tiles(cb, cop, tiles);
+        upload_pass(c->l2l_pass[k], all_s, all_d, tiles, st);
+        run_pass(c, O, c->l2l_pass[k], Lx, Lx, false);
+    }
+    c->rec(E_L2L);
+    c->stage = ST_DOWN;
+}
+
+__global__ void k_fill_nan(double* p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = __longlong_as_double(0x7ff8000000000000ll);
+}
+
+void stage_evaluate(fmm_ctx* c, double current, double* Bdev) {
+    require(c->stage >= ST_DOWN, FMM_ERR_STATE, "local expansions are not ready");
+    hipStream_t st = c->stream;
+    int nts = c->T.num_slots;
+    double* Bfar = c->Bfar.get<double>(3 * (size_t)std::max(nts, 1));
+    double* Bnear = c->Bnear.get<double>(3 * (size_t)std::max(nts, 1));
+    launch_l2p(c->f64(), c->tgt_leaves.as<int>(), (int)c->h_tgt_leaves.size(), c->T, c->sigT, c->txyz.as<double>(),
+               c->Lx.ptr, c->p, c->Lp, Bfar, st);
+    c->rec(E_L2P);
+    if (c->cfg.near_field) {
+        launch_p2p(c->f64(), c->tgt_leaves.as<int>(), (int)c->h_tgt_leaves.size(), c->T, c->S, c->lists.close_begin.as<int>(),
+                   c->lists.close_end.as<int>(), c->lists.close_s.as<int>(), c->sxyz.as<double>(), c->sq.as<double>(),
+                   c->txyz.as<double>(), Bnear, st);
+    } else {
+        FMM_HIP(hipMemsetAsync(Bnear, 0, 3 * (size_t)std::max(nts, 1) * sizeof(double), st));
+    }
+    c->rec(E_P2P);
+    size_t nout = 3 * (size_t)c->nt;
+    if (nout) k_fill_nan<<<(unsigned)((nout + 255) / 256), 256, 0, st>>>(Bdev, nout);
+    c->current = current;
+    launch_assemble(c->T.slots.as<int>(), nts, Bfar, Bnear, current * REF_MU0 / (4.0 * REF_PI), Bdev, st);
+    c->rec(E_END);
+    c->stage = ST_EVAL;
+}
+
+void set_sources_common(fmm_ctx* c, size_t n, const double* xyz, const double* w, const double* J, hipMemcpyKind kind) {
+    require(n < (size_t)INT32_MAX / 3, FMM_ERR_LIMIT, "too many sources");
+    require(n == 0 || (xyz && w), FMM_ERR_ARG, "null source arrays");
+    c->ns = (int)n;
+    c->have_J = J != nullptr;
+    size_t nn = std::max<size_t>(n, 1);
+    if (n) {
+        FMM_HIP(hipMemcpyAsync(c->src_xyz.get<double>(3 * nn), xyz, 3 * n * sizeof(double), kind, c->stream));
+        FMM_HIP(hipMemcpyAsync(c->src_w.get<double>(nn), w, n * sizeof(double), kind, c->stream));
+        if (J) FMM_HIP(hipMemcpyAsync(c->src_J.get<double>(3 * nn), J, 3 * n * sizeof(double), kind, c->stream));
+    } else {
+        c->src_xyz.get<double>(3);
+        c->src_w.get<double>(1);
+    }
+    FMM_HIP(hipStreamSynchronize(c->stream));
+    c->stage = ST_EMPTY;
+}
+
+void set_targets_common(fmm_ctx* c, size_t m, const double* xyz, hipMemcpyKind kind) {
+    require(m < (size_t)INT32_MAX / 3, FMM_ERR_LIMIT, "too many targets");
+    require(m == 0 || xyz, FMM_ERR_ARG, "null target array");
+    c->nt = (int)m;
+    size_t mm = std::max<size_t>(m, 1);
+    if (m) FMM_HIP(hipMemcpyAsync(c->tgt_xyz.get<double>(3 * mm), xyz, 3 * m * sizeof(double), kind, c->stream));
+    else c->tgt_xyz.get<double>(3);
+    FMM_HIP(hipStreamSynchronize(c->stream));
+    c->stage = ST_EMPTY;
+}
+
+}  // namespace
+
+extern "C" {
+
+void fmm_config_init(fmm_config* g) {
+    if (!g) return;
+    std::memset(g, 0, sizeof(*g));
+    g->order = 10;
+    g->source_leaf_capacity = 1000;   // reference defaults (fast_multipole_solver.hpp:29-30)
+    g->target_leaf_capacity = 100;
+    g->precision = FMM_PREC_FP64;
+    g->frame = FMM_FRAME_SHARED;
+    g->m2l_full = 1;
+    g->max_depth = 21;
+    g->near_field = 1;
+    g->mac_scale = 1.0;
+    for (int a = 0; a < 3; a++) { g->source_root_center[a] = 0.0; g->source_root_half[a] = 3.0; }
+}
+
+int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
+    if (!out || !cfg) return FMM_ERR_ARG;
+    *out = nullptr;
+    if (cfg->order < 1 || cfg->order > MAX_ORDER) return FMM_ERR_ARG;
+    if (cfg->source_leaf_capacity < 2 || cfg->target_leaf_capacity < 1) return FMM_ERR_ARG;
+    if (cfg->precision < 0 || cfg->precision > 2 || cfg->frame < 0 || cfg->frame > 1) return FMM_ERR_ARG;
+    if (!(cfg->mac_scale >= 1.0)) return FMM_ERR_ARG;
+    fmm_ctx* c = new fmm_ctx();
+    c->cfg = *cfg;
+    if (c->cfg.max_depth <= 0 || c->cfg.max_depth > MAX_LEVELS - 1) c->cfg.max_depth = MAX_LEVELS - 1;
+    c->p = cfg->order;
+    c->L = nlen(c->p);
+    c->Lp = pad_to(c->L, 32);
+    c->prec = (Precision)cfg->precision;
+    int rc = guard(c, [&] {
+        int ndev = 0;
+        FMM_HIP(hipGetDeviceCount(&ndev));
+        if (ndev < 1) throw Error(FMM_ERR_HIP, "no HIP device");
+        FMM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+        for (int i = 0; i < E_COUNT; i++) FMM_HIP(hipEventCreate(&c->ev[i]));
+        c->ev_ok = true;
+    });
+    if (rc != FMM_OK) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return FMM_OK;
+}
+
+void fmm_destroy(fmm_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ev_ok)
+        for (int i = 0; i < E_COUNT; i++) (void)hipEventDestroy(c->ev[i]);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* fmm_last_error(const fmm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int fmm_set_stream(fmm_ctx* c, void* s) {
+    return guard(c, [&] {
+        FMM_HIP(hipStreamSynchronize(c->stream));
+        if (s) {
+            if (c->own_stream) FMM_HIP(hipStreamDestroy(c->stream));
+            c->own_stream = false;
+            c->stream = (hipStream_t)s;
+        }
+    });
+}
+
+int fmm_synchronize(fmm_ctx* c) { return guard(c, [&] { FMM_HIP(hipStreamSynchronize(c->stream)); }); }
+
+int fmm_set_sources(fmm_ctx* c, size_t n, const double* xyz, const double* w, const double* J) {
+    return guard(c, [&] { set_sources_common(c, n, xyz, w, J, hipMemcpyHostToDevice); });
+}
+int fmm_set_targets(fmm_ctx* c, size_t m, const double* xyz) {
+    return guard(c, [&] { set_targets_common(c, m, xyz, hipMemcpyHostToDevice); });
+}
+int fmm_set_sources_dev(fmm_ctx* c, size_t n, const double* xyz, const double* w, const double* J) {
+    return guard(c, [&] { set_sources_common(c, n, xyz, w, J, hipMemcpyDeviceToDevice); });
+}
+int fmm_set_targets_dev(fmm_ctx* c, size_t m, const double* xyz) {
+    return guard(c, [&] { set_targets_common(c, m, xyz, hipMemcpyDeviceToDevice); });
+}
+int fmm_set_weights_dev(fmm_ctx* c, const double* w) {
+    return guard(c, [&] {
+        require(w != nullptr, FMM_ERR_ARG, "null weights");
+        if (c->ns) FMM_HIP(hipMemcpyAsync(c->src_w.ptr, w, c->ns * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        if (c->stage >= ST_TREES) {
+            int nss = c->S.num_slots;
+            launch_charges(c->S.slots.as<int>(), nss, c->src_xyz.as<double>(), c->src_w.as<double>(),
+                           c->have_J ? c->src_J.as<double>() : nullptr, c->sxyz.as<double>(), c->sq.as<double>(),
+                           c->sJ.as<double>(), c->stream);
+            c->stage = ST_TREES;
+        }
+    });
+}
+
+int fmm_build_trees(fmm_ctx* c) { return guard(c, [&] { stage_trees(c); }); }
+int fmm_leaf_expansions(fmm_ctx* c) { return guard(c, [&] { stage_leaves(c); }); }
+int fmm_upward(fmm_ctx* c) {
+    return guard(c, [&] {
+        if (c->stage == ST_TREES) stage_leaves(c);   // calclMultipoleExpansions runs P2M if needed (:104-107)
+        stage_upward(c);
+    });
+}
+int fmm_downward(fmm_ctx* c) { return guard(c, [&] { stage_downward(c); }); }
+
+int fmm_evaluate_B_dev(fmm_ctx* c, double current, double* B) {
+    return guard(c, [&] {
+        require(B != nullptr || c->nt == 0, FMM_ERR_ARG, "null output");
+        stage_evaluate(c, current, B);
+    });
+}
+int fmm_evaluate_B(fmm_ctx* c, double current, double* B) {
+    return guard(c, [&] {
+        require(B != nullptr || c->nt == 0, FMM_ERR_ARG, "null output");
+        double* d = c->Bout.get<double>(3 * (size_t)std::max(c->nt, 1));
+        stage_evaluate(c, current, d);
+        if (c->nt) FMM_HIP(hipMemcpyAsync(B, d, 3 * (size_t)c->nt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        FMM_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+int fmm_run_B_dev(fmm_ctx* c, double current, double* B) {
+    return guard(c, [&] {
+        stage_trees(c);
+        stage_leaves(c);
+        stage_upward(c);
+        stage_downward(c);
+        stage_evaluate(c, current, B);
+    });
+}
+
+int fmm_leaf_order(fmm_ctx* c, int64_t* order) {
+    return guard(c, [&] {
+        require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built");
+        std::vector<int> s(c->T.num_slots);
+        if (!s.empty())
+            FMM_HIP(hipMemcpyAsync(s.data(), c->T.slots.ptr, s.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        FMM_HIP(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < s.size(); i++) order[i] = s[i];
+    });
+}
+
+int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, double* dLdw) {
+    return guard(c, [&] {
+        require(c->stage >= ST_DOWN, FMM_ERR_STATE, "run the forward pass first");
+        require(dLdB && dLdw, FMM_ERR_ARG, "null adjoint arrays");
+        AdjointArgs a;
+        a.prec = c->prec;
+        a.p = c->p;
+        a.Lp = c->Lp;
+        a.S = &c->S;
+        a.T = &c->T;
+        a.sigS = c->sigS;
+        a.sigT = c->sigT;
+        a.src_leaves = c->src_leaves.as<int>();
+        a.n_src_leaves = (int)c->h_src_leaves.size();
+        a.tgt_leaves = c->tgt_leaves.as<int>();
+        a.n_tgt_leaves = (int)c->h_tgt_leaves.size();
+        a.sxyz = c->sxyz.as<double>();
+        a.sJ = c->sJ.as<double>();
+        a.txyz = c->txyz.as<double>();
+        a.lists = &c->lists;
+        a.near_field = c->cfg.near_field;
+        a.up_ops = ops_ptr(c, c->up_ops);
+        a.up_ops_lo = c->up_ops.lo.ptr;
+        a.down_ops = ops_ptr(c, c->down_ops);
+        a.down_ops_lo = c->down_ops.lo.ptr;
+        a.m2m_pass_src.resize(MAX_LEVELS);
+        for (int k = 0; k < MAX_LEVELS; k++) {
+            const Pass* pm = k < (int)c->m2m_pass.size() ? &c->m2m_pass[k] : nullptr;
+            const Pass* pl = k < (int)c->l2l_pass.size() ? &c->l2l_pass[k] : nullptr;
+            a.m2m_levels.push_back(pm ? PassView{pm->ntiles, pm->tiles.as<GemmTile>(), pm->src.as<int>(), pm->dst.as<int>()}
+                                      : PassView{});
+            a.l2l_levels.push_back(pl ? PassView{pl->ntiles, pl->tiles.as<GemmTile>(), pl->src.as<int>(), pl->dst.as<int>()}
+                                      : PassView{});
+        }
+        a.m2l = PassView{c->m2l_pass.ntiles, c->m2l_pass.tiles.as<GemmTile>(), c->lists.far_t.as<int>(),
+                         c->lists.far_s.as<int>()};
+        a.n_up_ops = c->up_ops.nops;
+        a.n_down_ops = c->down_ops.nops;
+        a.scale = current * REF_MU0 / (4.0 * REF_PI);
+        a.nt = c->nt;
+        a.ns = c->ns;
+        run_adjoint(c->adj, a, dLdB, dLdw, c->stream);
+    });
+}
+
+int fmm_direct_B_dev(fmm_ctx* c, double current, size_t m, const double* xyz, double* B) {
+    return guard(c, [&] {
+        require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built (sources are gathered with them)");
+        launch_direct(c->sxyz.as<double>(), c->sq.as<double>(), c->S.num_slots, xyz, (int)m,
+                      current * REF_MU0 / (4.0 * REF_PI), B, c->stream);
+        FMM_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int fmm_get_stats(fmm_ctx* c, fmm_stats* s) {
+    return guard(c, [&] {
+        require(s != nullptr, FMM_ERR_ARG, "null stats");
+        std::memset(s, 0, sizeof(*s));
+        FMM_HIP(hipStreamSynchronize(c->stream));
+        s->source_nodes = c->S.num_nodes;
+        s->target_nodes = c->T.num_nodes;
+        s->source_levels = c->S.num_levels;
+        s->target_levels = c->T.num_levels;
+        s->source_slots = c->S.num_slots;
+        s->target_slots = c->T.num_slots;
+        s->dropped_sources = c->S.dropped;
+        s->dropped_targets = c->T.dropped;
+        s->far_pairs = c->lists.num_far;
+        s->close_pairs = c->lists.num_close;
+        s->m2l_classes = c->lists.num_classes;
+        s->order = c->p;
+        s->padded_length = c->Lp;
+        s->bfs_steps = c->lists.bfs_steps;
+        if (c->stage >= ST_DOWN && c->near_inter < 0 && c->lists.num_close > 0) {
+            std::vector<int> ct(c->lists.num_close), cs(c->lists.num_close);
+            FMM_HIP(hipMemcpy(ct.data(), c->lists.close_t.ptr, ct.size() * sizeof(int), hipMemcpyDeviceToHost));
+            FMM_HIP(hipMemcpy(cs.data(), c->lists.close_s.ptr, cs.size() * sizeof(int), hipMemcpyDeviceToHost));
+            long long tot = 0;
+            for (size_t i = 0; i < ct.size(); i++)
+                tot += (long long)(c->T.h_end[ct[i]] - c->T.h_begin[ct[i]]) * (c->S.h_end[cs[i]] - c->S.h_begin[cs[i]]);
+            c->near_inter = tot;
+        }
+        s->near_interactions = c->near_inter < 0 ? 0 : c->near_inter;
+        auto el = [&](int a, int b) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.0;
+            return (double)ms;
+        };
+        if (c->stage >= ST_EVAL) {
+            s->t_tree = el(E_START, E_TREE);
+            s->t_p2m = el(E_TREE, E_P2M);
+            s->t_m2m = el(E_P2M, E_M2M);
+            s->t_lists = el(E_M2M, E_LISTS);
+            s->t_tables = el(E_LISTS, E_TABLES);
+            s->t_m2l = el(E_TABLES, E_M2L);
+            s->t_l2l = el(E_M2L, E_L2L);
+            s->t_l2p = el(E_L2L, E_L2P);
+            s->t_p2p = el(E_L2P, E_P2P);
+            s->t_total = el(E_START, E_END);
+        }
+    });
+}
+
+int fmm_tree_size(fmm_ctx* c, int which, int* nn, int* ns) {
+    return guard(c, [&] {
+        require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built");
+        const TreeDev& T = which ? c->T : c->S;
+        *nn = T.num_nodes;
+        *ns = T.num_slots;
+    });
+}
+
+int fmm_tree_export(fmm_ctx* c, int which, int* level, int* parent, int* first_child, int* begin, int* end, int* npts,
+                    double* center, int* slots) {
+    return guard(c, [&] {
+        require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built");
+        const TreeDev& T = which ? c->T : c->S;
+        int n = T.num_nodes;
+        std::memcpy(level, T.h_level.data(), n * sizeof(int));
+        std::memcpy(parent, T.h_parent.data(), n * sizeof(int));
+        std::memcpy(first_child, T.h_first_child.data(), n * sizeof(int));
+        std::memcpy(begin, T.h_begin.data(), n * sizeof(int));
+        std::memcpy(end, T.h_end.data(), n * sizeof(int));
+        std::memcpy(npts, T.h_npts.data(), n * sizeof(int));
+        std::memcpy(center, T.h_center.data(), 3 * (size_t)n * sizeof(double));
+        if (T.num_slots)
+            FMM_HIP(hipMemcpy(slots, T.slots.ptr, T.num_slots * sizeof(int), hipMemcpyDeviceToHost));
+    });
+}
+
+int fmm_lists_size(fmm_ctx* c, long long* nfar, long long* nclose, long long* ncls) {
+    return guard(c, [&] {
+        require(c->lists_ready, FMM_ERR_STATE, "lists are not built");
+        *nfar = c->lists.num_far;
+        *nclose = c->lists.num_close;
+        *ncls = c->lists.num_classes;
+    });
+}
+
+int fmm_lists_export(fmm_ctx* c, int* ft, int* fs, int* ct, int* cs) {
+    return guard(c, [&] {
+        require(c->lists_ready, FMM_ERR_STATE, "lists are not built");
+        FMM_HIP(hipStreamSynchronize(c->stream));
+        const Lists& L = c->lists;
+        if (L.num_far) {
+            FMM_HIP(hipMemcpy(ft, L.far_t.ptr, L.num_far * sizeof(int), hipMemcpyDeviceToHost));
+            FMM_HIP(hipMemcpy(fs, L.far_s.ptr, L.num_far * sizeof(int), hipMemcpyDeviceToHost));
+        }
+        if (L.num_close) {
+            FMM_HIP(hipMemcpy(ct, L.close_t.ptr, L.num_close * sizeof(int), hipMemcpyDeviceToHost));
+            FMM_HIP(hipMemcpy(cs, L.close_s.ptr, L.num_close * sizeof(int), hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int fmm_export_B_parts(fmm_ctx* c, double* Bfar, double* Bnear) {
+    return guard(c, [&] {
+        require(c->stage >= ST_EVAL, FMM_ERR_STATE, "no evaluation yet");
+        FMM_HIP(hipStreamSynchronize(c->stream));
+        size_t n = 3 * (size_t)c->T.num_slots;
+        double sc = c->current * REF_MU0 / (4.0 * REF_PI);
+        if (n) {
+            FMM_HIP(hipMemcpy(Bfar, c->Bfar.ptr, n * sizeof(double), hipMemcpyDeviceToHost));
+            FMM_HIP(hipMemcpy(Bnear, c->Bnear.ptr, n * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        for (size_t i = 0; i < n; i++) { Bfar[i] *= sc; Bnear[i] *= sc; }
+    });
+}
+
+int fmm_export_expansion(fmm_ctx* c, int which, int node, double* out) {
+    return guard(c, [&] {
+        require(which ? c->stage >= ST_DOWN : c->stage >= ST_LEAVES, FMM_ERR_STATE, "expansion not computed");
+        const TreeDev& T = which ? c->T : c->S;
+        require(node >= 0 && node < T.num_nodes, FMM_ERR_ARG, "node out of range");
+        FMM_HIP(hipStreamSynchronize(c->stream));
+        const DevBuf& E = which ? c->Lx : c->M;
+        std::vector<double> v(3 * (size_t)c->Lp);
+        if (c->f64()) {
+            FMM_HIP(hipMemcpy(v.data(), (const double*)E.ptr + (size_t)node * 3 * c->Lp, v.size() * sizeof(double),
+                              hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> f(v.size());
+            FMM_HIP(hipMemcpy(f.data(), (const float*)E.ptr + (size_t)node * 3 * c->Lp, f.size() * sizeof(float),
+                              hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < f.size(); i++) v[i] = f[i];
+        }
+        double sg = T.scale[T.h_level[node]];
+        for (int comp = 0; comp < 3; comp++)
+            for (int l = 0; l <= c->p; l++)
+                for (int m = -l; m <= l; m++) {
+                    int k = l * l + l + m;
+                    double f = which ? std::pow(sg, -(l + 1)) : std::pow(sg, l);
+                    out[comp * c->L + k] = v[comp * c->Lp + k] * f;
+                }
+    });
+}
+
+int fmm_make_operator(int kind, int order, const double* t, double sig_in, double sig_out, int m2l_full, double* out) {
+    if (kind < 0 || kind > 2 || order < 1 || order > MAX_ORDER || !t || !out) return FMM_ERR_ARG;
+    try {
+        int L = nlen(order), Lp = pad_to(L, 32);
+        OpSpec s;
+        for (int a = 0; a < 3; a++) s.t[a] = t[a];
+        s.sig_in = sig_in;
+        s.sig_out = sig_out;
+        s.kind = kind;
+        s.m2l_full = m2l_full;
+        DevBuf d, o;
+        OpSpec* ds = d.get<OpSpec>(1);
+        FMM_HIP(hipMemcpy(ds, &s, sizeof(s), hipMemcpyHostToDevice));
+        double* dout = o.get<double>((size_t)Lp * Lp);
+        make_operators(ds, 1, order, Lp, dout, 0);
+        std::vector<double> h((size_t)Lp * Lp);
+        FMM_HIP(hipMemcpy(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < L; i++)
+            for (int j = 0; j < L; j++) out[(size_t)i * L + j] = h[(size_t)i * Lp + j];
+        return FMM_OK;
+    } catch (const Error& e) {
+        return e.code;
+    } catch (...) {
+        return FMM_ERR_HIP;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,394 @@
+// Particle <-> expansion kernels: source charges, P2M, L2P (curl of the local
+// expansion from analytic gradients), near-field P2P, final assembly, and the FP64
+// direct Biot-Savart sum (math::calcBioSavartLaplace, integration.cpp:101-120).
+//
+// Harmonics use the Cartesian recurrences of the complex regular solid harmonics
+// Phi_l^m = r^l P_l^m(cos) e^{im phi}/(l+m)! (m >= 0); the reference's real basis is
+// R(l,0) = (-1)^l Phi_l^0, R(l,m) = (-1)^l sqrt2 Re Phi_l^m, R(l,-m) = (-1)^l sqrt2 Im Phi_l^m
+// (harmonics.cpp:20-56 with the sectoral term m = p included).  All recurrences run in
+// FP64 on the scaled vector u = (x - c) / sigma; only P2P runs in FP32 (with FP64
+// accumulation per target).
+#include "fmm_expand.hpp"
+
+namespace fmm {
+
+__device__ __forceinline__ int lmi(int l, int m) { return l * l + l + m; }
+__device__ __forceinline__ double sgnl(int l) { return (l & 1) ? -1.0 : 1.0; }
+constexpr double SQRT2 = 1.41421356237309504880;
+
+// ---------------------------------------------------------------- charges
+__global__ void k_charges(const int* __restrict__ slots, int n, const double* __restrict__ xyz,
+                          const double* __restrict__ w, const double* __restrict__ J, double* __restrict__ sxyz,
+                          double* __restrict__ sq, double* __restrict__ sJ) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int i = slots[k];
+    double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    double j0, j1, j2;
+    if (J) {
+        j0 = J[3 * i]; j1 = J[3 * i + 1]; j2 = J[3 * i + 2];
+    } else {
+        // current direction perp().normalized() (vector3.cu:154-167, 201-204)
+        double px = -y, py = x;
+        double l = sqrt(px * px + py * py);
+        j0 = l != 0 ? px / l : 0.0;
+        j1 = l != 0 ? py / l : 0.0;
+        j2 = 0.0;
+    }
+    double ww = w[i];
+    sxyz[3 * k] = x; sxyz[3 * k + 1] = y; sxyz[3 * k + 2] = z;
+    sq[3 * k] = ww * j0; sq[3 * k + 1] = ww * j1; sq[3 * k + 2] = ww * j2;
+    if (sJ) { sJ[3 * k] = j0; sJ[3 * k + 1] = j1; sJ[3 * k + 2] = j2; }
+}
+
+__global__ void k_gather_xyz(const int* __restrict__ slots, int n, const double* __restrict__ xyz,
+                             double* __restrict__ out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int i = slots[k];
+    out[3 * k] = xyz[3 * i]; out[3 * k + 1] = xyz[3 * i + 1]; out[3 * k + 2] = xyz[3 * i + 2];
+}
+
+// ---------------------------------------------------------------- P2M
+// One wave per useful source leaf.  Lane (m, g) owns column m of the harmonic table for
+// the points g, g+G, ...; per-group partial sums live in LDS and are reduced at the end.
+template <class T>
+__global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, const double* __restrict__ center,
+                                            const int* __restrict__ begin, const int* __restrict__ end,
+                                            const int* __restrict__ level, ScaleTable sig,
+                                            const double* __restrict__ sxyz, const double* __restrict__ sq, int p,
+                                            int Lp, T* __restrict__ M) {
+    extern __shared__ double acc[];   // [G][3][L]
+    const int leaf = leaves[blockIdx.x];
+    const int L = (p + 1) * (p + 1), P1 = p + 1, G = 64 / P1;
+    const int lane = threadIdx.x, m = lane % P1, g = lane / P1;
+    for (int e = lane; e < G * 3 * L; e += 64) acc[e] = 0.0;
+    __syncthreads();
+    const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
+    const double inv = 1.0 / sig.s[level[leaf]];
+    if (g < G) {
+        double* a = acc + (size_t)g * 3 * L;
+        for (int j = begin[leaf] + g; j < end[leaf]; j += G) {
+            double ux = (sxyz[3 * j] - cx) * inv, uy = (sxyz[3 * j + 1] - cy) * inv, uz = (sxyz[3 * j + 2] - cz) * inv;
+            double r2 = ux * ux + uy * uy + uz * uz;
+            double q0 = sq[3 * j], q1 = sq[3 * j + 1], q2 = sq[3 * j + 2];
+            double dr = 1.0, di = 0.0;   // Phi_m^m
+            for (int i = 1; i <= m; i++) {
+                double s = 0.5 / i;
+                double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
+                dr = nr; di = ni;
+            }
+            double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
+            for (int l = m; l <= p; l++) {
+                double vr, vi;
+                if (l == m) { vr = dr; vi = di; }
+                else {
+                    double f = 1.0 / (double)(l * l - m * m);
+                    vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
+                    vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
+                    p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
+                }
+                if (l == m) { p1r = vr; p1i = vi; p2r = 0.0; p2i = 0.0; }
+                double s = sgnl(l);
+                if (m == 0) {
+                    int k0 = lmi(l, 0);
+                    a[k0] += s * q0 * vr; a[L + k0] += s * q1 * vr; a[2 * L + k0] += s * q2 * vr;
+                } else {
+                    double sr = s * SQRT2 * vr, si = s * SQRT2 * vi;
+                    int kp = lmi(l, m), kn = lmi(l, -m);
+                    a[kp] += q0 * sr; a[L + kp] += q1 * sr; a[2 * L + kp] += q2 * sr;
+                    a[kn] += q0 * si; a[L + kn] += q1 * si; a[2 * L + kn] += q2 * si;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    T* out = M + (size_t)leaf * 3 * Lp;
+    for (int e = lane; e < 3 * Lp; e += 64) {
+        int c = e / Lp, k = e - c * Lp;
+        double s = 0.0;
+        if (k < L)
+            for (int gg = 0; gg < G; gg++) s += acc[(size_t)gg * 3 * L + c * L + k];
+        out[e] = (T)s;
+    }
+}
+
+// ---------------------------------------------------------------- L2P
+// One wave per useful target leaf.  The local expansion is turned into the complex
+// coefficients beta_c (degree <= p-1) of B = sigma^-2 Re sum beta_c(l,m) Phi_l^m(u), using
+// dz Phi_l^m = Phi_{l-1}^m, (dx - i dy) Phi_l^m = Phi_{l-1}^{m-1}, (dx + i dy) Phi_l^m = -Phi_{l-1}^{m+1}.
+__device__ __forceinline__ void alpha_of(const double* Ls, int L, int c, int l, int m, double& ar, double& ai) {
+    const double* Lc = Ls + c * L;
+    double s = sgnl(l);
+    if (m == 0) { ar = s * Lc[lmi(l, 0)]; ai = 0.0; }
+    else { ar = s * SQRT2 * Lc[lmi(l, m)]; ai = -s * SQRT2 * Lc[lmi(l, -m)]; }
+}
+// derivative coefficients of component c along axis a (0 x, 1 y, 2 z) at (l', m')
+__device__ __forceinline__ void deriv_coef(const double* Ls, int L, int c, int a, int lp, int mp, double& br,
+                                           double& bi) {
+    int l = lp + 1;
+    double r, i;
+    if (a == 2) { alpha_of(Ls, L, c, l, mp, br, bi); return; }
+    double sr = 0, si = 0;
+    alpha_of(Ls, L, c, l, mp + 1, r, i);
+    double ur = 0.5 * r, ui = 0.5 * i;            // 1/2 alpha(l, m'+1)
+    double vr = 0, vi = 0;                        // 1/2 alpha(l, m'-1) (+ the m' = 1 extra)
+    if (mp >= 1) {
+        alpha_of(Ls, L, c, l, mp - 1, r, i);
+        vr = 0.5 * r; vi = 0.5 * i;
+        if (mp == 1) { vr *= 2.0; vi *= 2.0; }
+    }
+    if (a == 0) { sr = ur - vr; si = ui - vi; }   // x: 1/2 a(m'+1) - 1/2 a(m'-1)
+    else {                                        // y: (i/2) (a(m'+1) + a(m'-1))
+        double wr = ur + vr, wi = ui + vi;
+        sr = -wi; si = wr;
+    }
+    br = sr; bi = si;
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, const double* __restrict__ center,
+                                            const int* __restrict__ begin, const int* __restrict__ end,
+                                            const int* __restrict__ level, ScaleTable sig,
+                                            const double* __restrict__ txyz, const T* __restrict__ Lexp, int p, int Lp,
+                                            double* __restrict__ Bfar) {
+    extern __shared__ double sh[];
+    const int L = (p + 1) * (p + 1);
+    const int NB = p * (p + 1) / 2;               // (l', m') pairs with l' <= p-1
+    double* Ls = sh;                               // [3][L]
+    double* beta = sh + 3 * L;                     // [3][NB][2]
+    const int leaf = leaves[blockIdx.x];
+    const int lane = threadIdx.x;
+    const T* Lsrc = Lexp + (size_t)leaf * 3 * Lp;
+    for (int e = lane; e < 3 * L; e += 64) {
+        int c = e / L, k = e - c * L;
+        Ls[e] = (double)Lsrc[c * Lp + k];
+    }
+    __syncthreads();
+    for (int e = lane; e < NB; e += 64) {
+        int lp = 0;
+        while ((lp + 1) * (lp + 2) / 2 <= e) lp++;
+        int mp = e - lp * (lp + 1) / 2;
+        double d[3][3][2];   // d[comp][axis]
+        for (int c = 0; c < 3; c++)
+            for (int a = 0; a < 3; a++) deriv_coef(Ls, L, c, a, lp, mp, d[c][a][0], d[c][a][1]);
+        // B = curl A
+        for (int q = 0; q < 2; q++) {
+            beta[(0 * NB + e) * 2 + q] = d[2][1][q] - d[1][2][q];
+            beta[(1 * NB + e) * 2 + q] = d[0][2][q] - d[2][0][q];
+            beta[(2 * NB + e) * 2 + q] = d[1][0][q] - d[0][1][q];
+        }
+    }
+    __syncthreads();
+    const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
+    const double inv = 1.0 / sig.s[level[leaf]];
+    for (int k = begin[leaf] + lane; k < end[leaf]; k += 64) {
+        double ux = (txyz[3 * k] - cx) * inv, uy = (txyz[3 * k + 1] - cy) * inv, uz = (txyz[3 * k + 2] - cz) * inv;
+        double r2 = ux * ux + uy * uy + uz * uz;
+        double b0 = 0, b1 = 0, b2 = 0;
+        double dr = 1.0, di = 0.0;
+        for (int m = 0; m < p; m++) {
+            if (m > 0) {
+                double s = 0.5 / m;
+                double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
+                dr = nr; di = ni;
+            }
+            double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
+            for (int l = m; l < p; l++) {
+                double vr, vi;
+                if (l == m) { vr = dr; vi = di; }
+                else {
+                    double f = 1.0 / (double)(l * l - m * m);
+                    vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
+                    vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
+                    p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
+                }
+                int e = l * (l + 1) / 2 + m;
+                b0 += beta[(0 * NB + e) * 2] * vr - beta[(0 * NB + e) * 2 + 1] * vi;
+                b1 += beta[(1 * NB + e) * 2] * vr - beta[(1 * NB + e) * 2 + 1] * vi;
+                b2 += beta[(2 * NB + e) * 2] * vr - beta[(2 * NB + e) * 2 + 1] * vi;
+            }
+        }
+        double s2 = inv * inv;
+        Bfar[3 * k] = b0 * s2;
+        Bfar[3 * k + 1] = b1 * s2;
+        Bfar[3 * k + 2] = b2 * s2;
+    }
+}
+
+// ---------------------------------------------------------------- P2P
+// One workgroup per useful target leaf; source blocks of 64 are staged in LDS as offsets
+// from the target-leaf centre (formed in FP64), one target per lane.  R = float: FP32
+// pair math with FP64 accumulation per target; R = double: FP64 throughout.
+template <class R> struct V4;
+template <> struct V4<float> { typedef float4 t; static __device__ float4 make(float a, float b, float c) { return make_float4(a, b, c, 0.f); } };
+template <> struct V4<double> { typedef double4 t; static __device__ double4 make(double a, double b, double c) { return make_double4(a, b, c, 0.0); } };
+__device__ __forceinline__ float rsqrt_t(float x) { return rsqrtf(x); }
+__device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
+
+template <class R>
+__global__ __launch_bounds__(64) void k_p2p(const int* __restrict__ leaves, const double* __restrict__ tcenter,
+                                            const int* __restrict__ tbegin, const int* __restrict__ tend,
+                                            const int* __restrict__ close_begin, const int* __restrict__ close_end,
+                                            const int* __restrict__ close_s, const int* __restrict__ sbegin,
+                                            const int* __restrict__ send, const double* __restrict__ sxyz,
+                                            const double* __restrict__ sq, const double* __restrict__ txyz,
+                                            double* __restrict__ Bnear) {
+    typedef typename V4<R>::t v4;
+    __shared__ v4 spos[64];
+    __shared__ v4 sqv[64];
+    const int leaf = leaves[blockIdx.x];
+    const int lane = threadIdx.x;
+    const double cx = tcenter[3 * leaf], cy = tcenter[3 * leaf + 1], cz = tcenter[3 * leaf + 2];
+    const int cb = close_begin[leaf], ce = close_end[leaf];
+    for (int k0 = tbegin[leaf]; k0 < tend[leaf]; k0 += 64) {
+        int k = k0 + lane;
+        bool valid = k < tend[leaf];
+        R xr = 0, yr = 0, zr = 0;
+        if (valid) {
+            xr = (R)(txyz[3 * k] - cx);
+            yr = (R)(txyz[3 * k + 1] - cy);
+            zr = (R)(txyz[3 * k + 2] - cz);
+        }
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (int ci = cb; ci < ce; ci++) {
+            int s = close_s[ci];
+            int sb = sbegin[s], se = send[s];
+            for (int j0 = sb; j0 < se; j0 += 64) {
+                int j = j0 + lane;
+                __syncthreads();
+                if (j < se) {
+                    spos[lane] = V4<R>::make((R)(sxyz[3 * j] - cx), (R)(sxyz[3 * j + 1] - cy), (R)(sxyz[3 * j + 2] - cz));
+                    sqv[lane] = V4<R>::make((R)sq[3 * j], (R)sq[3 * j + 1], (R)sq[3 * j + 2]);
+                }
+                __syncthreads();
+                int nj = min(64, se - j0);
+                R f0 = 0, f1 = 0, f2 = 0;
+                for (int jj = 0; jj < nj; jj++) {
+                    v4 y = spos[jj], q = sqv[jj];
+                    R dx = xr - y.x, dy = yr - y.y, dz = zr - y.z;
+                    R r2 = dx * dx + dy * dy + dz * dz;
+                    R ri = r2 > (R)0 ? rsqrt_t(r2) : (R)0;
+                    R ri3 = ri * ri * ri;
+                    f0 += (q.y * dz - q.z * dy) * ri3;
+                    f1 += (q.z * dx - q.x * dz) * ri3;
+                    f2 += (q.x * dy - q.y * dx) * ri3;
+                }
+                a0 += f0; a1 += f1; a2 += f2;
+            }
+        }
+        if (valid) {
+            Bnear[3 * k] = a0;
+            Bnear[3 * k + 1] = a1;
+            Bnear[3 * k + 2] = a2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- assembly
+__global__ void k_assemble(const int* __restrict__ slots, int n, const double* __restrict__ Bfar,
+                           const double* __restrict__ Bnear, double scale, double* __restrict__ B) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int i = slots[k];
+    for (int c = 0; c < 3; c++) B[3 * i + c] = scale * (Bfar[3 * k + c] + Bnear[3 * k + c]);
+}
+
+// ---------------------------------------------------------------- direct sum (FP64)
+__global__ __launch_bounds__(256) void k_direct(const double* __restrict__ sxyz, const double* __restrict__ sq, int ns,
+                                                const double* __restrict__ txyz, int nt, double scale,
+                                                double* __restrict__ B) {
+    __shared__ double3 sp[256];
+    __shared__ double3 sqq[256];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double x = 0, y = 0, z = 0;
+    if (t < nt) { x = txyz[3 * t]; y = txyz[3 * t + 1]; z = txyz[3 * t + 2]; }
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int j0 = 0; j0 < ns; j0 += 256) {
+        int j = j0 + threadIdx.x;
+        __syncthreads();
+        if (j < ns) {
+            sp[threadIdx.x] = make_double3(sxyz[3 * j], sxyz[3 * j + 1], sxyz[3 * j + 2]);
+            sqq[threadIdx.x] = make_double3(sq[3 * j], sq[3 * j + 1], sq[3 * j + 2]);
+        }
+        __syncthreads();
+        int nj = min(256, ns - j0);
+        for (int jj = 0; jj < nj; jj++) {
+            double dx = x - sp[jj].x, dy = y - sp[jj].y, dz = z - sp[jj].z;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0) continue;
+            double r = sqrt(r2);
+            double r3 = 1.0 / (r2 * r);
+            double3 q = sqq[jj];
+            a0 += (q.y * dz - q.z * dy) * r3;
+            a1 += (q.z * dx - q.x * dz) * r3;
+            a2 += (q.x * dy - q.y * dx) * r3;
+        }
+    }
+    if (t < nt) {
+        B[3 * t] = a0 * scale;
+        B[3 * t + 1] = a1 * scale;
+        B[3 * t + 2] = a2 * scale;
+    }
+}
+
+static inline int blocks(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+void launch_charges(const int* slots, int n, const double* xyz, const double* w, const double* J, double* sxyz,
+                    double* sq, double* sJ, hipStream_t st) {
+    if (n) k_charges<<<blocks(n), 256, 0, st>>>(slots, n, xyz, w, J, sxyz, sq, sJ);
+}
+void launch_gather_xyz(const int* slots, int n, const double* xyz, double* out, hipStream_t st) {
+    if (n) k_gather_xyz<<<blocks(n), 256, 0, st>>>(slots, n, xyz, out);
+}
+void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
+                const double* sq, int p, int Lp, void* M, hipStream_t st) {
+    if (!nleaves) return;
+    int G = 64 / (p + 1);
+    size_t lds = (size_t)G * 3 * (p + 1) * (p + 1) * sizeof(double);
+    if (f64)
+        k_p2m<double><<<nleaves, 64, lds, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
+                                                S.level.as<int>(), sig, sxyz, sq, p, Lp, (double*)M);
+    else
+        k_p2m<float><<<nleaves, 64, lds, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
+                                               S.level.as<int>(), sig, sxyz, sq, p, Lp, (float*)M);
+    FMM_HIP(hipGetLastError());
+}
+void launch_l2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig, const double* txyz,
+                const void* Lexp, int p, int Lp, double* Bfar, hipStream_t st) {
+    if (!nleaves) return;
+    int L = (p + 1) * (p + 1), NB = p * (p + 1) / 2;
+    size_t lds = (size_t)(3 * L + 6 * NB) * sizeof(double);
+    if (f64)
+        k_l2p<double><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
+                                                T.level.as<int>(), sig, txyz, (const double*)Lexp, p, Lp, Bfar);
+    else
+        k_l2p<float><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
+                                               T.level.as<int>(), sig, txyz, (const float*)Lexp, p, Lp, Bfar);
+    FMM_HIP(hipGetLastError());
+}
+void launch_p2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
+                const int* close_end, const int* close_s, const double* sxyz, const double* sq, const double* txyz,
+                double* Bnear, hipStream_t st) {
+    if (!nleaves) return;
+    if (f64)
+        k_p2p<double><<<nleaves, 64, 0, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
+                                              close_begin, close_end, close_s, S.begin.as<int>(), S.end.as<int>(), sxyz,
+                                              sq, txyz, Bnear);
+    else
+        k_p2p<float><<<nleaves, 64, 0, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
+                                             close_begin, close_end, close_s, S.begin.as<int>(), S.end.as<int>(), sxyz,
+                                             sq, txyz, Bnear);
+    FMM_HIP(hipGetLastError());
+}
+void launch_assemble(const int* slots, int n, const double* Bfar, const double* Bnear, double scale, double* B,
+                     hipStream_t st) {
+    if (n) k_assemble<<<blocks(n), 256, 0, st>>>(slots, n, Bfar, Bnear, scale, B);
+}
+void launch_direct(const double* sxyz, const double* sq, int ns, const double* txyz, int nt, double scale, double* B,
+                   hipStream_t st) {
+    if (nt) k_direct<<<blocks(nt), 256, 0, st>>>(sxyz, sq, ns, txyz, nt, scale, B);
+    FMM_HIP(hipGetLastError());
+}
+
+}  // namespace fmm

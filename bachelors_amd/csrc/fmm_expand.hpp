@@ -1,0 +1,21 @@
+#pragma once
+#include "fmm_common.hpp"
+
+namespace fmm {
+
+void launch_charges(const int* slots, int n, const double* xyz, const double* w, const double* J, double* sxyz,
+                    double* sq, double* sJ, hipStream_t st);
+void launch_gather_xyz(const int* slots, int n, const double* xyz, double* out, hipStream_t st);
+void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
+                const double* sq, int p, int Lp, void* M, hipStream_t st);
+void launch_l2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig, const double* txyz,
+                const void* Lexp, int p, int Lp, double* Bfar, hipStream_t st);
+void launch_p2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
+                const int* close_end, const int* close_s, const double* sxyz, const double* sq, const double* txyz,
+                double* Bnear, hipStream_t st);
+void launch_assemble(const int* slots, int n, const double* Bfar, const double* Bnear, double scale, double* B,
+                     hipStream_t st);
+void launch_direct(const double* sxyz, const double* sq, int ns, const double* txyz, int nt, double scale, double* B,
+                   hipStream_t st);
+
+}  // namespace fmm

@@ -1,0 +1,307 @@
+// Interaction lists on the GPU: the reference's dual-tree recursion
+// (FastMultipoleSolver::formInteractionMaps, fast_multipole_solver.cpp:105-165, MAC
+// checkIfFarEnough :167-184) evaluated breadth-first over a frontier of (source,
+// target) node pairs.  Each pair's fate is a pure function of the two nodes, so the far
+// and close sets equal the recursive ones; they are then sorted (and de-duplicated, as
+// the reference's std::set does).  The MAC is evaluated in double with the reference's
+// operation order and no fused multiply-add, so ties resolve identically.
+#include "fmm_lists.hpp"
+#include "fmm_prims.hpp"
+
+namespace fmm {
+
+struct TreeView {
+    const double* center;
+    const int* first_child;
+    const int* npts;
+    const int* level;
+};
+struct RadiusTable {
+    double r[MAX_LEVELS];
+};
+
+__device__ __forceinline__ bool nonempty(const TreeView& T, int n) { return T.first_child[n] >= 0 || T.npts[n] > 0; }
+__device__ __forceinline__ bool useful(const TreeView& T, int n) { return T.first_child[n] < 0 && T.npts[n] > 0; }
+
+__device__ __forceinline__ bool far_enough(const TreeView& S, const RadiusTable& RS, int s, const TreeView& T,
+                                           const RadiusTable& RT, int t, double mac_scale) {
+    double r1 = RS.r[S.level[s]], r2 = RT.r[T.level[t]];
+    double mn = fmin(r1, r2), mx = fmax(r1, r2);
+    double md = __dadd_rn(__dmul_rn(2.0, mx), mn);
+    md = __dmul_rn(md, mac_scale);
+    double dx = __dsub_rn(S.center[3 * s], T.center[3 * t]);
+    double dy = __dsub_rn(S.center[3 * s + 1], T.center[3 * t + 1]);
+    double dz = __dsub_rn(S.center[3 * s + 2], T.center[3 * t + 2]);
+    double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+    return __dmul_rn(md, md) < d2;
+}
+
+__device__ __forceinline__ void push(int2* buf, unsigned* cnt, unsigned cap, int a, int b) {
+    unsigned i = atomicAdd(cnt, 1u);
+    if (i < cap) buf[i] = make_int2(a, b);
+}
+
+// frontier entries are (s, t); far/close outputs are (t, s)
+__global__ void k_lists_step(const int2* __restrict__ front, int nf, TreeView S, RadiusTable RS, TreeView T,
+                             RadiusTable RT, double mac_scale, int2* far_out, unsigned* far_cnt, unsigned far_cap,
+                             int2* close_out, unsigned* close_cnt, unsigned close_cap, int2* next,
+                             unsigned* next_cnt, unsigned next_cap) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    int s = front[i].x, t = front[i].y;
+    if (far_enough(S, RS, s, T, RT, t, mac_scale)) {
+        push(far_out, far_cnt, far_cap, t, s);
+        return;
+    }
+    if (useful(T, t)) {
+        bool close_enough = true;
+        int fc = S.first_child[s];
+        if (fc >= 0)
+            for (int o = 0; o < 8; o++)
+                if (far_enough(S, RS, fc + o, T, RT, t, mac_scale)) { close_enough = false; break; }
+        if (close_enough) {
+            push(close_out, close_cnt, close_cap, t, s);
+        } else {
+            for (int o = 0; o < 8; o++)
+                if (nonempty(S, fc + o)) push(next, next_cnt, next_cap, fc + o, t);
+        }
+        return;
+    }
+    double rt = RT.r[T.level[t]], rs = RS.r[S.level[s]];
+    if ((rt < rs || useful(T, t)) && !useful(S, s)) {
+        int fc = S.first_child[s];
+        for (int o = 0; o < 8; o++)
+            if (nonempty(S, fc + o)) push(next, next_cnt, next_cap, fc + o, t);
+    } else {
+        int fc = T.first_child[t];
+        for (int o = 0; o < 8; o++)
+            if (nonempty(T, fc + o)) push(next, next_cnt, next_cap, s, fc + o);
+    }
+}
+
+// 64-bit sort key: high word = class code (or 0), low word = target; value = source
+__global__ void k_far_keys(const int2* __restrict__ far, int n, TreeView S, TreeView T, const double* __restrict__ hs,
+                           const double* __restrict__ ht, int use_classes, uint64_t* __restrict__ key,
+                           int* __restrict__ val, unsigned* __restrict__ err) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int t = far[i].x, s = far[i].y;
+    uint32_t code = 0;
+    if (use_classes) {
+        int ls = S.level[s], lt = T.level[t];
+        double hf = ls > lt ? hs[ls] : ht[lt];
+        int nx = (int)rint((S.center[3 * s] - T.center[3 * t]) / hf);
+        int ny = (int)rint((S.center[3 * s + 1] - T.center[3 * t + 1]) / hf);
+        int nz = (int)rint((S.center[3 * s + 2] - T.center[3 * t + 2]) / hf);
+        int d = ls - lt;
+        if (nx < -127 || nx > 127 || ny < -127 || ny > 127 || nz < -127 || nz > 127 || d < -31 || d > 31)
+            atomicOr(err, 1u);
+        code = ((uint32_t)(d + 32) << 24) | ((uint32_t)(nx + 128) << 16) | ((uint32_t)(ny + 128) << 8) |
+               (uint32_t)(nz + 128);
+    }
+    key[i] = ((uint64_t)code << 32) | (uint32_t)t;
+    val[i] = s;
+}
+
+__global__ void k_close_keys(const int2* __restrict__ close, int n, uint64_t* __restrict__ key, int* __restrict__ val) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = ((uint64_t)(uint32_t)close[i].x << 32) | (uint32_t)close[i].y;
+    val[i] = close[i].y;
+}
+
+// keep the first of equal (key, value) runs
+__global__ void k_unique_flags(const uint64_t* __restrict__ key, const int* __restrict__ val, int n,
+                               uint32_t* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || key[i] != key[i - 1] || val[i] != val[i - 1]) ? 1u : 0u;
+}
+
+// far keys carry the target in the low word (class code above), close keys in the high word
+__global__ void k_compact_pairs(const uint64_t* __restrict__ key, const int* __restrict__ val,
+                                const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int n,
+                                int target_high, int* __restrict__ out_t, int* __restrict__ out_s,
+                                uint32_t* __restrict__ out_code) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    uint32_t p = pos[i];
+    out_t[p] = target_high ? (int)(uint32_t)(key[i] >> 32) : (int)(uint32_t)(key[i] & 0xffffffffu);
+    out_s[p] = val[i];
+    if (out_code) out_code[p] = (uint32_t)(key[i] >> 32);
+}
+
+// segment boundaries: class starts (far) / target starts (close)
+__global__ void k_seg_flags(const uint32_t* __restrict__ code, int n, uint32_t* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || code[i] != code[i - 1]) ? 1u : 0u;
+}
+__global__ void k_seg_starts(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int n,
+                             int* __restrict__ starts) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    starts[pos[i]] = i;
+}
+__global__ void k_target_ranges(const int* __restrict__ ct, int n, int* __restrict__ cbeg, int* __restrict__ cend) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int t = ct[i];
+    if (i == 0 || ct[i - 1] != t) cbeg[t] = i;
+    if (i == n - 1 || ct[i + 1] != t) cend[t] = i + 1;
+}
+__global__ void k_fill_int(int* p, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static inline int blocks(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+static TreeView view(const TreeDev& T) {
+    return TreeView{T.center.as<double>(), T.first_child.as<int>(), T.npts.as<int>(), T.level.as<int>()};
+}
+static RadiusTable radii(const TreeDev& T) {
+    RadiusTable r;
+    for (int k = 0; k < MAX_LEVELS; k++) r.r[k] = T.radius[k];
+    return r;
+}
+
+static uint32_t read_u32(const void* p, hipStream_t st) {
+    uint32_t v;
+    FMM_HIP(hipMemcpyAsync(&v, p, 4, hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    return v;
+}
+
+// sort by 64-bit key, drop duplicate (key, value) pairs; returns the unique count
+static int sort_unique(ListScratch& L, int n, int end_bit, int target_high, int* out_t, int* out_s, uint32_t* out_code,
+                       hipStream_t st) {
+    if (n == 0) return 0;
+    uint64_t* k2 = L.key2.get<uint64_t>(n);
+    int* v2 = L.val2.get<int>(n);
+    sort_pairs_u64(L.key.as<uint64_t>(), k2, L.val.as<int>(), v2, n, end_bit, L.tmp, st);
+    uint32_t* flag = L.flag.get<uint32_t>(n);
+    uint32_t* pos = L.pos.get<uint32_t>(n);
+    k_unique_flags<<<blocks(n), 256, 0, st>>>(k2, v2, n, flag);
+    exclusive_scan_u32(flag, pos, n, L.tmp, st);
+    int nu = (int)(read_u32(pos + n - 1, st) + read_u32(flag + n - 1, st));
+    k_compact_pairs<<<blocks(n), 256, 0, st>>>(k2, v2, flag, pos, n, target_high, out_t, out_s, out_code);
+    return nu;
+}
+
+void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T, double mac_scale, bool classes,
+                 hipStream_t st) {
+    TreeView sv = view(S), tv = view(T);
+    RadiusTable rs = radii(S), rt = radii(T);
+    unsigned* cnt = L.counters.get<unsigned>(4);
+    unsigned far_cap = std::max<unsigned>(L.far_cap, 1u << 16), close_cap = std::max<unsigned>(L.close_cap, 1u << 16),
+             front_cap = std::max<unsigned>(L.front_cap, 1u << 16);
+    for (int attempt = 0; attempt < 8; attempt++) {
+        int2* far = L.far.get<int2>(far_cap);
+        int2* close = L.close.get<int2>(close_cap);
+        int2* fa = L.frontA.get<int2>(front_cap);
+        int2* fb = L.frontB.get<int2>(front_cap);
+        FMM_HIP(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned), st));
+        int2 root = make_int2(0, 0);
+        int nf = 0;
+        // the reference returns at once if either root is empty (fast_multipole_solver.cpp:206-210)
+        bool s_ok = !S.h_first_child.empty() && (S.h_first_child[0] >= 0 || S.h_npts[0] > 0);
+        bool t_ok = !T.h_first_child.empty() && (T.h_first_child[0] >= 0 || T.h_npts[0] > 0);
+        if (s_ok && t_ok) {
+            FMM_HIP(hipMemcpyAsync(fa, &root, sizeof(int2), hipMemcpyHostToDevice, st));
+            nf = 1;
+        }
+        bool overflow = false;
+        int steps = 0;
+        while (nf > 0) {
+            FMM_HIP(hipMemsetAsync(cnt + 2, 0, sizeof(unsigned), st));
+            k_lists_step<<<blocks(nf), 256, 0, st>>>(fa, nf, sv, rs, tv, rt, mac_scale, far, cnt + 0, far_cap, close,
+                                                     cnt + 1, close_cap, fb, cnt + 2, front_cap);
+            unsigned h[3];
+            FMM_HIP(hipMemcpyAsync(h, cnt, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+            FMM_HIP(hipStreamSynchronize(st));
+            steps++;
+            if (h[0] > far_cap || h[1] > close_cap || h[2] > front_cap) {
+                overflow = true;
+                far_cap = std::max(far_cap, 2 * h[0]);
+                close_cap = std::max(close_cap, 2 * h[1]);
+                front_cap = std::max(front_cap, 2 * h[2]);
+                break;
+            }
+            nf = (int)h[2];
+            std::swap(fa, fb);
+        }
+        if (overflow) continue;
+        L.far_cap = far_cap;
+        L.close_cap = close_cap;
+        L.front_cap = front_cap;
+        unsigned h[2];
+        FMM_HIP(hipMemcpyAsync(h, cnt, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipStreamSynchronize(st));
+        int nfar = (int)h[0], nclose = (int)h[1];
+        out.bfs_steps = steps;
+        // ---- far pairs: sort by (class, target), de-duplicate, segment into classes
+        {
+            uint64_t* key = L.key.get<uint64_t>(std::max(nfar, 1));
+            int* val = L.val.get<int>(std::max(nfar, 1));
+            unsigned* err = cnt + 3;
+            FMM_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), st));
+            DevBuf& hsb = L.hs;
+            double hsv[MAX_LEVELS], htv[MAX_LEVELS];
+            for (int k = 0; k < MAX_LEVELS; k++) { hsv[k] = S.half[k][0]; htv[k] = T.half[k][0]; }
+            double* dh = hsb.get<double>(2 * MAX_LEVELS);
+            FMM_HIP(hipMemcpyAsync(dh, hsv, sizeof(hsv), hipMemcpyHostToDevice, st));
+            FMM_HIP(hipMemcpyAsync(dh + MAX_LEVELS, htv, sizeof(htv), hipMemcpyHostToDevice, st));
+            if (nfar > 0)
+                k_far_keys<<<blocks(nfar), 256, 0, st>>>(far, nfar, sv, tv, dh, dh + MAX_LEVELS, classes ? 1 : 0, key,
+                                                         val, err);
+            if (classes && read_u32(err, st)) throw Error(4, "interaction offset out of the class-key range");
+            int* ft = out.far_t.get<int>(std::max(nfar, 1));
+            int* fs = out.far_s.get<int>(std::max(nfar, 1));
+            uint32_t* code = out.far_code.get<uint32_t>(std::max(nfar, 1));
+            int nu = sort_unique(L, nfar, classes ? 62 : 32, 0, ft, fs, code, st);
+            out.num_far = nu;
+            // classes
+            out.num_classes = 0;
+            out.h_class_begin.clear();
+            if (nu > 0) {
+                if (classes) {
+                    uint32_t* flag = L.flag.get<uint32_t>(nu);
+                    uint32_t* pos = L.pos.get<uint32_t>(nu);
+                    k_seg_flags<<<blocks(nu), 256, 0, st>>>(code, nu, flag);
+                    exclusive_scan_u32(flag, pos, nu, L.tmp, st);
+                    int nc = (int)(read_u32(pos + nu - 1, st) + read_u32(flag + nu - 1, st));
+                    int* starts = out.class_begin.get<int>(nc + 1);
+                    k_seg_starts<<<blocks(nu), 256, 0, st>>>(flag, pos, nu, starts);
+                    out.h_class_begin.resize(nc + 1);
+                    FMM_HIP(hipMemcpyAsync(out.h_class_begin.data(), starts, nc * sizeof(int), hipMemcpyDeviceToHost, st));
+                    FMM_HIP(hipStreamSynchronize(st));
+                    out.h_class_begin[nc] = nu;
+                    out.num_classes = nc;
+                } else {
+                    out.num_classes = 0;
+                }
+            }
+        }
+        // ---- close pairs: sort by (target, source), de-duplicate, per-target ranges
+        {
+            uint64_t* key = L.key.get<uint64_t>(std::max(nclose, 1));
+            int* val = L.val.get<int>(std::max(nclose, 1));
+            if (nclose > 0) k_close_keys<<<blocks(nclose), 256, 0, st>>>(close, nclose, key, val);
+            int* ct = out.close_t.get<int>(std::max(nclose, 1));
+            int* cs = out.close_s.get<int>(std::max(nclose, 1));
+            int nu = sort_unique(L, nclose, 64, 1, ct, cs, nullptr, st);
+            out.num_close = nu;
+            int* cb = out.close_begin.get<int>(std::max(T.num_nodes, 1));
+            int* ce = out.close_end.get<int>(std::max(T.num_nodes, 1));
+            k_fill_int<<<blocks(T.num_nodes), 256, 0, st>>>(cb, T.num_nodes, 0);
+            k_fill_int<<<blocks(T.num_nodes), 256, 0, st>>>(ce, T.num_nodes, 0);
+            if (nu > 0) k_target_ranges<<<blocks(nu), 256, 0, st>>>(ct, nu, cb, ce);
+        }
+        FMM_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    throw Error(4, "interaction list buffers did not converge");
+}
+
+}  // namespace fmm

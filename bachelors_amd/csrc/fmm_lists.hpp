@@ -1,0 +1,26 @@
+#pragma once
+#include "fmm_common.hpp"
+
+namespace fmm {
+
+struct ListScratch {
+    DevBuf far, close, frontA, frontB, counters, key, val, key2, val2, flag, pos, tmp, hs;
+    unsigned far_cap = 0, close_cap = 0, front_cap = 0;
+};
+
+// Far pairs sorted by (class, target) -- class = (level difference, centre offset in
+// units of the finer half size) in the shared frame -- and close pairs sorted by
+// (target, source) with per-target ranges.
+struct Lists {
+    int num_far = 0, num_close = 0, num_classes = 0, bfs_steps = 0;
+    DevBuf far_t, far_s, far_code;          // int, int, uint32 class code
+    DevBuf class_begin;                     // int[num_classes]
+    std::vector<int> h_class_begin;         // num_classes + 1
+    DevBuf close_t, close_s;                // int
+    DevBuf close_begin, close_end;          // per target node
+};
+
+void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T, double mac_scale, bool classes,
+                 hipStream_t st);
+
+}  // namespace fmm

@@ -1,0 +1,26 @@
+#pragma once
+#include "fmm_common.hpp"
+
+namespace fmm {
+
+enum OpKind { OPK_M2M = 0, OPK_M2L = 1, OPK_L2L = 2 };
+
+// One translation operator: t = translation vector of the reference translator
+// (M2M: child - parent, M2L: source - target, L2L: parent - child), sig_in / sig_out =
+// expansion scales of the input / output nodes.
+struct OpSpec {
+    double t[3];
+    double sig_in, sig_out;
+    int kind;
+    int m2l_full;   // M2L keeps l + lambda <= 2p (else the reference's l + lambda <= p)
+};
+
+// d_out: nops x Lp x Lp doubles, row-major [out][in], zero padded.
+void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, double* d_out, hipStream_t st);
+void operators_to_f32(const double* in, float* out, size_t n, hipStream_t st);
+void operators_to_bf16x2(const double* in, __bf16* hi, __bf16* lo, size_t n, hipStream_t st);
+void make_m2l_specs(const int* class_begin, int nc, const int* far_t, const int* far_s, const double* sc,
+                    const double* tc, const int* slev, const int* tlev, const ScaleTable& sigS, const ScaleTable& sigT,
+                    int m2l_full, OpSpec* out, hipStream_t st);
+
+}  // namespace fmm

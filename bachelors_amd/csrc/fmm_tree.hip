@@ -1,0 +1,362 @@
+// Octree build on the GPU, level by level, with the reference's exact rules:
+//  - half-open boxes [c-h, c+h) compared in double (box.cpp:13-24);
+//  - a node subdivides when its arrivals reach `thr` (source tree thr = cap,
+//    quadrature_octree.cpp:38; target tree thr = cap + 1, calculation_point_octree.cpp:41);
+//  - 8 children in reference order with centres rounded through float
+//    (quadrature_octree.cpp:74-97) -- the child boxes of a non-dyadic root can leave
+//    one-ulp gaps (points dropped) or overlaps (points inserted twice); both are kept;
+//  - inside a leaf the points keep insertion (input) order.
+// A point's path is computed from the actual float-rounded child boxes, so the tree is
+// identical to the one the reference's pointer insertion builds, in canonical BFS node
+// numbering, with the leaf contents laid out in DFS order ("slots").
+#include <cmath>
+
+#include "fmm_prims.hpp"
+#include "fmm_tree.hpp"
+
+namespace fmm {
+
+static const int CHILD_SIGN[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                     {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+__constant__ int c_child_sign[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                       {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+
+__device__ __forceinline__ bool in_range(double v, double c, double h) {
+    return __dsub_rn(c, h) <= v && v < __dadd_rn(c, h);
+}
+
+__global__ void k_root_filter(const double* __restrict__ P, int n, double3 c, double3 h, uint32_t* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = P[3 * i], y = P[3 * i + 1], z = P[3 * i + 2];
+    flag[i] = (in_range(x, c.x, h.x) && in_range(y, c.y, h.y) && in_range(z, c.z, h.z)) ? 1u : 0u;
+}
+
+__global__ void k_compact_root(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int n,
+                               int* __restrict__ idx, int* __restrict__ node) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    idx[pos[i]] = i;
+    node[pos[i]] = 0;
+}
+
+// subdivide flags of the level's nodes
+__global__ void k_subdiv_flags(const int* __restrict__ arrivals, int lb, int nnodes, int thr, int allow,
+                               uint32_t* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    flag[i] = (allow && arrivals[lb + i] >= thr) ? 1u : 0u;
+}
+
+// create the 8 children of every subdivided node (centres through float, quadrature_octree.cpp:76-96)
+__global__ void k_make_children(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank, int lb,
+                                int nnodes, int child_lb, int child_level, double3 ph,
+                                double* __restrict__ center, int* __restrict__ first_child,
+                                int* __restrict__ parent, int* __restrict__ level, int* __restrict__ arrivals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    int node = lb + i;
+    if (!flag[i]) { first_child[node] = -1; return; }
+    int fc = child_lb + 8 * (int)rank[i];
+    first_child[node] = fc;
+    float x = (float)center[3 * node], y = (float)center[3 * node + 1], z = (float)center[3 * node + 2];
+    float w = (float)ph.x, h = (float)ph.y, d = (float)ph.z;
+    float w2 = w / 2.0f, h2 = h / 2.0f, d2 = d / 2.0f;
+    for (int o = 0; o < 8; o++) {
+        int c = fc + o;
+        float cx = c_child_sign[o][0] < 0 ? __fsub_rn(x, w2) : __fadd_rn(x, w2);
+        float cy = c_child_sign[o][1] < 0 ? __fsub_rn(y, h2) : __fadd_rn(y, h2);
+        float cz = c_child_sign[o][2] < 0 ? __fsub_rn(z, d2) : __fadd_rn(z, d2);
+        center[3 * c] = cx;
+        center[3 * c + 1] = cy;
+        center[3 * c + 2] = cz;
+        parent[c] = node;
+        level[c] = child_level;
+        arrivals[c] = 0;
+        first_child[c] = -1;
+    }
+}
+
+// membership of every slot of a subdividing node in its 8 children (usually exactly one)
+__global__ void k_child_mask(const double* __restrict__ P, const int* __restrict__ idx, const int* __restrict__ node,
+                             int n, const int* __restrict__ first_child, const double* __restrict__ center,
+                             double3 ch, uint32_t* __restrict__ count, uint8_t* __restrict__ mask) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int fc = first_child[node[i]];
+    uint32_t m = 0;
+    if (fc >= 0) {
+        int q = idx[i];
+        double x = P[3 * q], y = P[3 * q + 1], z = P[3 * q + 2];
+        for (int o = 0; o < 8; o++) {
+            const double* c = center + 3 * (fc + o);
+            if (in_range(x, c[0], ch.x) && in_range(y, c[1], ch.y) && in_range(z, c[2], ch.z)) m |= 1u << o;
+        }
+    }
+    mask[i] = (uint8_t)m;
+    count[i] = __popc(m);
+}
+
+__global__ void k_emit_children(const int* __restrict__ idx, const int* __restrict__ node, int n,
+                                const int* __restrict__ first_child, const uint8_t* __restrict__ mask,
+                                const uint32_t* __restrict__ pos, int child_lb, uint32_t* __restrict__ key,
+                                int* __restrict__ val) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t m = mask[i];
+    if (!m) return;
+    int fc = first_child[node[i]] - child_lb;
+    uint32_t p = pos[i];
+    for (int o = 0; o < 8; o++)
+        if (m & (1u << o)) {
+            key[p] = (uint32_t)(fc + o);
+            val[p] = idx[i];
+            p++;
+        }
+}
+
+__global__ void k_key_to_node(const uint32_t* __restrict__ key, int n, int child_lb, int* __restrict__ node,
+                              int* __restrict__ arrivals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = child_lb + (int)key[i];
+    node[i] = c;
+    atomicAdd(&arrivals[c], 1);
+}
+
+// bottom-up subtree totals (leaf: arrivals)
+__global__ void k_totals(const int* __restrict__ first_child, const int* __restrict__ arrivals, int lb, int nnodes,
+                         int* __restrict__ total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    int node = lb + i;
+    int fc = first_child[node];
+    if (fc < 0) { total[node] = arrivals[node]; return; }
+    int s = 0;
+    for (int o = 0; o < 8; o++) s += total[fc + o];
+    total[node] = s;
+}
+
+// top-down DFS ranges
+__global__ void k_ranges(const int* __restrict__ first_child, const int* __restrict__ total, int lb, int nnodes,
+                         int* __restrict__ begin, int* __restrict__ end) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    int node = lb + i;
+    int b = begin[node];
+    end[node] = b + total[node];
+    int fc = first_child[node];
+    if (fc < 0) return;
+    for (int o = 0; o < 8; o++) {
+        begin[fc + o] = b;
+        b += total[fc + o];
+    }
+}
+
+// leaf slots of one level go to their DFS position
+__global__ void k_place_leaves(const int* __restrict__ idx, const int* __restrict__ node, int n,
+                               const int* __restrict__ first_child, const int* __restrict__ begin,
+                               const uint32_t* __restrict__ seg_start, int lb, int* __restrict__ slots) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int nd = node[i];
+    if (first_child[nd] >= 0) return;
+    slots[begin[nd] + (i - (int)seg_start[nd - lb])] = idx[i];
+}
+
+__global__ void k_leaf_npts(const int* __restrict__ first_child, const int* __restrict__ arrivals, int n,
+                            int* __restrict__ npts) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    npts[i] = first_child[i] < 0 ? arrivals[i] : 0;
+}
+
+static void grow_keep(DevBuf& b, size_t need, size_t used, hipStream_t st) {
+    if (need <= b.bytes && b.ptr) return;
+    size_t nb = std::max(need, b.bytes * 2);
+    void* np = nullptr;
+    FMM_HIP(hipMalloc(&np, nb));
+    if (b.ptr && used) FMM_HIP(hipMemcpyAsync(np, b.ptr, used, hipMemcpyDeviceToDevice, st));
+    if (b.ptr) {
+        FMM_HIP(hipStreamSynchronize(st));
+        FMM_HIP(hipFree(b.ptr));
+    }
+    b.ptr = np;
+    b.bytes = nb;
+}
+
+static inline int blocks(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const double root_c[3],
+                const double root_h[3], int thr, int max_depth, bool cubic_scale, hipStream_t st) {
+    if (thr < 2) throw Error(1, "leaf capacity too small: a node would subdivide forever");
+    if (max_depth < 0 || max_depth >= MAX_LEVELS) max_depth = MAX_LEVELS - 1;
+    // per-level half sizes: child half = (float)h / 2 (quadrature_octree.cpp:83-89)
+    for (int a = 0; a < 3; a++) T.half[0][a] = root_h[a];
+    for (int k = 1; k < MAX_LEVELS; k++)
+        for (int a = 0; a < 3; a++) T.half[k][a] = (double)((float)T.half[k - 1][a] / 2.0f);
+    for (int k = 0; k < MAX_LEVELS; k++) {
+        const double* h = T.half[k];
+        T.radius[k] = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+        T.scale[k] = cubic_scale ? h[0] : T.radius[k] / std::sqrt(3.0);
+    }
+    // root
+    size_t node_cap = 1024;
+    grow_keep(T.center, node_cap * 3 * sizeof(double), 0, st);
+    grow_keep(T.first_child, node_cap * sizeof(int), 0, st);
+    grow_keep(T.parent, node_cap * sizeof(int), 0, st);
+    grow_keep(T.level, node_cap * sizeof(int), 0, st);
+    grow_keep(T.arrivals, node_cap * sizeof(int), 0, st);
+    uint32_t* flag = S.flag.get<uint32_t>(std::max(n, 1));
+    uint32_t* pos = S.pos.get<uint32_t>(std::max(n, 1) + 1);
+    if (S.lvl_idx.size() < (size_t)MAX_LEVELS) {
+        S.lvl_idx.resize(MAX_LEVELS);
+        S.lvl_node.resize(MAX_LEVELS);
+    }
+    S.lvl_n.assign(MAX_LEVELS, 0);
+    double3 rc = make_double3(root_c[0], root_c[1], root_c[2]);
+    double3 rh = make_double3(root_h[0], root_h[1], root_h[2]);
+    int n0 = 0;
+    if (n > 0) {
+        k_root_filter<<<blocks(n), 256, 0, st>>>(d_xyz, n, rc, rh, flag);
+        exclusive_scan_u32(flag, pos, n, S.tmp, st);
+        uint32_t last_pos, last_flag;
+        FMM_HIP(hipMemcpyAsync(&last_pos, pos + n - 1, 4, hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipMemcpyAsync(&last_flag, flag + n - 1, 4, hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipStreamSynchronize(st));
+        n0 = (int)(last_pos + last_flag);
+    }
+    T.dropped = n - n0;
+    int* idx0 = (int*)S.lvl_idx[0].reserve(std::max(n0, 1) * sizeof(int));
+    int* nd0 = (int*)S.lvl_node[0].reserve(std::max(n0, 1) * sizeof(int));
+    if (n0 > 0) k_compact_root<<<blocks(n), 256, 0, st>>>(flag, pos, n, idx0, nd0);
+    S.lvl_n[0] = n0;
+    {
+        double c3[3] = {root_c[0], root_c[1], root_c[2]};
+        int zero = 0, m1 = -1;
+        FMM_HIP(hipMemcpyAsync(T.center.ptr, c3, 3 * sizeof(double), hipMemcpyHostToDevice, st));
+        FMM_HIP(hipMemcpyAsync(T.parent.ptr, &m1, sizeof(int), hipMemcpyHostToDevice, st));
+        FMM_HIP(hipMemcpyAsync(T.level.ptr, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+        FMM_HIP(hipMemcpyAsync(T.first_child.ptr, &m1, sizeof(int), hipMemcpyHostToDevice, st));
+        FMM_HIP(hipMemcpyAsync(T.arrivals.ptr, &n0, sizeof(int), hipMemcpyHostToDevice, st));
+        FMM_HIP(hipStreamSynchronize(st));
+    }
+    T.level_begin[0] = 0;
+    T.level_begin[1] = 1;
+    int num_nodes = 1;
+    int k = 0;
+    for (;; k++) {
+        int lb = T.level_begin[k], le = T.level_begin[k + 1];
+        int nn = le - lb;
+        uint32_t* sflag = S.sflag.get<uint32_t>(nn + 1);
+        uint32_t* srank = S.srank.get<uint32_t>(nn + 1);
+        int allow = (k < max_depth) ? 1 : 0;
+        k_subdiv_flags<<<blocks(nn), 256, 0, st>>>(T.arrivals.as<int>(), lb, nn, thr, allow, sflag);
+        exclusive_scan_u32(sflag, srank, nn, S.tmp, st);
+        uint32_t lr, lf;
+        FMM_HIP(hipMemcpyAsync(&lr, srank + nn - 1, 4, hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipMemcpyAsync(&lf, sflag + nn - 1, 4, hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipStreamSynchronize(st));
+        int nsub = (int)(lr + lf);
+        int child_lb = le;
+        int new_nodes = num_nodes + 8 * nsub;
+        size_t used = (size_t)num_nodes;
+        grow_keep(T.center, (size_t)new_nodes * 3 * sizeof(double), used * 3 * sizeof(double), st);
+        grow_keep(T.first_child, (size_t)new_nodes * sizeof(int), used * sizeof(int), st);
+        grow_keep(T.parent, (size_t)new_nodes * sizeof(int), used * sizeof(int), st);
+        grow_keep(T.level, (size_t)new_nodes * sizeof(int), used * sizeof(int), st);
+        grow_keep(T.arrivals, (size_t)new_nodes * sizeof(int), used * sizeof(int), st);
+        double3 ph = make_double3(T.half[k][0], T.half[k][1], T.half[k][2]);
+        k_make_children<<<blocks(nn), 256, 0, st>>>(sflag, srank, lb, nn, child_lb, k + 1, ph,
+                                                    T.center.as<double>(), T.first_child.as<int>(),
+                                                    T.parent.as<int>(), T.level.as<int>(), T.arrivals.as<int>());
+        num_nodes = new_nodes;
+        T.level_begin[k + 2] = new_nodes;
+        if (nsub == 0) break;
+        // partition the level's slots into the children
+        int nk = S.lvl_n[k];
+        uint32_t* cnt = S.flag.get<uint32_t>(std::max(nk, 1) + 1);
+        uint32_t* off = S.pos.get<uint32_t>(std::max(nk, 1) + 1);
+        uint8_t* mask = S.mask.get<uint8_t>(std::max(nk, 1));
+        double3 chh = make_double3(T.half[k + 1][0], T.half[k + 1][1], T.half[k + 1][2]);
+        const int* idxk = (const int*)S.lvl_idx[k].ptr;
+        const int* ndk = (const int*)S.lvl_node[k].ptr;
+        k_child_mask<<<blocks(nk), 256, 0, st>>>(d_xyz, idxk, ndk, nk, T.first_child.as<int>(),
+                                                 T.center.as<double>(), chh, cnt, mask);
+        exclusive_scan_u32(cnt, off, nk, S.tmp, st);
+        uint32_t lo, lc;
+        FMM_HIP(hipMemcpyAsync(&lo, off + nk - 1, 4, hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipMemcpyAsync(&lc, cnt + nk - 1, 4, hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipStreamSynchronize(st));
+        int nk1 = (int)(lo + lc);
+        uint32_t* key = S.key.get<uint32_t>(std::max(nk1, 1));
+        int* val = S.val.get<int>(std::max(nk1, 1));
+        uint32_t* key2 = S.key2.get<uint32_t>(std::max(nk1, 1));
+        int* idx1 = (int*)S.lvl_idx[k + 1].reserve(std::max(nk1, 1) * sizeof(int));
+        int* nd1 = (int*)S.lvl_node[k + 1].reserve(std::max(nk1, 1) * sizeof(int));
+        k_emit_children<<<blocks(nk), 256, 0, st>>>(idxk, ndk, nk, T.first_child.as<int>(), mask, off, child_lb, key,
+                                                    val);
+        int nchild = 8 * nsub;
+        int bits = 1;
+        while ((1 << bits) < nchild) bits++;
+        sort_pairs_u32(key, key2, val, idx1, nk1, bits, S.tmp, st);
+        k_key_to_node<<<blocks(nk1), 256, 0, st>>>(key2, nk1, child_lb, nd1, T.arrivals.as<int>());
+        S.lvl_n[k + 1] = nk1;
+    }
+    T.num_levels = k + 1;   // levels 0..k hold nodes
+    T.num_nodes = num_nodes;
+    // totals bottom-up, ranges top-down
+    T.npts.reserve((size_t)num_nodes * sizeof(int));
+    T.begin.reserve((size_t)num_nodes * sizeof(int));
+    T.end.reserve((size_t)num_nodes * sizeof(int));
+    int* total = S.total.get<int>(num_nodes);
+    for (int lv = T.num_levels; lv >= 0; lv--) {
+        int lb = T.level_begin[lv], nn = T.level_begin[lv + 1] - lb;
+        if (nn > 0) k_totals<<<blocks(nn), 256, 0, st>>>(T.first_child.as<int>(), T.arrivals.as<int>(), lb, nn, total);
+    }
+    {
+        int zero = 0;
+        FMM_HIP(hipMemcpyAsync(T.begin.ptr, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    for (int lv = 0; lv <= T.num_levels; lv++) {
+        int lb = T.level_begin[lv], nn = T.level_begin[lv + 1] - lb;
+        if (nn > 0) k_ranges<<<blocks(nn), 256, 0, st>>>(T.first_child.as<int>(), total, lb, nn, T.begin.as<int>(),
+                                                       T.end.as<int>());
+    }
+    int nslots = 0;
+    FMM_HIP(hipMemcpyAsync(&nslots, total, sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    T.num_slots = nslots;
+    int* slots = T.slots.get<int>(std::max(nslots, 1));
+    for (int lv = 0; lv <= k; lv++) {
+        int nk = S.lvl_n[lv];
+        if (nk == 0) continue;
+        int lb = T.level_begin[lv], nn = T.level_begin[lv + 1] - lb;
+        // segment start of each level node inside the level arrays = prefix of arrivals
+        uint32_t* seg = S.srank.get<uint32_t>(nn + 1);
+        exclusive_scan_u32((const uint32_t*)(T.arrivals.as<int>() + lb), seg, nn, S.tmp, st);
+        k_place_leaves<<<blocks(nk), 256, 0, st>>>((const int*)S.lvl_idx[lv].ptr, (const int*)S.lvl_node[lv].ptr, nk,
+                                                  T.first_child.as<int>(), T.begin.as<int>(), seg, lb, slots);
+    }
+    k_leaf_npts<<<blocks(num_nodes), 256, 0, st>>>(T.first_child.as<int>(), T.arrivals.as<int>(), num_nodes,
+                                                   T.npts.as<int>());
+    // host mirrors of the node tables (small; used to plan later stages)
+    T.h_first_child.resize(num_nodes);
+    T.h_npts.resize(num_nodes);
+    T.h_begin.resize(num_nodes);
+    T.h_end.resize(num_nodes);
+    T.h_level.resize(num_nodes);
+    T.h_parent.resize(num_nodes);
+    T.h_center.resize(3 * (size_t)num_nodes);
+    FMM_HIP(hipMemcpyAsync(T.h_first_child.data(), T.first_child.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_npts.data(), T.npts.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_begin.data(), T.begin.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_end.data(), T.end.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_level.data(), T.level.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_parent.data(), T.parent.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_center.data(), T.center.ptr, 3 * (size_t)num_nodes * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace fmm

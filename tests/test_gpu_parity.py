@@ -1,0 +1,190 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on the same
+seeded inputs.  Integer work (trees, lists) must match bit for bit; floating point to
+the tolerance written in each test."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2, shell_sources
+
+pytestmark = pytest.mark.gpu
+
+SHARED_ROOT = (np.zeros(3), np.full(3, 3.0))
+
+
+@pytest.fixture(scope="module")
+def fm():
+    import bachelors_amd
+    bachelors_amd.lib()
+    return bachelors_amd
+
+
+def _inputs(seed, ns, nt, kind="shell"):
+    rng = np.random.default_rng(seed)
+    src = shell_sources(rng, ns)
+    w = np.full(ns, 1.0 / ns)
+    if kind == "uniform":
+        tgt = rng.uniform(-2, 2, (nt, 3))
+    else:
+        tgt = np.concatenate([rng.uniform(-2, 2, (nt // 2, 3)), shell_sources(rng, nt - nt // 2)])
+    return src, w, tgt
+
+
+def _boundary_targets(orc, tgt, cap, rng, nadd=400):
+    """Targets placed exactly on child-box faces of the reference (bounding-box) target
+    tree: exercises half-open containment and the float-rounded one-ulp gaps/overlaps."""
+    T = orc.target_tree(tgt, cap)
+    sub = np.nonzero(T.first_child >= 0)[0]
+    pts = []
+    for _ in range(nadd):
+        b = sub[rng.integers(len(sub))]
+        ch = T.first_child[b] + rng.integers(8)
+        c, h = T.center[ch], T.half[ch]
+        p = c + rng.uniform(-1, 1, 3) * h
+        ax = rng.integers(3)
+        p[ax] = c[ax] + (h[ax] if rng.random() < 0.5 else -h[ax])
+        pts.append(p)
+    return np.concatenate([tgt, np.array(pts)])
+
+
+def _cmp_tree(g, o):
+    assert len(g["level"]) == len(o.level)
+    for k in ("level", "parent", "first_child", "begin", "end", "npts"):
+        assert np.array_equal(g[k], getattr(o, k)), k
+    assert np.array_equal(g["center"], o.center)
+    assert np.array_equal(g["slots"], o.slots)
+
+
+@pytest.mark.parametrize("frame", [0, 1])
+def test_trees_bit_exact(fm, orc, frame):
+    rng = np.random.default_rng(7)
+    src, w, tgt = _inputs(3, 20000, 6000)
+    if frame == 0:
+        tgt = _boundary_targets(orc, tgt, 24, rng)
+    F = fm.FMM(order=4, source_leaf_capacity=20, target_leaf_capacity=24, frame=frame)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    F.build_trees()
+    _cmp_tree(F.tree(0), orc.source_tree(src, 20))
+    _cmp_tree(F.tree(1), orc.target_tree(tgt, 24, root=SHARED_ROOT if frame == 1 else None))
+
+
+@pytest.mark.parametrize("frame", [0, 1])
+def test_interaction_lists_bit_exact(fm, orc, frame):
+    src, w, tgt = _inputs(4, 12000, 4000)
+    F = fm.FMM(order=4, source_leaf_capacity=24, target_leaf_capacity=24, frame=frame)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    F.build_trees()
+    S = orc.source_tree(src, 24)
+    T = orc.target_tree(tgt, 24, root=SHARED_ROOT if frame == 1 else None)
+    (oft, ofs), (oct_, ocs) = orc.interaction_lists(S, T)
+    F.leaf_expansions()
+    F.upward()
+    if frame == 1:
+        F.downward()
+    else:
+        # the reference frame builds the lists, then stops: its M2L has no shared operators
+        with pytest.raises(fm.FMMError):
+            F.downward()
+    (ft, fs), (ct, cs), ncls = F.lists()
+    assert (ncls > 0) == (frame == 1)
+    assert set(zip(ft.tolist(), fs.tolist())) == set(zip(oft.tolist(), ofs.tolist()))
+    assert set(zip(ct.tolist(), cs.tolist())) == set(zip(oct_.tolist(), ocs.tolist()))
+    assert len(ft) == len(oft) and len(ct) == len(oct_)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("p", [4, 12])
+def test_operator_tables_fp64(fm, orc, kind, p):
+    rng = np.random.default_rng(kind * 10 + p)
+    deg = np.array([int(np.sqrt(i)) for i in range((p + 1) ** 2)])
+    for _ in range(3):
+        if kind == 1:
+            t = rng.uniform(-3, 3, 3) + np.array([4.0, 0, 0])
+            si, so = 0.5, 1.0
+        else:
+            t = rng.uniform(-0.5, 0.5, 3)
+            si, so = (0.25, 0.5) if kind == 0 else (0.5, 0.25)
+        G = fm.make_operator(kind, p, t, si, so, 1)
+        T = orc.operator(kind, p, t, orc.CORRECTED)
+        if kind == 0:
+            E = T * (si ** deg[None, :]) / (so ** deg[:, None])
+        elif kind == 2:
+            E = T * (so ** (deg[:, None] + 1.0)) / (si ** (deg[None, :] + 1.0))
+        else:
+            E = T * (so ** (deg[:, None] + 1.0)) * (si ** deg[None, :])
+        # 1e-10: the oracle keeps the reference's truncated 1/sqrt2 (math.hpp:16)
+        assert np.abs(G - E).max() <= 1e-10 * np.abs(E).max()
+
+
+def _oracle_run(orc, src, w, tgt, p, caps, cap_t, flags=None):
+    return orc.FMMRun(src, w, tgt, p=p, cap_s=caps, cap_t=cap_t, flags=orc.CORRECTED if flags is None else flags,
+                      target_root=SHARED_ROOT)
+
+
+def _gpu_run(fm, src, w, tgt, p, caps, cap_t, precision):
+    F = fm.FMM(order=p, source_leaf_capacity=caps, target_leaf_capacity=cap_t, precision=precision, frame=1)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    B = F.run(1.0)
+    return F, B
+
+
+@pytest.fixture(scope="module")
+def small_case(orc):
+    src, w, tgt = _inputs(11, 4096, 1500)
+    R = _oracle_run(orc, src, w, tgt, 8, 32, 32)
+    return src, w, tgt, R
+
+
+def test_expansions_fp64(fm, orc, small_case):
+    src, w, tgt, R = small_case
+    F, _ = _gpu_run(fm, src, w, tgt, 8, 32, 32, "fp64")
+    S = orc.source_tree(src, 32)
+    T = orc.target_tree(tgt, 32, root=SHARED_ROOT)
+    checked = 0
+    for b in range(0, len(S.level), max(1, len(S.level) // 40)):
+        if S.first_child[b] < 0 and S.npts[b] == 0:
+            continue
+        Mo = R.multipole(b)
+        Mg = F.expansion(0, b)
+        assert np.abs(Mg - Mo).max() <= 1e-9 * max(np.abs(Mo).max(), 1e-300)
+        checked += 1
+    for b in range(0, len(T.level), max(1, len(T.level) // 40)):
+        if T.first_child[b] < 0 and T.npts[b] == 0:
+            continue
+        Lo = R.local(b)
+        Lg = F.expansion(1, b)
+        if np.abs(Lo).max() == 0:
+            assert np.abs(Lg).max() == 0
+        else:
+            assert np.abs(Lg - Lo).max() <= 1e-9 * np.abs(Lo).max()
+        checked += 1
+    assert checked > 20
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 3e-6), ("bf16x3", 1e-4)])
+def test_B_matches_oracle(fm, orc, small_case, precision, tol):
+    src, w, tgt, R = small_case
+    F, B = _gpu_run(fm, src, w, tgt, 8, 32, 32, precision)
+    Bo = R.B_input_order(len(tgt))
+    assert np.all(np.isfinite(B))
+    assert rel_l2(B, Bo) < tol
+    Bf, Bn = F.B_parts()
+    assert rel_l2(Bn, R.Bnear) < (1e-12 if precision == "fp64" else 1e-6)
+    assert rel_l2(Bf, R.Bfar) < tol * 10
+
+
+def test_direct_sum_gpu_vs_oracle(fm, orc):
+    import ctypes
+    src, w, tgt = _inputs(12, 3000, 500)
+    F = fm.FMM(order=4, source_leaf_capacity=32, target_leaf_capacity=32)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    F.build_trees()
+    Bo = orc.direct_B(src, w, tgt, current=2.0)
+    import torch
+    t = torch.from_numpy(tgt).cuda()
+    out = torch.zeros_like(t)
+    F.direct_B_dev(2.0, len(tgt), ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(out.data_ptr()))
+    assert rel_l2(out.cpu().numpy(), Bo) < 1e-13
