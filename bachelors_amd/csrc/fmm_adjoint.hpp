@@ -32,7 +32,6 @@ struct AdjointArgs {
     const void* down_ops_lo;
     int n_up_ops, n_down_ops;
     std::vector<PassView> m2m_levels, l2l_levels;   // indexed by child level
-    std::vector<int> m2m_pass_src;
     PassView m2l;                                   // src = far_t, dst = far_s (already transposed roles)
     double scale;
     int nt, ns;

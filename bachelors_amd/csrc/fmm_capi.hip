@@ -373,6 +373,8 @@ void stage_evaluate(fmm_ctx* c, double current, double* Bdev) {
 
 void set_sources_common(fmm_ctx* c, size_t n, const double* xyz, const double* w, const double* J, hipMemcpyKind kind) {
     require(n < (size_t)INT32_MAX / 3, FMM_ERR_LIMIT, "too many sources");
+    // caller memory may still be written by work on another stream
+    if (kind == hipMemcpyDeviceToDevice) FMM_HIP(hipDeviceSynchronize());
     require(n == 0 || (xyz && w), FMM_ERR_ARG, "null source arrays");
     c->ns = (int)n;
     c->have_J = J != nullptr;
@@ -391,6 +393,7 @@ void set_sources_common(fmm_ctx* c, size_t n, const double* xyz, const double* w
 
 void set_targets_common(fmm_ctx* c, size_t m, const double* xyz, hipMemcpyKind kind) {
     require(m < (size_t)INT32_MAX / 3, FMM_ERR_LIMIT, "too many targets");
+    if (kind == hipMemcpyDeviceToDevice) FMM_HIP(hipDeviceSynchronize());
     require(m == 0 || xyz, FMM_ERR_ARG, "null target array");
     c->nt = (int)m;
     size_t mm = std::max<size_t>(m, 1);
@@ -489,6 +492,7 @@ int fmm_set_targets_dev(fmm_ctx* c, size_t m, const double* xyz) {
 int fmm_set_weights_dev(fmm_ctx* c, const double* w) {
     return guard(c, [&] {
         require(w != nullptr, FMM_ERR_ARG, "null weights");
+        FMM_HIP(hipDeviceSynchronize());
         if (c->ns) FMM_HIP(hipMemcpyAsync(c->src_w.ptr, w, c->ns * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         if (c->stage >= ST_TREES) {
             int nss = c->S.num_slots;
@@ -550,6 +554,7 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
     return guard(c, [&] {
         require(c->stage >= ST_DOWN, FMM_ERR_STATE, "run the forward pass first");
         require(dLdB && dLdw, FMM_ERR_ARG, "null adjoint arrays");
+        FMM_HIP(hipDeviceSynchronize());
         AdjointArgs a;
         a.prec = c->prec;
         a.p = c->p;
@@ -571,7 +576,6 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
         a.up_ops_lo = c->up_ops.lo.ptr;
         a.down_ops = ops_ptr(c, c->down_ops);
         a.down_ops_lo = c->down_ops.lo.ptr;
-        a.m2m_pass_src.resize(MAX_LEVELS);
         for (int k = 0; k < MAX_LEVELS; k++) {
             const Pass* pm = k < (int)c->m2m_pass.size() ? &c->m2m_pass[k] : nullptr;
             const Pass* pl = k < (int)c->l2l_pass.size() ? &c->l2l_pass[k] : nullptr;
@@ -594,6 +598,7 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
 int fmm_direct_B_dev(fmm_ctx* c, double current, size_t m, const double* xyz, double* B) {
     return guard(c, [&] {
         require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built (sources are gathered with them)");
+        FMM_HIP(hipDeviceSynchronize());
         launch_direct(c->sxyz.as<double>(), c->sq.as<double>(), c->S.num_slots, xyz, (int)m,
                       current * REF_MU0 / (4.0 * REF_PI), B, c->stream);
         FMM_HIP(hipStreamSynchronize(c->stream));

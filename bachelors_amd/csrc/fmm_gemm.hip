@@ -34,7 +34,7 @@ __device__ __forceinline__ void add_out(double* p, double v, int atomic) {
 }
 
 // ---------------------------------------------------------------- FP64 / FP32
-template <class T>
+template <class T, bool TRANS>
 __global__ __launch_bounds__(256) void k_class_gemm(const T* __restrict__ ops, const GemmTile* __restrict__ tiles,
                                                     const int* __restrict__ pair_src, const int* __restrict__ pair_dst,
                                                     const T* __restrict__ src_exp, T* __restrict__ dst_exp, int Lp,
@@ -63,7 +63,8 @@ __global__ __launch_bounds__(256) void k_class_gemm(const T* __restrict__ ops, c
         for (int b = 0; b < 2; b++) acc[a][b] = acc_t{0, 0, 0, 0};
     for (int k0 = 0; k0 < Lp; k0 += BK) {
         for (int q = 0; q < 4; q++) {
-            As[lk + q][lr] = (arow < Lp) ? A[(size_t)arow * Lp + k0 + lk + q] : T(0);
+            size_t ai = TRANS ? (size_t)(k0 + lk + q) * Lp + arow : (size_t)arow * Lp + k0 + lk + q;
+            As[lk + q][lr] = (arow < Lp) ? A[ai] : T(0);
             Bs[lk + q][lr] = bvalid ? bptr[k0 + lk + q] : T(0);
         }
         __syncthreads();
@@ -105,6 +106,7 @@ __device__ __forceinline__ void split_bf16(float v, __bf16& h, __bf16& l) {
     l = (__bf16)(v - (float)h);
 }
 
+template <bool TRANS>
 __global__ __launch_bounds__(256) void k_class_gemm_bf16x3(const __bf16* __restrict__ ops_hi,
                                                            const __bf16* __restrict__ ops_lo,
                                                            const GemmTile* __restrict__ tiles,
@@ -139,8 +141,9 @@ __global__ __launch_bounds__(256) void k_class_gemm_bf16x3(const __bf16* __restr
         for (int q = 0; q < 8; q++) {
             int k = k0 + lk + q;
             bool ain = arow < Lp && k < Lp;
-            Ah[lr][lk + q] = ain ? ops_hi[opoff + (size_t)arow * Lp + k] : (__bf16)0.0f;
-            Al[lr][lk + q] = ain ? ops_lo[opoff + (size_t)arow * Lp + k] : (__bf16)0.0f;
+            size_t ai = opoff + (TRANS ? (size_t)k * Lp + arow : (size_t)arow * Lp + k);
+            Ah[lr][lk + q] = ain ? ops_hi[ai] : (__bf16)0.0f;
+            Al[lr][lk + q] = ain ? ops_lo[ai] : (__bf16)0.0f;
             float v = (bvalid && k < Lp) ? bptr[k] : 0.0f;
             __bf16 h, l;
             split_bf16(v, h, l);
@@ -182,23 +185,31 @@ __global__ __launch_bounds__(256) void k_class_gemm_bf16x3(const __bf16* __restr
     }
 }
 
-void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
-                const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, bool atomic,
-                hipStream_t st) {
-    if (ntiles == 0) return;
-    if (Lp % 32 != 0) throw Error(1, "padded expansion length must be a multiple of 32");
+template <bool TR>
+static void launch(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
+                   const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, bool atomic,
+                   hipStream_t st) {
     dim3 grid(ntiles, (Lp + BM - 1) / BM);
     if (prec == Precision::F64)
-        k_class_gemm<double><<<grid, 256, 0, st>>>((const double*)ops, d_tiles, pair_src, pair_dst,
-                                                   (const double*)src_exp, (double*)dst_exp, Lp, atomic ? 1 : 0);
+        k_class_gemm<double, TR><<<grid, 256, 0, st>>>((const double*)ops, d_tiles, pair_src, pair_dst,
+                                                       (const double*)src_exp, (double*)dst_exp, Lp, atomic ? 1 : 0);
     else if (prec == Precision::F32)
-        k_class_gemm<float><<<grid, 256, 0, st>>>((const float*)ops, d_tiles, pair_src, pair_dst,
-                                                  (const float*)src_exp, (float*)dst_exp, Lp, atomic ? 1 : 0);
+        k_class_gemm<float, TR><<<grid, 256, 0, st>>>((const float*)ops, d_tiles, pair_src, pair_dst,
+                                                      (const float*)src_exp, (float*)dst_exp, Lp, atomic ? 1 : 0);
     else
-        k_class_gemm_bf16x3<<<grid, 256, 0, st>>>((const __bf16*)ops, (const __bf16*)ops_lo, d_tiles, pair_src,
-                                                 pair_dst, (const float*)src_exp, (float*)dst_exp, Lp,
-                                                 atomic ? 1 : 0);
+        k_class_gemm_bf16x3<TR><<<grid, 256, 0, st>>>((const __bf16*)ops, (const __bf16*)ops_lo, d_tiles, pair_src,
+                                                     pair_dst, (const float*)src_exp, (float*)dst_exp, Lp,
+                                                     atomic ? 1 : 0);
     FMM_HIP(hipGetLastError());
+}
+
+void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
+                const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, bool atomic,
+                hipStream_t st, bool transpose) {
+    if (ntiles == 0) return;
+    if (Lp % 32 != 0) throw Error(1, "padded expansion length must be a multiple of 32");
+    if (transpose) launch<true>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, Lp, atomic, st);
+    else launch<false>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, Lp, atomic, st);
 }
 
 // host-side tiling of class column ranges: class c owns pairs [begin[c], begin[c+1])

@@ -23,8 +23,9 @@
  *
  * Conventions: every function returns FMM_OK (0) or an error code and never exits; the
  * message is in fmm_last_error(ctx).  The context owns all device memory.  Host arrays
- * are copied in; *_dev entry points take device pointers (copied device-to-device) and
- * run on the context's HIP stream.  One context per host thread.
+ * are copied in; *_dev entry points take device pointers (the device is synchronised
+ * before caller memory is read, then it is copied device-to-device) and run on the
+ * context's HIP stream.  One context per host thread.
  */
 #ifndef FMM_CAPI_H
 #define FMM_CAPI_H

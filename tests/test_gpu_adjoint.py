@@ -1,0 +1,64 @@
+"""GPU: the hand-written adjoint FMM (dL/dw for L = sum_i |B_i - B*_i|^2) through the C ABI
+and the torch.autograd harness."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2, shell_sources
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(seed=21, ns=6000, nt=1500):
+    rng = np.random.default_rng(seed)
+    src = shell_sources(rng, ns)
+    w = rng.uniform(0.5, 1.5, ns)
+    tgt = rng.uniform(-2, 2, (nt, 3))
+    return rng, src, w, tgt
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32", 2e-6)])
+def test_adjoint_is_the_transpose(precision, tol):
+    """B is linear in w, so g.B(w) == w.(adjoint g) for every g and w (exact up to rounding,
+    independent of the FMM truncation error)."""
+    import torch
+    from bachelors_amd.torch_fmm import FMMBiotSavart
+    rng, src, w, tgt = _case()
+    F = FMMBiotSavart(src, tgt, order=8, cap=32, precision=precision)
+    wt = torch.tensor(w, dtype=torch.float64, device="cuda", requires_grad=True)
+    B = F(wt)
+    g = torch.tensor(rng.normal(size=(len(tgt), 3)), dtype=torch.float64, device="cuda")
+    (B * g).sum().backward()
+    lhs = float((B * g).sum())
+    rhs = float((wt.detach() * wt.grad).sum())
+    assert abs(lhs - rhs) <= tol * abs(lhs)
+
+
+def test_adjoint_gradient_vs_finite_differences_and_direct():
+    import torch
+    from bachelors_amd.torch_fmm import FMMBiotSavart
+    from oracle import oracle as O
+    rng, src, w, tgt = _case(22)
+    wstar = w * (1 + 0.1 * (2 * rng.random(len(w)) - 1))
+    F = FMMBiotSavart(src, tgt, order=10, cap=32, precision="fp64")
+    with torch.no_grad():
+        Bstar = F(torch.tensor(wstar, device="cuda"))
+    wt = torch.tensor(w, dtype=torch.float64, device="cuda", requires_grad=True)
+    loss = ((F(wt) - Bstar) ** 2).sum()
+    loss.backward()
+    grad = wt.grad.cpu().numpy()
+    # central finite differences, h = 1e-3, on 64 seeded weights
+    idx = rng.choice(len(w), 64, replace=False)
+    fd = []
+    with torch.no_grad():
+        for j in idx:
+            wp = w.copy(); wp[j] += 1e-3
+            wm = w.copy(); wm[j] -= 1e-3
+            lp = float(((F(torch.tensor(wp, device="cuda")) - Bstar) ** 2).sum())
+            lm = float(((F(torch.tensor(wm, device="cuda")) - Bstar) ** 2).sum())
+            fd.append((lp - lm) / 2e-3)
+    fd = np.array(fd)
+    assert rel_l2(grad[idx], fd) < 1e-5
+    # against the FP64 direct-sum gradient (same loss with direct-sum B)
+    Bs = O.direct_B(src, wstar, tgt)
+    gd = O.direct_grad_w(src, w, tgt, Bs)
+    assert rel_l2(grad, gd) < 1e-3
