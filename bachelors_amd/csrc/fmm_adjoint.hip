@@ -200,9 +200,9 @@ __global__ __launch_bounds__(64) void k_p2p_adj(const int* __restrict__ leaves, 
                     a1 += (dz * g0[i] - dx * g2[i]) * ri3;
                     a2 += (dx * g1[i] - dy * g0[i]) * ri3;
                 }
-                atomicAdd(&gq[3 * j], (double)a0);
-                atomicAdd(&gq[3 * j + 1], (double)a1);
-                atomicAdd(&gq[3 * j + 2], (double)a2);
+                unsafeAtomicAdd(&gq[3 * j], (double)a0);
+                unsafeAtomicAdd(&gq[3 * j + 1], (double)a1);
+                unsafeAtomicAdd(&gq[3 * j + 2], (double)a2);
             }
         }
     }

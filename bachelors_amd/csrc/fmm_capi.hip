@@ -48,6 +48,7 @@ struct fmm_ctx {
     // inputs (input order)
     int ns = 0, nt = 0;
     bool have_J = false;
+    double tmin[3] = {1e+16, 1e+16, 1e+16}, tmax[3] = {-1e+16, -1e+16, -1e+16};   // target extent
     DevBuf src_xyz, src_w, src_J, tgt_xyz;
     // trees
     TreeDev S, T;
@@ -151,21 +152,32 @@ void stage_trees(fmm_ctx* c) {
                g.source_leaf_capacity, g.max_depth, true, st);
     double tc[3], th[3];
     if (g.frame == FMM_FRAME_SHARED) {
-        for (int a = 0; a < 3; a++) { tc[a] = g.source_root_center[a]; th[a] = g.source_root_half[a]; }
-    } else {
-        // math::getBoundingBox (math.cpp:39-57) on the host copy of the targets
-        std::vector<double> P(3 * (size_t)c->nt);
-        FMM_HIP(hipMemcpyAsync(P.data(), c->tgt_xyz.ptr, P.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-        FMM_HIP(hipStreamSynchronize(st));
-        for (int a = 0; a < 3; a++) {
-            double mx = -1e+16, mn = 1e+16;
-            for (int i = 0; i < c->nt; i++) {
-                double v = P[3 * (size_t)i + a];
-                if (v > mx) mx = v;
-                if (v < mn) mn = v;
+        // the source root, grown by doublings along the source lattice until it holds every
+        // target (half-open), so all boxes of both trees stay on one dyadic lattice
+        const double* c0 = g.source_root_center;
+        const double* h0 = g.source_root_half;
+        int k = 0;
+        for (;; k++) {
+            require(k <= 12, FMM_ERR_LIMIT, "targets too far from the source root for the shared frame");
+            double sc = std::ldexp(1.0, k);
+            bool ok = true;
+            for (int a = 0; a < 3 && ok; a++) {
+                double cell = 2 * h0[a];
+                double j = c->nt ? std::ceil((c0[a] - h0[a] - c->tmin[a]) / cell) : 0.0;
+                if (j < 0) j = 0;
+                if (j > sc - 1) { ok = false; break; }
+                double lo = c0[a] - h0[a] - cell * j;
+                if (c->nt && !(c->tmax[a] < lo + cell * sc)) { ok = false; break; }
+                th[a] = h0[a] * sc;
+                tc[a] = lo + th[a];
             }
-            double hd = (mx - mn) / 2 + 1e-5;
-            tc[a] = mn + hd;
+            if (ok) break;
+        }
+    } else {
+        // math::getBoundingBox (math.cpp:39-57)
+        for (int a = 0; a < 3; a++) {
+            double hd = (c->tmax[a] - c->tmin[a]) / 2 + 1e-5;
+            tc[a] = c->tmin[a] + hd;
             th[a] = hd * 1.1;
         }
     }
@@ -257,6 +269,69 @@ void stage_upward(fmm_ctx* c) {
     c->stage = ST_UP;
 }
 
+// Reference frame: every M2L and L2L translation vector is unique (bounding-box target
+// root, float-rounded centres), so operators are built per pair, in chunks, and applied
+// with the same GEMM kernels (one 3-column tile per pair).  A correctness path: the
+// shared frame is the fast one.
+void run_per_pair(fmm_ctx* c, int kind, const int* d_src, const int* d_dst, int npairs, const TreeDev& Tsrc,
+                  const TreeDev& Tdst, const ScaleTable& sg_src, const ScaleTable& sg_dst, const void* src_exp,
+                  void* dst_exp, bool atomic) {
+    hipStream_t st = c->stream;
+    OpTables& O = c->down_ops;
+    const size_t per = (size_t)c->Lp * c->Lp * sizeof(double);
+    int chunk = (int)std::max<size_t>(1, ((size_t)256 << 20) / per);
+    std::vector<GemmTile> tiles;
+    for (int b = 0; b < npairs; b += chunk) {
+        int n = std::min(chunk, npairs - b);
+        OpSpec* sp = O.specs.get<OpSpec>(n);
+        make_pair_specs(d_src + b, d_dst + b, n, Tsrc.center.as<double>(), Tdst.center.as<double>(),
+                        Tsrc.level.as<int>(), Tdst.level.as<int>(), sg_src, sg_dst, kind, c->cfg.m2l_full, sp, st);
+        O.nops = n;
+        make_operators(sp, n, c->p, c->Lp, O.f64.get<double>((size_t)n * c->Lp * c->Lp), st);
+        finish_tables(c, O);
+        tiles.resize(n);
+        for (int i = 0; i < n; i++) tiles[i] = GemmTile{i, 3 * (b + i), 3};
+        GemmTile* dt = c->m2l_pass.tiles.get<GemmTile>(n);
+        FMM_HIP(hipMemcpyAsync(dt, tiles.data(), n * sizeof(GemmTile), hipMemcpyHostToDevice, st));
+        class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, dt, n, d_src, d_dst, src_exp, dst_exp, c->Lp, atomic, st);
+        FMM_HIP(hipStreamSynchronize(st));   // the chunk buffers are reused
+    }
+}
+
+void downward_per_pair(fmm_ctx* c) {
+    hipStream_t st = c->stream;
+    const TreeDev& S = c->S;
+    const TreeDev& T = c->T;
+    size_t n = (size_t)T.num_nodes * 3 * c->Lp;
+    void* Lx = c->Lx.reserve(n * c->esize());
+    FMM_HIP(hipMemsetAsync(Lx, 0, n * c->esize(), st));
+    c->rec(E_TABLES);
+    run_per_pair(c, OPK_M2L, c->lists.far_s.as<int>(), c->lists.far_t.as<int>(), c->lists.num_far, S, T, c->sigS,
+                 c->sigT, c->M.ptr, Lx, true);
+    c->rec(E_M2L);
+    // L2L with each child's exact translation (calculation_point_octree.cpp:277-299)
+    c->l2l_pass.resize(MAX_LEVELS);
+    for (int k = 1; k < T.num_levels; k++) {
+        std::vector<int> src, dst;
+        for (int nd = T.level_begin[k - 1]; nd < T.level_begin[k]; nd++) {
+            int fc = T.h_first_child[nd];
+            if (fc < 0) continue;
+            for (int o = 0; o < 8; o++)
+                if (nonempty_host(T, fc + o)) { src.push_back(nd); dst.push_back(fc + o); }
+        }
+        if (src.empty()) continue;
+        Pass& P = c->l2l_pass[k];
+        upload_pass(P, src, dst, std::vector<GemmTile>(), st);
+        P.ntiles = 0;   // per-pair operators: this pass has no shared tiles (no adjoint)
+        run_per_pair(c, OPK_L2L, P.src.as<int>(), P.dst.as<int>(), (int)src.size(), T, T, c->sigT, c->sigT, Lx, Lx,
+                     false);
+    }
+    c->down_ops.nops = 0;
+    c->m2l_pass.ntiles = 0;
+    c->rec(E_L2L);
+    c->stage = ST_DOWN;
+}
+
 void stage_downward(fmm_ctx* c) {
     require(c->stage >= ST_UP, FMM_ERR_STATE, "Multipoles are not ready!");
     hipStream_t st = c->stream;
@@ -265,9 +340,11 @@ void stage_downward(fmm_ctx* c) {
     const bool shared = c->cfg.frame == FMM_FRAME_SHARED;
     build_lists(c->lists, c->lscr, S, T, c->cfg.mac_scale, shared, st);
     c->lists_ready = true;
-    require(shared, FMM_ERR_UNSUPPORTED,
-            "the M2L pass needs the shared frame (reference-frame trees and lists are available for parity)");
     c->rec(E_LISTS);
+    if (!shared) {
+        downward_per_pair(c);
+        return;
+    }
     // operator tables: L2L per (level, orientation), then one M2L operator per class
     OpTables& O = c->down_ops;
     int nlev = T.num_levels;
@@ -399,7 +476,25 @@ void set_targets_common(fmm_ctx* c, size_t m, const double* xyz, hipMemcpyKind k
     size_t mm = std::max<size_t>(m, 1);
     if (m) FMM_HIP(hipMemcpyAsync(c->tgt_xyz.get<double>(3 * mm), xyz, 3 * m * sizeof(double), kind, c->stream));
     else c->tgt_xyz.get<double>(3);
+    // target extent, once per target set (math::max / math::min, math.cpp:60-84)
+    std::vector<double> h;
+    const double* P = xyz;
+    if (kind == hipMemcpyDeviceToDevice && m) {
+        h.resize(3 * m);
+        FMM_HIP(hipMemcpyAsync(h.data(), xyz, 3 * m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        P = h.data();
+    }
     FMM_HIP(hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; a++) {
+        double mx = -1e+16, mn = 1e+16;
+        for (size_t i = 0; i < m; i++) {
+            double v = P[3 * i + a];
+            if (v > mx) mx = v;
+            if (v < mn) mn = v;
+        }
+        c->tmin[a] = mn;
+        c->tmax[a] = mx;
+    }
     c->stage = ST_EMPTY;
 }
 
@@ -434,7 +529,7 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
     if (c->cfg.max_depth <= 0 || c->cfg.max_depth > MAX_LEVELS - 1) c->cfg.max_depth = MAX_LEVELS - 1;
     c->p = cfg->order;
     c->L = nlen(c->p);
-    c->Lp = pad_to(c->L, 32);
+    c->Lp = pad_to(c->L, 64);
     c->prec = (Precision)cfg->precision;
     int rc = guard(c, [&] {
         int ndev = 0;
@@ -554,6 +649,7 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
     return guard(c, [&] {
         require(c->stage >= ST_DOWN, FMM_ERR_STATE, "run the forward pass first");
         require(dLdB && dLdw, FMM_ERR_ARG, "null adjoint arrays");
+        require(c->cfg.frame == FMM_FRAME_SHARED, FMM_ERR_UNSUPPORTED, "the adjoint needs the shared frame");
         FMM_HIP(hipDeviceSynchronize());
         AdjointArgs a;
         a.prec = c->prec;
@@ -751,7 +847,7 @@ int fmm_export_expansion(fmm_ctx* c, int which, int node, double* out) {
 int fmm_make_operator(int kind, int order, const double* t, double sig_in, double sig_out, int m2l_full, double* out) {
     if (kind < 0 || kind > 2 || order < 1 || order > MAX_ORDER || !t || !out) return FMM_ERR_ARG;
     try {
-        int L = nlen(order), Lp = pad_to(L, 32);
+        int L = nlen(order), Lp = pad_to(L, 64);
         OpSpec s;
         for (int a = 0; a < 3; a++) s.t[a] = t[a];
         s.sig_in = sig_in;

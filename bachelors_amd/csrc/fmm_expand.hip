@@ -226,6 +226,9 @@ template <> struct V4<double> { typedef double4 t; static __device__ double4 mak
 __device__ __forceinline__ float rsqrt_t(float x) { return rsqrtf(x); }
 __device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
 
+// Coordinates are offsets from the target-leaf centre formed in FP64 and kept as an
+// unevaluated sum hi + lo of two R values, so the pair difference
+// d = (x_hi - y_hi) + (x_lo - y_lo) stays accurate for very close pairs.
 template <class R>
 __global__ __launch_bounds__(64) void k_p2p(const int* __restrict__ leaves, const double* __restrict__ tcenter,
                                             const int* __restrict__ tbegin, const int* __restrict__ tend,
@@ -235,7 +238,9 @@ __global__ __launch_bounds__(64) void k_p2p(const int* __restrict__ leaves, cons
                                             const double* __restrict__ sq, const double* __restrict__ txyz,
                                             double* __restrict__ Bnear) {
     typedef typename V4<R>::t v4;
-    __shared__ v4 spos[64];
+    constexpr bool split = sizeof(R) == 4;
+    __shared__ v4 shi[64];
+    __shared__ v4 slo[64];
     __shared__ v4 sqv[64];
     const int leaf = leaves[blockIdx.x];
     const int lane = threadIdx.x;
@@ -244,11 +249,11 @@ __global__ __launch_bounds__(64) void k_p2p(const int* __restrict__ leaves, cons
     for (int k0 = tbegin[leaf]; k0 < tend[leaf]; k0 += 64) {
         int k = k0 + lane;
         bool valid = k < tend[leaf];
-        R xr = 0, yr = 0, zr = 0;
+        R xh = 0, yh = 0, zh = 0, xl = 0, yl = 0, zl = 0;
         if (valid) {
-            xr = (R)(txyz[3 * k] - cx);
-            yr = (R)(txyz[3 * k + 1] - cy);
-            zr = (R)(txyz[3 * k + 2] - cz);
+            double dx = txyz[3 * k] - cx, dy = txyz[3 * k + 1] - cy, dz = txyz[3 * k + 2] - cz;
+            xh = (R)dx; yh = (R)dy; zh = (R)dz;
+            if (split) { xl = (R)(dx - (double)xh); yl = (R)(dy - (double)yh); zl = (R)(dz - (double)zh); }
         }
         double a0 = 0, a1 = 0, a2 = 0;
         for (int ci = cb; ci < ce; ci++) {
@@ -258,15 +263,22 @@ __global__ __launch_bounds__(64) void k_p2p(const int* __restrict__ leaves, cons
                 int j = j0 + lane;
                 __syncthreads();
                 if (j < se) {
-                    spos[lane] = V4<R>::make((R)(sxyz[3 * j] - cx), (R)(sxyz[3 * j + 1] - cy), (R)(sxyz[3 * j + 2] - cz));
+                    double dx = sxyz[3 * j] - cx, dy = sxyz[3 * j + 1] - cy, dz = sxyz[3 * j + 2] - cz;
+                    R hx = (R)dx, hy = (R)dy, hz = (R)dz;
+                    shi[lane] = V4<R>::make(hx, hy, hz);
+                    if (split) slo[lane] = V4<R>::make((R)(dx - (double)hx), (R)(dy - (double)hy), (R)(dz - (double)hz));
                     sqv[lane] = V4<R>::make((R)sq[3 * j], (R)sq[3 * j + 1], (R)sq[3 * j + 2]);
                 }
                 __syncthreads();
                 int nj = min(64, se - j0);
                 R f0 = 0, f1 = 0, f2 = 0;
                 for (int jj = 0; jj < nj; jj++) {
-                    v4 y = spos[jj], q = sqv[jj];
-                    R dx = xr - y.x, dy = yr - y.y, dz = zr - y.z;
+                    v4 y = shi[jj], q = sqv[jj];
+                    R dx = xh - y.x, dy = yh - y.y, dz = zh - y.z;
+                    if (split) {
+                        v4 yl4 = slo[jj];
+                        dx += xl - yl4.x; dy += yl - yl4.y; dz += zl - yl4.z;
+                    }
                     R r2 = dx * dx + dy * dy + dz * dz;
                     R ri = r2 > (R)0 ? rsqrt_t(r2) : (R)0;
                     R ri3 = ri * ri * ri;

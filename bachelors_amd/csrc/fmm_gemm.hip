@@ -3,17 +3,21 @@
 // A "class" is one translation operator T (Lp x Lp, row = output coefficient) shared
 // by a list of (source node, destination node) pairs: an M2L offset class, or one M2M /
 // L2L orientation of a level.  Column j of a class is component j % 3 of pair j / 3:
-// B[:, j] = src_exp[src][comp][:], and the product T B[:, j] is added to
-// dst_exp[dst][comp][:].  One launch covers every column tile of every class; the
-// column tiles never straddle classes.  Expansions are stored node-major as
-// [node][3][Lp] so a column is one contiguous, 16-byte-aligned vector.
+// B[:, j] = src_exp[src][comp][:], and T B[:, j] is added to dst_exp[dst][comp][:].  One
+// launch covers every 64-column tile of every class (tiles never straddle classes).
+// Expansions are node-major [node][3][Lp] so a column is one contiguous vector.
 //
-// Precisions (template Mode):
-//   F64    : v_mfma_f64_16x16x4f64, FP64 operators and expansions;
-//   F32    : v_mfma_f32_16x16x4f32, FP32 operators/expansions, FP32 accumulate;
-//   BF16X3 : v_mfma_f32_16x16x32_bf16 on split operands A = Ah + Al (pre-split tables),
-//            B = Bh + Bl (split on load), C += Ah Bh + Ah Bl + Al Bh, FP32 accumulate.
-// Tile: 64 rows x 64 columns per 256-thread workgroup; each wave owns a 32 x 32 block.
+// Each workgroup (4 waves) computes the whole Lp-row panel of its 64 columns, so every
+// gathered column is read from HBM once; wave w owns rows [w Lp/4, (w+1) Lp/4) as RB x 4
+// MFMA blocks of 16 x 16 (Lp = 64 RB).  K is staged through double-buffered LDS with the
+// next step's global loads issued before the current step's MFMAs (one barrier per step).
+//
+// Precisions:
+//   F64    : v_mfma_f64_16x16x4f64 (C register r holds row 4r + lane/16);
+//   F32    : v_mfma_f32_16x16x4f32 (C register r holds row 4(lane/16) + r), FP32 accumulate;
+//   BF16X3 : v_mfma_f32_16x16x32_bf16 on A = Ah + Al (tables split offline from FP64) and
+//            B = Bh + Bl (split on load): C += Al Bh + Ah Bl + Ah Bh, FP32 accumulate.
+// TRANS applies T^T (the adjoint passes) by reading the operator column-wise.
 #include "fmm_gemm.hpp"
 
 namespace fmm {
@@ -22,194 +26,292 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int BM = 64, BN = 64;
+constexpr int BN = 64;
 
+// hardware float atomics (global_atomic_add_f32/f64) on coarse-grained device memory
 __device__ __forceinline__ void add_out(float* p, float v, int atomic) {
-    if (atomic) atomicAdd(p, v);
+    if (atomic) unsafeAtomicAdd(p, v);
     else *p += v;
 }
 __device__ __forceinline__ void add_out(double* p, double v, int atomic) {
-    if (atomic) atomicAdd(p, v);
+    if (atomic) unsafeAtomicAdd(p, v);
     else *p += v;
 }
 
+template <class T> struct Vec4;
+template <> struct Vec4<float> {
+    typedef float4 t;
+    static __device__ __forceinline__ float4 load(const float* p) { return *(const float4*)p; }
+    static __device__ __forceinline__ float get(const float4& v, int i) { return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w; }
+};
+template <> struct Vec4<double> {
+    struct t { double2 a, b; };
+    static __device__ __forceinline__ t load(const double* p) { return t{*(const double2*)p, *(const double2*)(p + 2)}; }
+    static __device__ __forceinline__ double get(const t& v, int i) { return i == 0 ? v.a.x : i == 1 ? v.a.y : i == 2 ? v.b.x : v.b.y; }
+};
+
 // ---------------------------------------------------------------- FP64 / FP32
-template <class T, bool TRANS>
-__global__ __launch_bounds__(256) void k_class_gemm(const T* __restrict__ ops, const GemmTile* __restrict__ tiles,
-                                                    const int* __restrict__ pair_src, const int* __restrict__ pair_dst,
-                                                    const T* __restrict__ src_exp, T* __restrict__ dst_exp, int Lp,
-                                                    int atomic) {
-    constexpr int BK = 16;
-    __shared__ T As[BK][BM + 1];
-    __shared__ T Bs[BK][BN + 1];
+template <class T, int RB, bool TRANS>
+__global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const GemmTile* __restrict__ tiles,
+                                              const int* __restrict__ pair_src, const int* __restrict__ pair_dst,
+                                              const T* __restrict__ src_exp, T* __restrict__ dst_exp, int atomic) {
+    constexpr int Lp = 64 * RB, BK = 16, SA = Lp + 4, SB = BN + 4, NK = Lp / BK;
+    typedef Vec4<T> V;
+    typedef typename V::t v4;
+    typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
+    __shared__ T As[2][BK][SA];
+    __shared__ T Bs[2][BK][SB];
     const GemmTile tile = tiles[blockIdx.x];
-    const int row0 = blockIdx.y * BM;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const T* A = ops + (size_t)tile.op * Lp * Lp;
-    // loader coordinates: 256 threads x 4 elements for each of A (64x16) and B (16x64)
-    const int lr = tid >> 2, lk = (tid & 3) * 4;
-    const int arow = row0 + lr;
-    const int bcol = tile.col0 + lr;
-    const bool bvalid = lr < tile.ncol;
+    // B loader: one column and 4 consecutive k per thread
+    const int bc = tid >> 2, bq = (tid & 3) * 4;
     const T* bptr = nullptr;
-    if (bvalid) {
-        int pair = bcol / 3, comp = bcol - 3 * pair;
+    if (bc < tile.ncol) {
+        int col = tile.col0 + bc;
+        int pair = col / 3, comp = col - 3 * pair;
         bptr = src_exp + ((size_t)pair_src[pair] * 3 + comp) * Lp;
     }
-    typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
-    acc_t acc[2][2];
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = acc_t{0, 0, 0, 0};
-    for (int k0 = 0; k0 < Lp; k0 += BK) {
-        for (int q = 0; q < 4; q++) {
-            size_t ai = TRANS ? (size_t)(k0 + lk + q) * Lp + arow : (size_t)arow * Lp + k0 + lk + q;
-            As[lk + q][lr] = (arow < Lp) ? A[ai] : T(0);
-            Bs[lk + q][lr] = bvalid ? bptr[k0 + lk + q] : T(0);
+    // A loader: RB quads of 4 per thread.  Non-transposed: (row, 4 consecutive k);
+    // transposed: (4 consecutive rows, one k) of T^T = columns of T.
+    v4 ra[RB], rb;
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < RB; j++) {
+            int qi = tid + 256 * j;
+            if (!TRANS) {
+                int row = qi >> 2, kq = (qi & 3) * 4;
+                ra[j] = V::load(A + (size_t)row * Lp + k0 + kq);
+            } else {
+                int k = qi / (Lp / 4), rq = (qi % (Lp / 4)) * 4;
+                ra[j] = V::load(A + (size_t)(k0 + k) * Lp + rq);
+            }
         }
-        __syncthreads();
+        if (bptr) rb = V::load(bptr + k0 + bq);
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < RB; j++) {
+            int qi = tid + 256 * j;
+            if (!TRANS) {
+                int row = qi >> 2, kq = (qi & 3) * 4;
+                for (int i = 0; i < 4; i++) As[buf][kq + i][row] = V::get(ra[j], i);
+            } else {
+                int k = qi / (Lp / 4), rq = (qi % (Lp / 4)) * 4;
+                for (int i = 0; i < 4; i++) As[buf][k][rq + i] = V::get(ra[j], i);
+            }
+        }
+        for (int i = 0; i < 4; i++) Bs[buf][bq + i][bc] = bptr ? V::get(rb, i) : T(0);
+    };
+    acc_t acc[RB][4];
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    const int wrow = wave * 16 * RB;
+    for (int kt = 0; kt < NK; kt++) {
+        const int cur = kt & 1;
+        if (kt + 1 < NK) gload((kt + 1) * BK);
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
-            T af[2], bf[2];
-            for (int a = 0; a < 2; a++) af[a] = As[ks + (lane >> 4)][wm + a * 16 + (lane & 15)];
-            for (int b = 0; b < 2; b++) bf[b] = Bs[ks + (lane >> 4)][wn + b * 16 + (lane & 15)];
-            for (int a = 0; a < 2; a++)
-                for (int b = 0; b < 2; b++) {
+            T af[RB], bf[4];
+            const int kk = ks + (lane >> 4);
+#pragma unroll
+            for (int a = 0; a < RB; a++) af[a] = As[cur][kk][wrow + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 4; b++) bf[b] = Bs[cur][kk][b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < RB; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
                     if constexpr (sizeof(T) == 8)
                         acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
                     else
                         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
                 }
         }
+        if (kt + 1 < NK) lstore(cur ^ 1);
         __syncthreads();
     }
-    // epilogue: scatter-add into the destination expansions
-    for (int b = 0; b < 2; b++) {
-        int cl = wn + b * 16 + (lane & 15);
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        int cl = b * 16 + (lane & 15);
         if (cl >= tile.ncol) continue;
         int col = tile.col0 + cl;
         int pair = col / 3, comp = col - 3 * pair;
-        T* dst = dst_exp + ((size_t)pair_dst[pair] * 3 + comp) * Lp;
-        for (int a = 0; a < 2; a++)
+        T* dst = dst_exp + ((size_t)pair_dst[pair] * 3 + comp) * Lp + wrow;
+#pragma unroll
+        for (int a = 0; a < RB; a++)
+#pragma unroll
             for (int r = 0; r < 4; r++) {
-                // f64 MFMA: reg r holds row 4r + lane/16; f32 MFMA: row 4(lane/16) + r
                 int rr = (sizeof(T) == 8) ? (4 * r + (lane >> 4)) : (4 * (lane >> 4) + r);
-                int row = row0 + wm + a * 16 + rr;
-                if (row < Lp) add_out(dst + row, (T)acc[a][b][r], atomic);
+                add_out(dst + a * 16 + rr, (T)acc[a][b][r], atomic);
             }
     }
 }
 
 // ---------------------------------------------------------------- BF16x3
-__device__ __forceinline__ void split_bf16(float v, __bf16& h, __bf16& l) {
-    h = (__bf16)v;
-    l = (__bf16)(v - (float)h);
-}
-
-template <bool TRANS>
-__global__ __launch_bounds__(256) void k_class_gemm_bf16x3(const __bf16* __restrict__ ops_hi,
-                                                           const __bf16* __restrict__ ops_lo,
-                                                           const GemmTile* __restrict__ tiles,
-                                                           const int* __restrict__ pair_src,
-                                                           const int* __restrict__ pair_dst,
-                                                           const float* __restrict__ src_exp,
-                                                           float* __restrict__ dst_exp, int Lp, int atomic) {
-    constexpr int BK = 32;
-    // k-contiguous rows so each lane reads its 8 consecutive k values as one vector
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[BM][BK + 8];
-    __shared__ __attribute__((aligned(16))) __bf16 Al[BM][BK + 8];
-    __shared__ __attribute__((aligned(16))) __bf16 Bh[BN][BK + 8];
-    __shared__ __attribute__((aligned(16))) __bf16 Bl[BN][BK + 8];
+template <int RB, bool TRANS>
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ ops_hi, const __bf16* __restrict__ ops_lo,
+                                                     const GemmTile* __restrict__ tiles,
+                                                     const int* __restrict__ pair_src, const int* __restrict__ pair_dst,
+                                                     const float* __restrict__ src_exp, float* __restrict__ dst_exp,
+                                                     int atomic) {
+    constexpr int Lp = 64 * RB, BK = 32, S = BK + 8, NK = Lp / BK;
+    // k-contiguous rows: each lane reads its 8 consecutive k as one 16-byte vector
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[2][Lp][S];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[2][Lp][S];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[2][BN][S];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[2][BN][S];
     const GemmTile tile = tiles[blockIdx.x];
-    const int row0 = blockIdx.y * BM;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const size_t opoff = (size_t)tile.op * Lp * Lp;
-    const int lr = tid >> 2, lk = (tid & 3) * 8;   // 256 threads x 8 elements = 64 x 32
-    const int arow = row0 + lr;
-    const int bcol = tile.col0 + lr;
-    const bool bvalid = lr < tile.ncol;
+    const __bf16* Ahg = ops_hi + opoff;
+    const __bf16* Alg = ops_lo + opoff;
+    const int bc = tid >> 2, bq = (tid & 3) * 8;
     const float* bptr = nullptr;
-    if (bvalid) {
-        int pair = bcol / 3, comp = bcol - 3 * pair;
+    if (bc < tile.ncol) {
+        int col = tile.col0 + bc;
+        int pair = col / 3, comp = col - 3 * pair;
         bptr = src_exp + ((size_t)pair_src[pair] * 3 + comp) * Lp;
     }
-    f32x4 acc[2][2];
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0, 0, 0, 0};
-    for (int k0 = 0; k0 < Lp; k0 += BK) {
-        for (int q = 0; q < 8; q++) {
-            int k = k0 + lk + q;
-            bool ain = arow < Lp && k < Lp;
-            size_t ai = opoff + (TRANS ? (size_t)k * Lp + arow : (size_t)arow * Lp + k);
-            Ah[lr][lk + q] = ain ? ops_hi[ai] : (__bf16)0.0f;
-            Al[lr][lk + q] = ain ? ops_lo[ai] : (__bf16)0.0f;
-            float v = (bvalid && k < Lp) ? bptr[k] : 0.0f;
-            __bf16 h, l;
-            split_bf16(v, h, l);
-            Bh[lr][lk + q] = h;
-            Bl[lr][lk + q] = l;
-        }
-        __syncthreads();
-        bf16x8 ah[2], al[2], bh[2], bl[2];
-        const int kk = 8 * (lane >> 4);
-        for (int a = 0; a < 2; a++) {
-            int r = wm + a * 16 + (lane & 15);
-            ah[a] = *(const bf16x8*)&Ah[r][kk];
-            al[a] = *(const bf16x8*)&Al[r][kk];
-        }
-        for (int b = 0; b < 2; b++) {
-            int c = wn + b * 16 + (lane & 15);
-            bh[b] = *(const bf16x8*)&Bh[c][kk];
-            bl[b] = *(const bf16x8*)&Bl[c][kk];
-        }
-        for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) {
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+    // A loader: RB units of 8 bf16 per thread and part
+    bf16x8 rah[RB], ral[RB];
+    float4 rb0 = {0, 0, 0, 0}, rb1 = {0, 0, 0, 0};
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < RB; j++) {
+            int qi = tid + 256 * j;
+            if (!TRANS) {
+                int row = qi >> 2, kq = (qi & 3) * 8;
+                rah[j] = *(const bf16x8*)(Ahg + (size_t)row * Lp + k0 + kq);
+                ral[j] = *(const bf16x8*)(Alg + (size_t)row * Lp + k0 + kq);
+            } else {
+                int k = qi / (Lp / 8), rq = (qi % (Lp / 8)) * 8;
+                rah[j] = *(const bf16x8*)(Ahg + (size_t)(k0 + k) * Lp + rq);
+                ral[j] = *(const bf16x8*)(Alg + (size_t)(k0 + k) * Lp + rq);
             }
+        }
+        if (bptr) {
+            rb0 = *(const float4*)(bptr + k0 + bq);
+            rb1 = *(const float4*)(bptr + k0 + bq + 4);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < RB; j++) {
+            int qi = tid + 256 * j;
+            if (!TRANS) {
+                int row = qi >> 2, kq = (qi & 3) * 8;
+                *(bf16x8*)&Ah[buf][row][kq] = rah[j];
+                *(bf16x8*)&Al[buf][row][kq] = ral[j];
+            } else {
+                int k = qi / (Lp / 8), rq = (qi % (Lp / 8)) * 8;
+                for (int i = 0; i < 8; i++) {
+                    Ah[buf][rq + i][k] = rah[j][i];
+                    Al[buf][rq + i][k] = ral[j][i];
+                }
+            }
+        }
+        float v[8] = {rb0.x, rb0.y, rb0.z, rb0.w, rb1.x, rb1.y, rb1.z, rb1.w};
+        bf16x8 h, l;
+        for (int i = 0; i < 8; i++) {
+            float x = bptr ? v[i] : 0.0f;
+            __bf16 hh = (__bf16)x;
+            h[i] = hh;
+            l[i] = (__bf16)(x - (float)hh);
+        }
+        *(bf16x8*)&Bh[buf][bc][bq] = h;
+        *(bf16x8*)&Bl[buf][bc][bq] = l;
+    };
+    f32x4 acc[RB][4];
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    const int wrow = wave * 16 * RB;
+    const int kk = 8 * (lane >> 4);
+    for (int kt = 0; kt < NK; kt++) {
+        const int cur = kt & 1;
+        if (kt + 1 < NK) gload((kt + 1) * BK);
+        bf16x8 bh[4], bl[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            bh[b] = *(const bf16x8*)&Bh[cur][b * 16 + (lane & 15)][kk];
+            bl[b] = *(const bf16x8*)&Bl[cur][b * 16 + (lane & 15)][kk];
+        }
+#pragma unroll
+        for (int a = 0; a < RB; a++) {
+            int r = wrow + a * 16 + (lane & 15);
+            bf16x8 ah = *(const bf16x8*)&Ah[cur][r][kk];
+            bf16x8 al = *(const bf16x8*)&Al[cur][r][kk];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[b], acc[a][b], 0, 0, 0);
+            }
+        }
+        if (kt + 1 < NK) lstore(cur ^ 1);
         __syncthreads();
     }
-    for (int b = 0; b < 2; b++) {
-        int cl = wn + b * 16 + (lane & 15);
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        int cl = b * 16 + (lane & 15);
         if (cl >= tile.ncol) continue;
         int col = tile.col0 + cl;
         int pair = col / 3, comp = col - 3 * pair;
-        float* dst = dst_exp + ((size_t)pair_dst[pair] * 3 + comp) * Lp;
-        for (int a = 0; a < 2; a++)
-            for (int r = 0; r < 4; r++) {
-                int row = row0 + wm + a * 16 + 4 * (lane >> 4) + r;
-                if (row < Lp) add_out(dst + row, acc[a][b][r], atomic);
-            }
+        float* dst = dst_exp + ((size_t)pair_dst[pair] * 3 + comp) * Lp + wrow;
+#pragma unroll
+        for (int a = 0; a < RB; a++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) add_out(dst + a * 16 + 4 * (lane >> 4) + r, acc[a][b][r], atomic);
     }
 }
 
-template <bool TR>
-static void launch(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
-                   const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, bool atomic,
-                   hipStream_t st) {
-    dim3 grid(ntiles, (Lp + BM - 1) / BM);
+template <int RB, bool TR>
+static void launch_rb(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
+                      const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, bool atomic,
+                      hipStream_t st) {
     if (prec == Precision::F64)
-        k_class_gemm<double, TR><<<grid, 256, 0, st>>>((const double*)ops, d_tiles, pair_src, pair_dst,
-                                                       (const double*)src_exp, (double*)dst_exp, Lp, atomic ? 1 : 0);
+        k_gemm<double, RB, TR><<<ntiles, 256, 0, st>>>((const double*)ops, d_tiles, pair_src, pair_dst,
+                                                      (const double*)src_exp, (double*)dst_exp, atomic ? 1 : 0);
     else if (prec == Precision::F32)
-        k_class_gemm<float, TR><<<grid, 256, 0, st>>>((const float*)ops, d_tiles, pair_src, pair_dst,
-                                                      (const float*)src_exp, (float*)dst_exp, Lp, atomic ? 1 : 0);
+        k_gemm<float, RB, TR><<<ntiles, 256, 0, st>>>((const float*)ops, d_tiles, pair_src, pair_dst,
+                                                     (const float*)src_exp, (float*)dst_exp, atomic ? 1 : 0);
     else
-        k_class_gemm_bf16x3<TR><<<grid, 256, 0, st>>>((const __bf16*)ops, (const __bf16*)ops_lo, d_tiles, pair_src,
-                                                     pair_dst, (const float*)src_exp, (float*)dst_exp, Lp,
-                                                     atomic ? 1 : 0);
+        k_gemm_bf16x3<RB, TR><<<ntiles, 256, 0, st>>>((const __bf16*)ops, (const __bf16*)ops_lo, d_tiles, pair_src,
+                                                     pair_dst, (const float*)src_exp, (float*)dst_exp, atomic ? 1 : 0);
     FMM_HIP(hipGetLastError());
+}
+
+template <bool TR>
+static void launch(int RB, Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
+                   const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, bool atomic,
+                   hipStream_t st) {
+#define FMM_RB(R) \
+    case R: launch_rb<R, TR>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, atomic, st); break;
+    switch (RB) {
+        FMM_RB(1) FMM_RB(2) FMM_RB(3) FMM_RB(4) FMM_RB(5) FMM_RB(6) FMM_RB(7)
+        default: throw Error(4, "padded expansion length above 448 (order > 20)");
+    }
+#undef FMM_RB
 }
 
 void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
                 const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, bool atomic,
                 hipStream_t st, bool transpose) {
     if (ntiles == 0) return;
-    if (Lp % 32 != 0) throw Error(1, "padded expansion length must be a multiple of 32");
-    if (transpose) launch<true>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, Lp, atomic, st);
-    else launch<false>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, Lp, atomic, st);
+    if (Lp % 64 != 0) throw Error(1, "padded expansion length must be a multiple of 64");
+    int RB = Lp / 64;
+    if (transpose) launch<true>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, atomic, st);
+    else launch<false>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, atomic, st);
 }
 
 // host-side tiling of class column ranges: class c owns pairs [begin[c], begin[c+1])

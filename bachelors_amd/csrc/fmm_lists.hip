@@ -36,46 +36,67 @@ __device__ __forceinline__ bool far_enough(const TreeView& S, const RadiusTable&
     return __dmul_rn(md, md) < d2;
 }
 
-__device__ __forceinline__ void push(int2* buf, unsigned* cnt, unsigned cap, int a, int b) {
-    unsigned i = atomicAdd(cnt, 1u);
-    if (i < cap) buf[i] = make_int2(a, b);
+// wave-aggregated append: one atomic per wave and buffer
+__device__ __forceinline__ unsigned wave_reserve(unsigned* cnt, unsigned n) {
+    const int lane = threadIdx.x & 63;
+    unsigned incl = n;
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    unsigned total = __shfl(incl, 63, 64);
+    unsigned base = 0;
+    if (lane == 63 && total) base = atomicAdd(cnt, total);
+    base = __shfl(base, 63, 64);
+    return base + incl - n;
 }
 
-// frontier entries are (s, t); far/close outputs are (t, s)
-__global__ void k_lists_step(const int2* __restrict__ front, int nf, TreeView S, RadiusTable RS, TreeView T,
-                             RadiusTable RT, double mac_scale, int2* far_out, unsigned* far_cnt, unsigned far_cap,
-                             int2* close_out, unsigned* close_cnt, unsigned close_cap, int2* next,
-                             unsigned* next_cnt, unsigned next_cap) {
+// frontier entries are (s, t); far/close outputs are (t, s).  Every lane of the wave
+// reaches the appends (threads past the end contribute zero items).
+__global__ __launch_bounds__(256) void k_lists_step(const int2* __restrict__ front, int nf, TreeView S, RadiusTable RS,
+                                                    TreeView T, RadiusTable RT, double mac_scale, int2* far_out,
+                                                    unsigned* far_cnt, unsigned far_cap, int2* close_out,
+                                                    unsigned* close_cnt, unsigned close_cap, int2* next,
+                                                    unsigned* next_cnt, unsigned next_cap) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nf) return;
-    int s = front[i].x, t = front[i].y;
-    if (far_enough(S, RS, s, T, RT, t, mac_scale)) {
-        push(far_out, far_cnt, far_cap, t, s);
-        return;
-    }
-    if (useful(T, t)) {
-        bool close_enough = true;
-        int fc = S.first_child[s];
-        if (fc >= 0)
-            for (int o = 0; o < 8; o++)
-                if (far_enough(S, RS, fc + o, T, RT, t, mac_scale)) { close_enough = false; break; }
-        if (close_enough) {
-            push(close_out, close_cnt, close_cap, t, s);
+    int nfar = 0, nclose = 0, nnext = 0;
+    int s = 0, t = 0, fc = -1;
+    bool desc_s = false;
+    if (i < nf) {
+        s = front[i].x;
+        t = front[i].y;
+        if (far_enough(S, RS, s, T, RT, t, mac_scale)) {
+            nfar = 1;
+        } else if (useful(T, t)) {
+            bool close_enough = true;
+            fc = S.first_child[s];
+            if (fc >= 0)
+                for (int o = 0; o < 8; o++)
+                    if (far_enough(S, RS, fc + o, T, RT, t, mac_scale)) { close_enough = false; break; }
+            if (close_enough) nclose = 1;
+            else desc_s = true;
         } else {
-            for (int o = 0; o < 8; o++)
-                if (nonempty(S, fc + o)) push(next, next_cnt, next_cap, fc + o, t);
+            double rt = RT.r[T.level[t]], rs = RS.r[S.level[s]];
+            if ((rt < rs || useful(T, t)) && !useful(S, s)) { desc_s = true; fc = S.first_child[s]; }
+            else fc = T.first_child[t];
         }
-        return;
+        if (nfar == 0 && nclose == 0) {
+            for (int o = 0; o < 8; o++) nnext += desc_s ? nonempty(S, fc + o) : nonempty(T, fc + o);
+        }
     }
-    double rt = RT.r[T.level[t]], rs = RS.r[S.level[s]];
-    if ((rt < rs || useful(T, t)) && !useful(S, s)) {
-        int fc = S.first_child[s];
-        for (int o = 0; o < 8; o++)
-            if (nonempty(S, fc + o)) push(next, next_cnt, next_cap, fc + o, t);
-    } else {
-        int fc = T.first_child[t];
-        for (int o = 0; o < 8; o++)
-            if (nonempty(T, fc + o)) push(next, next_cnt, next_cap, s, fc + o);
+    unsigned pf = wave_reserve(far_cnt, nfar);
+    unsigned pc = wave_reserve(close_cnt, nclose);
+    unsigned pn = wave_reserve(next_cnt, nnext);
+    if (nfar && pf < far_cap) far_out[pf] = make_int2(t, s);
+    if (nclose && pc < close_cap) close_out[pc] = make_int2(t, s);
+    if (nnext) {
+        for (int o = 0; o < 8; o++) {
+            if (desc_s) {
+                if (nonempty(S, fc + o)) { if (pn < next_cap) next[pn] = make_int2(fc + o, t); pn++; }
+            } else {
+                if (nonempty(T, fc + o)) { if (pn < next_cap) next[pn] = make_int2(s, fc + o); pn++; }
+            }
+        }
     }
 }
 
@@ -89,7 +110,7 @@ __global__ void k_far_keys(const int2* __restrict__ far, int n, TreeView S, Tree
     uint32_t code = 0;
     if (use_classes) {
         int ls = S.level[s], lt = T.level[t];
-        double hf = ls > lt ? hs[ls] : ht[lt];
+        double hf = fmin(hs[ls], ht[lt]);   // finer of the two boxes (physical size, not tree level)
         int nx = (int)rint((S.center[3 * s] - T.center[3 * t]) / hf);
         int ny = (int)rint((S.center[3 * s + 1] - T.center[3 * t + 1]) / hf);
         int nz = (int)rint((S.center[3 * s + 2] - T.center[3 * t + 2]) / hf);

@@ -187,3 +187,32 @@ void make_m2l_specs(const int* class_begin, int nc, const int* far_t, const int*
 }
 
 }  // namespace fmm
+
+namespace fmm {
+
+// one operator per (src, dst) node pair: t = c_src - c_dst (M2L: source - target,
+// L2L: parent - child, M2M: child - parent)
+__global__ void k_pair_specs(const int* __restrict__ src, const int* __restrict__ dst, int n,
+                             const double* __restrict__ csrc, const double* __restrict__ cdst,
+                             const int* __restrict__ lsrc, const int* __restrict__ ldst, ScaleTable sig_src,
+                             ScaleTable sig_dst, int kind, int m2l_full, OpSpec* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = src[i], d = dst[i];
+    OpSpec o;
+    for (int a = 0; a < 3; a++) o.t[a] = csrc[3 * s + a] - cdst[3 * d + a];
+    o.sig_in = sig_src.s[lsrc[s]];
+    o.sig_out = sig_dst.s[ldst[d]];
+    o.kind = kind;
+    o.m2l_full = m2l_full;
+    out[i] = o;
+}
+
+void make_pair_specs(const int* src, const int* dst, int n, const double* csrc, const double* cdst, const int* lsrc,
+                     const int* ldst, const ScaleTable& sig_src, const ScaleTable& sig_dst, int kind, int m2l_full,
+                     OpSpec* out, hipStream_t st) {
+    if (n) k_pair_specs<<<(n + 255) / 256, 256, 0, st>>>(src, dst, n, csrc, cdst, lsrc, ldst, sig_src, sig_dst, kind,
+                                                         m2l_full, out);
+}
+
+}  // namespace fmm

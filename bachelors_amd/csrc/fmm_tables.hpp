@@ -22,5 +22,8 @@ void operators_to_bf16x2(const double* in, __bf16* hi, __bf16* lo, size_t n, hip
 void make_m2l_specs(const int* class_begin, int nc, const int* far_t, const int* far_s, const double* sc,
                     const double* tc, const int* slev, const int* tlev, const ScaleTable& sigS, const ScaleTable& sigT,
                     int m2l_full, OpSpec* out, hipStream_t st);
+void make_pair_specs(const int* src, const int* dst, int n, const double* csrc, const double* cdst, const int* lsrc,
+                     const int* ldst, const ScaleTable& sig_src, const ScaleTable& sig_dst, int kind, int m2l_full,
+                     OpSpec* out, hipStream_t st);
 
 }  // namespace fmm

@@ -115,13 +115,22 @@ __global__ void k_emit_children(const int* __restrict__ idx, const int* __restri
         }
 }
 
-__global__ void k_key_to_node(const uint32_t* __restrict__ key, int n, int child_lb, int* __restrict__ node,
-                              int* __restrict__ arrivals) {
+// keys are sorted: each child's arrivals are the length of its run of equal keys
+__global__ void k_key_runs(const uint32_t* __restrict__ key, int n, int child_lb, int* __restrict__ node,
+                           int* __restrict__ arrivals, int* __restrict__ run_start) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int c = child_lb + (int)key[i];
+    uint32_t k = key[i];
+    int c = child_lb + (int)k;
     node[i] = c;
-    atomicAdd(&arrivals[c], 1);
+    if (i == 0 || key[i - 1] != k) run_start[c] = i;
+    if (i == n - 1 || key[i + 1] != k) arrivals[c] = i + 1;   // run end; children are created with 0
+}
+__global__ void k_run_lengths(int* __restrict__ arrivals, const int* __restrict__ run_start, int lb, int nn) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    int c = lb + i;
+    if (arrivals[c] > 0) arrivals[c] -= run_start[c];
 }
 
 // bottom-up subtree totals (leaf: arrivals)
@@ -300,7 +309,9 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
         int bits = 1;
         while ((1 << bits) < nchild) bits++;
         sort_pairs_u32(key, key2, val, idx1, nk1, bits, S.tmp, st);
-        k_key_to_node<<<blocks(nk1), 256, 0, st>>>(key2, nk1, child_lb, nd1, T.arrivals.as<int>());
+        int* rs = S.total.get<int>(new_nodes);
+        k_key_runs<<<blocks(nk1), 256, 0, st>>>(key2, nk1, child_lb, nd1, T.arrivals.as<int>(), rs);
+        k_run_lengths<<<blocks(nchild), 256, 0, st>>>(T.arrivals.as<int>(), rs, child_lb, nchild);
         S.lvl_n[k + 1] = nk1;
     }
     T.num_levels = k + 1;   // levels 0..k hold nodes

@@ -18,7 +18,7 @@ int main() {
     }
     std::vector<Vector3> points;
     for (int i = 0; i < 512; i++) points.emplace_back(4 * U(rng) - 2, 4 * U(rng) - 2, 4 * U(rng) - 2);
-    FastMultipoleSolver s(quads, points, Problem::BioSavartLaplace, 32, 32, 8);
+    FastMultipoleSolver s(quads, points, Problem::BioSavartLaplace, 32, 32, 12);
     s.calcMultipoleExpansionsAtLeaves();
     s.calclMultipoleExpansions(M2MAlg::Matrices, Device::GPU);
     s.calcLocalMultipoleExpansions(M2LAlg::ComplexTranslation, Device::CPU);

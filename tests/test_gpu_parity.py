@@ -80,12 +80,7 @@ def test_interaction_lists_bit_exact(fm, orc, frame):
     (oft, ofs), (oct_, ocs) = orc.interaction_lists(S, T)
     F.leaf_expansions()
     F.upward()
-    if frame == 1:
-        F.downward()
-    else:
-        # the reference frame builds the lists, then stops: its M2L has no shared operators
-        with pytest.raises(fm.FMMError):
-            F.downward()
+    F.downward()
     (ft, fs), (ct, cs), ncls = F.lists()
     assert (ncls > 0) == (frame == 1)
     assert set(zip(ft.tolist(), fs.tolist())) == set(zip(oft.tolist(), ofs.tolist()))
@@ -188,3 +183,21 @@ def test_direct_sum_gpu_vs_oracle(fm, orc):
     out = torch.zeros_like(t)
     F.direct_B_dev(2.0, len(tgt), ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(out.data_ptr()))
     assert rel_l2(out.cpu().numpy(), Bo) < 1e-13
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 3e-6)])
+def test_reference_frame_fmm_matches_oracle(fm, orc, precision, tol):
+    """The whole FMM on the reference's own trees (bounding-box target root, float-rounded
+    centres, per-pair M2L/L2L operators) against the oracle on the same trees."""
+    src, w, tgt = _inputs(13, 3000, 800)
+    F = fm.FMM(order=8, source_leaf_capacity=32, target_leaf_capacity=32, precision=precision, frame=0)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    B = F.run(1.0)
+    R = orc.FMMRun(src, w, tgt, p=8, cap_s=32, cap_t=32, flags=orc.CORRECTED)
+    Bf, Bn = F.B_parts()
+    assert rel_l2(Bf + Bn, R.Bfar + R.Bnear) < tol
+    Bo = R.B_input_order(len(tgt))
+    ok = np.isfinite(Bo[:, 0])
+    assert np.array_equal(np.isfinite(B[:, 0]), ok)
+    assert rel_l2(B[ok], Bo[ok]) < tol
