@@ -215,7 +215,6 @@ __global__ void k_weights(const int* __restrict__ slots, int n, const double* __
     dLdw[slots[k]] = sJ[3 * k] * gq[3 * k] + sJ[3 * k + 1] * gq[3 * k + 1] + sJ[3 * k + 2] * gq[3 * k + 2];
 }
 
-static inline int blocks(long long n, int b = 256) { return (int)((n + b - 1) / b); }
 
 void run_adjoint(AdjointState& S, const AdjointArgs& a, const double* dLdB, double* dLdw, hipStream_t st) {
     const TreeDev& Ts = *a.S;
@@ -245,18 +244,24 @@ void run_adjoint(AdjointState& S, const AdjointArgs& a, const double* dLdB, doub
     for (int k = Tt.num_levels - 1; k >= 1; k--) {
         const PassView& v = a.l2l_levels[k];
         if (v.ntiles)
-            class_gemm(a.prec, a.down_ops, a.down_ops_lo, v.tiles, v.ntiles, v.dst, v.src, lam, lam, a.Lp, true, st,
-                       true);
+            class_gemm(a.prec, a.down_ops, a.down_ops_lo, v.tiles, v.ntiles, v.dst, v.src, lam, lam, a.Lp, EPI_ATOMIC,
+                       st, true);
     }
-    // M2L^T (forward far_s -> far_t; here far_t -> far_s)
-    if (a.m2l.ntiles)
-        class_gemm(a.prec, a.down_ops, a.down_ops_lo, a.m2l.tiles, a.m2l.ntiles, a.m2l.src, a.m2l.dst, lam, mu, a.Lp,
-                   true, st, true);
+    // M2L^T (forward far_s -> far_t; here far_t -> far_s): one slab per (pair, component), then
+    // each source node sums its pairs' slabs in pair order
+    if (a.m2l.ntiles) {
+        const int npairs = a.lists->num_far;
+        build_pair_index(a.m2l.dst, npairs, Ts.num_nodes, S.m2l_ix, st);
+        void* slab = S.slab.reserve((size_t)3 * npairs * a.Lp * es);
+        class_gemm(a.prec, a.down_ops, a.down_ops_lo, a.m2l.tiles, a.m2l.ntiles, a.m2l.src, a.m2l.dst, lam, slab,
+                   a.Lp, EPI_STORE, st, true);
+        reduce_pairs(f64, S.m2l_ix, slab, a.Lp, mu, st);
+    }
     // M2M^T top-down (forward child -> parent; here parent -> child, one write per child)
     for (int k = 1; k < Ts.num_levels; k++) {
         const PassView& v = a.m2m_levels[k];
         if (v.ntiles)
-            class_gemm(a.prec, a.up_ops, a.up_ops_lo, v.tiles, v.ntiles, v.dst, v.src, mu, mu, a.Lp, false, st, true);
+            class_gemm(a.prec, a.up_ops, a.up_ops_lo, v.tiles, v.ntiles, v.dst, v.src, mu, mu, a.Lp, EPI_ADD, st, true);
     }
     // P2M^T
     double* gq = S.gq.get<double>(3 * (size_t)std::max(nss, 1));

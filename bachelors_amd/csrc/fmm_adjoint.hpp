@@ -39,6 +39,8 @@ struct AdjointArgs {
 
 struct AdjointState {
     DevBuf g, lam, mu, gq, Gexp;
+    PairIndex m2l_ix;   // far pairs by source node
+    DevBuf slab;        // transposed M2L result per (pair, component)
 };
 
 // dLdw (input order) = transposed FMM applied to dLdB (input order); device pointers.

@@ -24,7 +24,7 @@ using namespace fmm;
 namespace {
 
 enum Stage { ST_EMPTY = 0, ST_TREES = 1, ST_LEAVES = 2, ST_UP = 3, ST_DOWN = 4, ST_EVAL = 5 };
-enum Ev { E_START, E_TREE, E_P2M, E_M2M, E_LISTS, E_TABLES, E_M2L, E_L2L, E_L2P, E_P2P, E_END, E_COUNT };
+enum Ev { E_START, E_TREE, E_P2M, E_M2M, E_LISTS, E_TABLES, E_M2L, E_L2L, E_L2P, E_P2P, E_END, E_M2L_GEMM, E_COUNT };
 
 struct Pass {               // one batched GEMM launch: tiles over (src, dst) pairs
     int ntiles = 0, npairs = 0;
@@ -55,6 +55,7 @@ struct fmm_ctx {
     TreeScratch tscr;
     ScaleTable sigS, sigT;
     DevBuf sxyz, sq, sJ, txyz;                   // slot order
+    DevBuf nP0, nP1, nP2;                        // near-field sources packed per leaf (slot order)
     std::vector<int> h_src_leaves, h_tgt_leaves;
     DevBuf src_leaves, tgt_leaves;
     // expansions [node][3][Lp]
@@ -65,6 +66,8 @@ struct fmm_ctx {
     Pass m2l_pass;
     Lists lists;
     ListScratch lscr;
+    PairIndex m2l_ix;                            // far pairs by target node
+    DevBuf m2l_slab;                             // M2L result per (pair, component)
     // evaluation
     DevBuf Bfar, Bnear, Bout;
     double current = 1.0;
@@ -137,10 +140,10 @@ const void* ops_ptr(const fmm_ctx* c, const OpTables& O) {
     return O.hi.ptr;
 }
 
-void run_pass(fmm_ctx* c, const OpTables& O, const Pass& P, const void* src_exp, void* dst_exp, bool atomic) {
+void run_pass(fmm_ctx* c, const OpTables& O, const Pass& P, const void* src_exp, void* dst_exp, int epilogue) {
     if (!P.ntiles) return;
     class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, P.src.as<int>(), P.dst.as<int>(),
-               src_exp, dst_exp, c->Lp, atomic, c->stream);
+               src_exp, dst_exp, c->Lp, epilogue, c->stream);
 }
 
 void stage_trees(fmm_ctx* c) {
@@ -217,6 +220,11 @@ void stage_leaves(fmm_ctx* c) {
     FMM_HIP(hipMemsetAsync(M, 0, n * c->esize(), c->stream));
     launch_p2m(c->f64(), c->src_leaves.as<int>(), (int)c->h_src_leaves.size(), c->S, c->sigS, c->sxyz.as<double>(),
                c->sq.as<double>(), c->p, c->Lp, M, c->stream);
+    size_t nss = (size_t)std::max(c->S.num_slots, 1), rs = c->f64() ? sizeof(double) : sizeof(float);
+    // origin = source root centre (node 0); the P2P kernel reads the same three doubles
+    launch_pack_sources(c->f64(), c->S.num_slots, c->S.center.as<double>(), c->sxyz.as<double>(), c->sq.as<double>(),
+                        c->nP0.reserve(4 * rs * nss), c->nP1.reserve(4 * rs * nss), c->nP2.reserve(rs * nss),
+                        c->stream);
     c->rec(E_P2M);
     c->stage = ST_LEAVES;
 }
@@ -263,7 +271,7 @@ void stage_upward(fmm_ctx* c) {
         std::vector<GemmTile> tiles;
         plan_tiles(cb, cop, tiles);
         upload_pass(c->m2m_pass[k], all_s, all_d, tiles, c->stream);
-        run_pass(c, O, c->m2m_pass[k], c->M.ptr, c->M.ptr, true);
+        run_pass(c, O, c->m2m_pass[k], c->M.ptr, c->M.ptr, EPI_ATOMIC);
     }
     c->rec(E_M2M);
     c->stage = ST_UP;
@@ -293,7 +301,8 @@ void run_per_pair(fmm_ctx* c, int kind, const int* d_src, const int* d_dst, int 
         for (int i = 0; i < n; i++) tiles[i] = GemmTile{i, 3 * (b + i), 3};
         GemmTile* dt = c->m2l_pass.tiles.get<GemmTile>(n);
         FMM_HIP(hipMemcpyAsync(dt, tiles.data(), n * sizeof(GemmTile), hipMemcpyHostToDevice, st));
-        class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, dt, n, d_src, d_dst, src_exp, dst_exp, c->Lp, atomic, st);
+        class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, dt, n, d_src, d_dst, src_exp, dst_exp, c->Lp,
+                   atomic ? EPI_ATOMIC : EPI_ADD, st);
         FMM_HIP(hipStreamSynchronize(st));   // the chunk buffers are reused
     }
 }
@@ -340,6 +349,8 @@ void stage_downward(fmm_ctx* c) {
     const bool shared = c->cfg.frame == FMM_FRAME_SHARED;
     build_lists(c->lists, c->lscr, S, T, c->cfg.mac_scale, shared, st);
     c->lists_ready = true;
+    // far pairs grouped by target node, for the atomic-free M2L accumulation
+    if (shared) build_pair_index(c->lists.far_t.as<int>(), c->lists.num_far, T.num_nodes, c->m2l_ix, st);
     c->rec(E_LISTS);
     if (!shared) {
         downward_per_pair(c);
@@ -373,8 +384,9 @@ void stage_downward(fmm_ctx* c) {
     size_t n = (size_t)T.num_nodes * 3 * c->Lp;
     void* Lx = c->Lx.reserve(n * c->esize());
     FMM_HIP(hipMemsetAsync(Lx, 0, n * c->esize(), st));
-    // M2L: all classes in one launch (fast_multipole_solver.cpp:258-276); the events
-    // around it bracket the GEMM alone
+    // M2L: all classes in one launch (fast_multipole_solver.cpp:258-276) writing one slab per
+    // (pair, component), then each target node sums its pairs' slabs in pair order (no atomics,
+    // deterministic); E_TABLES .. E_M2L_GEMM brackets the GEMM alone
     Pass& P = c->m2l_pass;
     {
         std::vector<int> cop(nc);
@@ -388,9 +400,15 @@ void stage_downward(fmm_ctx* c) {
                                    hipMemcpyHostToDevice, st));
     }
     c->rec(E_TABLES);
-    if (P.ntiles)
+    if (P.ntiles) {
+        void* slab = c->m2l_slab.reserve((size_t)3 * P.npairs * c->Lp * c->esize());
         class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, c->lists.far_s.as<int>(),
-                   c->lists.far_t.as<int>(), c->M.ptr, Lx, c->Lp, true, st);
+                   c->lists.far_t.as<int>(), c->M.ptr, slab, c->Lp, EPI_STORE, st);
+        c->rec(E_M2L_GEMM);
+        reduce_pairs(c->f64(), c->m2l_ix, slab, c->Lp, Lx, st);
+    } else {
+        c->rec(E_M2L_GEMM);
+    }
     c->rec(E_M2L);
     // L2L top-down (calculation_point_octree.cpp:277-299)
     c->l2l_pass.resize(MAX_LEVELS);
@@ -412,7 +430,7 @@ void stage_downward(fmm_ctx* c) {
         std::vector<GemmTile> tiles;
         plan_tiles(cb, cop, tiles);
         upload_pass(c->l2l_pass[k], all_s, all_d, tiles, st);
-        run_pass(c, O, c->l2l_pass[k], Lx, Lx, false);
+        run_pass(c, O, c->l2l_pass[k], Lx, Lx, EPI_ADD);
     }
     c->rec(E_L2L);
     c->stage = ST_DOWN;
@@ -434,7 +452,7 @@ void stage_evaluate(fmm_ctx* c, double current, double* Bdev) {
     c->rec(E_L2P);
     if (c->cfg.near_field) {
         launch_p2p(c->f64(), c->tgt_leaves.as<int>(), (int)c->h_tgt_leaves.size(), c->T, c->S, c->lists.close_begin.as<int>(),
-                   c->lists.close_end.as<int>(), c->lists.close_s.as<int>(), c->sxyz.as<double>(), c->sq.as<double>(),
+                   c->lists.close_end.as<int>(), c->lists.close_s.as<int>(), c->nP0.ptr, c->nP1.ptr, c->nP2.ptr,
                    c->txyz.as<double>(), Bnear, st);
     } else {
         FMM_HIP(hipMemsetAsync(Bnear, 0, 3 * (size_t)std::max(nts, 1) * sizeof(double), st));
@@ -746,6 +764,7 @@ int fmm_get_stats(fmm_ctx* c, fmm_stats* s) {
             s->t_l2p = el(E_L2L, E_L2P);
             s->t_p2p = el(E_L2P, E_P2P);
             s->t_total = el(E_START, E_END);
+            s->t_m2l_gemm = c->cfg.frame == FMM_FRAME_SHARED ? el(E_TABLES, E_M2L_GEMM) : s->t_m2l;
         }
     });
 }

@@ -74,6 +74,9 @@ struct ScaleTable {
 // ---- per-tree device description -------------------------------------------------
 // Node arrays use canonical BFS numbering: level-k nodes are contiguous, and the 8
 // children of a subdivided node are consecutive ids in reference child order.
+// grid size for n work items in workgroups of b
+inline int blocks(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
 struct TreeDev {
     int num_nodes = 0;
     int num_slots = 0;                 // points kept by the tree (leaf DFS order)

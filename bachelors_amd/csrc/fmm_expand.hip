@@ -217,83 +217,137 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
 }
 
 // ---------------------------------------------------------------- P2P
-// One workgroup per useful target leaf; source blocks of 64 are staged in LDS as offsets
-// from the target-leaf centre (formed in FP64), one target per lane.  R = float: FP32
-// pair math with FP64 accumulation per target; R = double: FP64 throughout.
+// Near field B(x) = sum_j q_j x (x - y_j) / |x - y_j|^3 over the sources of each close
+// source node (the close-list loop of FastMultipoleSolver::calcB, fast_multipole_solver.cpp:214-229;
+// a close node may be internal, then all its subtree's sources are summed).
+//
+// Positions are taken relative to one origin o (the source root centre).  For R = float both
+// sides are unevaluated sums hi + lo of two floats formed in FP64 (~2^-48 of the root size),
+// so the pair difference d = (x_hi - y_hi) + (x_lo - y_lo) stays accurate for very close
+// pairs (x_hi - y_hi is exact when the two are within a factor 2) while the pair arithmetic
+// runs in FP32.  R = double keeps the offsets whole.
 template <class R> struct V4;
-template <> struct V4<float> { typedef float4 t; static __device__ float4 make(float a, float b, float c) { return make_float4(a, b, c, 0.f); } };
-template <> struct V4<double> { typedef double4 t; static __device__ double4 make(double a, double b, double c) { return make_double4(a, b, c, 0.0); } };
+template <> struct V4<float> {
+    typedef float4 t;
+    static __device__ float4 make(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+};
+template <> struct V4<double> {
+    typedef double4 t;
+    static __device__ double4 make(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
+};
 __device__ __forceinline__ float rsqrt_t(float x) { return rsqrtf(x); }
 __device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
 
-// Coordinates are offsets from the target-leaf centre formed in FP64 and kept as an
-// unevaluated sum hi + lo of two R values, so the pair difference
-// d = (x_hi - y_hi) + (x_lo - y_lo) stays accurate for very close pairs.
+// P0[j] = (hi(y_j - o), q_x), P1[j] = (lo(y_j - o), q_y), P2[j] = q_z for every source slot j
 template <class R>
-__global__ __launch_bounds__(64) void k_p2p(const int* __restrict__ leaves, const double* __restrict__ tcenter,
-                                            const int* __restrict__ tbegin, const int* __restrict__ tend,
-                                            const int* __restrict__ close_begin, const int* __restrict__ close_end,
-                                            const int* __restrict__ close_s, const int* __restrict__ sbegin,
-                                            const int* __restrict__ send, const double* __restrict__ sxyz,
-                                            const double* __restrict__ sq, const double* __restrict__ txyz,
-                                            double* __restrict__ Bnear) {
+__global__ void k_pack_sources(int n, const double* __restrict__ origin, const double* __restrict__ sxyz,
+                               const double* __restrict__ sq, typename V4<R>::t* __restrict__ P0,
+                               typename V4<R>::t* __restrict__ P1, R* __restrict__ P2) {
+    constexpr bool split = sizeof(R) == 4;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double dx = sxyz[3 * j] - origin[0], dy = sxyz[3 * j + 1] - origin[1], dz = sxyz[3 * j + 2] - origin[2];
+    R hx = (R)dx, hy = (R)dy, hz = (R)dz;
+    R lx = split ? (R)(dx - (double)hx) : (R)0, ly = split ? (R)(dy - (double)hy) : (R)0;
+    R lz = split ? (R)(dz - (double)hz) : (R)0;
+    P0[j] = V4<R>::make(hx, hy, hz, (R)sq[3 * j]);
+    P1[j] = V4<R>::make(lx, ly, lz, (R)sq[3 * j + 1]);
+    P2[j] = (R)sq[3 * j + 2];
+}
+
+// One workgroup (4 waves) per target leaf.  The leaf's close list is dealt round-robin to the
+// waves; each wave stages a close node's sources (64 per pass) in its own LDS slice and
+// its lanes sweep it, G = 64 / 2^ceil(log2 n) lanes per target for a leaf of n <= 64
+// targets (lane = target * G + g takes sources g, g + G, ...).  Lane partials are summed
+// over the G lanes, then over the 4 waves in a fixed order, and written once.
+template <class R>
+__global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, const int* __restrict__ tbegin, const int* __restrict__ tend,
+                                             const int* __restrict__ close_begin, const int* __restrict__ close_end,
+                                             const int* __restrict__ close_s, const double* __restrict__ origin,
+                                             const int* __restrict__ sbegin, const int* __restrict__ send,
+                                             const typename V4<R>::t* __restrict__ P0,
+                                             const typename V4<R>::t* __restrict__ P1, const R* __restrict__ P2,
+                                             const double* __restrict__ txyz, double* __restrict__ Bnear) {
     typedef typename V4<R>::t v4;
     constexpr bool split = sizeof(R) == 4;
-    __shared__ v4 shi[64];
-    __shared__ v4 slo[64];
-    __shared__ v4 sqv[64];
+    __shared__ v4 s0[4][64];
+    __shared__ v4 s1[4][64];
+    __shared__ R s2[4][64];
+    __shared__ double red[4][3][64];
     const int leaf = leaves[blockIdx.x];
-    const int lane = threadIdx.x;
-    const double cx = tcenter[3 * leaf], cy = tcenter[3 * leaf + 1], cz = tcenter[3 * leaf + 2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tb = tbegin[leaf], te = tend[leaf];
     const int cb = close_begin[leaf], ce = close_end[leaf];
-    for (int k0 = tbegin[leaf]; k0 < tend[leaf]; k0 += 64) {
-        int k = k0 + lane;
-        bool valid = k < tend[leaf];
-        R xh = 0, yh = 0, zh = 0, xl = 0, yl = 0, zl = 0;
+    if (cb >= ce) {
+        for (int k = tb + tid; k < te; k += 256) Bnear[3 * k] = Bnear[3 * k + 1] = Bnear[3 * k + 2] = 0.0;
+        return;
+    }
+    for (int k0 = tb; k0 < te; k0 += 64) {
+        const int n = min(64, te - k0);
+        const int lgG = 6 - (n <= 1 ? 0 : 32 - __clz(n - 1));
+        const int G = 1 << lgG;
+        const int ti = lane >> lgG, g = lane & (G - 1);
+        const bool valid = ti < n;
+        double ux = 0, uy = 0, uz = 0;
         if (valid) {
-            double dx = txyz[3 * k] - cx, dy = txyz[3 * k + 1] - cy, dz = txyz[3 * k + 2] - cz;
-            xh = (R)dx; yh = (R)dy; zh = (R)dz;
-            if (split) { xl = (R)(dx - (double)xh); yl = (R)(dy - (double)yh); zl = (R)(dz - (double)zh); }
+            ux = txyz[3 * (k0 + ti)] - origin[0];
+            uy = txyz[3 * (k0 + ti) + 1] - origin[1];
+            uz = txyz[3 * (k0 + ti) + 2] - origin[2];
         }
+        const R uhx = (R)ux, uhy = (R)uy, uhz = (R)uz;
+        const R ulx = split ? (R)(ux - (double)uhx) : (R)0, uly = split ? (R)(uy - (double)uhy) : (R)0;
+        const R ulz = split ? (R)(uz - (double)uhz) : (R)0;
         double a0 = 0, a1 = 0, a2 = 0;
-        for (int ci = cb; ci < ce; ci++) {
-            int s = close_s[ci];
-            int sb = sbegin[s], se = send[s];
+        for (int ci = cb + wave; ci < ce; ci += 4) {
+            const int s = close_s[ci];
+            const int sb = sbegin[s], se = send[s];
             for (int j0 = sb; j0 < se; j0 += 64) {
-                int j = j0 + lane;
-                __syncthreads();
-                if (j < se) {
-                    double dx = sxyz[3 * j] - cx, dy = sxyz[3 * j + 1] - cy, dz = sxyz[3 * j + 2] - cz;
-                    R hx = (R)dx, hy = (R)dy, hz = (R)dz;
-                    shi[lane] = V4<R>::make(hx, hy, hz);
-                    if (split) slo[lane] = V4<R>::make((R)(dx - (double)hx), (R)(dy - (double)hy), (R)(dz - (double)hz));
-                    sqv[lane] = V4<R>::make((R)sq[3 * j], (R)sq[3 * j + 1], (R)sq[3 * j + 2]);
+                const int nj = min(64, se - j0);
+                __syncwarp();
+                if (lane < nj) {
+                    s0[wave][lane] = P0[j0 + lane];
+                    s1[wave][lane] = P1[j0 + lane];
+                    s2[wave][lane] = P2[j0 + lane];
                 }
-                __syncthreads();
-                int nj = min(64, se - j0);
+                __syncwarp();
                 R f0 = 0, f1 = 0, f2 = 0;
-                for (int jj = 0; jj < nj; jj++) {
-                    v4 y = shi[jj], q = sqv[jj];
-                    R dx = xh - y.x, dy = yh - y.y, dz = zh - y.z;
+                for (int jj = g; jj < nj; jj += G) {
+                    const v4 yh = s0[wave][jj];
+                    const v4 yl = s1[wave][jj];
+                    const R qx = yh.w, qy = yl.w, qz = s2[wave][jj];
+                    R dx = uhx - yh.x, dy = uhy - yh.y, dz = uhz - yh.z;
                     if (split) {
-                        v4 yl4 = slo[jj];
-                        dx += xl - yl4.x; dy += yl - yl4.y; dz += zl - yl4.z;
+                        dx += ulx - yl.x;
+                        dy += uly - yl.y;
+                        dz += ulz - yl.z;
                     }
-                    R r2 = dx * dx + dy * dy + dz * dz;
-                    R ri = r2 > (R)0 ? rsqrt_t(r2) : (R)0;
-                    R ri3 = ri * ri * ri;
-                    f0 += (q.y * dz - q.z * dy) * ri3;
-                    f1 += (q.z * dx - q.x * dz) * ri3;
-                    f2 += (q.x * dy - q.y * dx) * ri3;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    const R ri = r2 > (R)0 ? rsqrt_t(r2) : (R)0;
+                    const R ri3 = ri * ri * ri;
+                    f0 += (qy * dz - qz * dy) * ri3;
+                    f1 += (qz * dx - qx * dz) * ri3;
+                    f2 += (qx * dy - qy * dx) * ri3;
                 }
-                a0 += f0; a1 += f1; a2 += f2;
+                a0 += f0;
+                a1 += f1;
+                a2 += f2;
             }
         }
-        if (valid) {
-            Bnear[3 * k] = a0;
-            Bnear[3 * k + 1] = a1;
-            Bnear[3 * k + 2] = a2;
+        for (int off = 1; off < G; off <<= 1) {
+            a0 += __shfl_xor(a0, off);
+            a1 += __shfl_xor(a1, off);
+            a2 += __shfl_xor(a2, off);
         }
+        red[wave][0][lane] = a0;
+        red[wave][1][lane] = a1;
+        red[wave][2][lane] = a2;
+        __syncthreads();
+        if (wave == 0 && valid && g == 0) {
+            const int k = k0 + ti;
+            for (int c = 0; c < 3; c++)
+                Bnear[3 * k + c] = ((red[0][c][lane] + red[1][c][lane]) + red[2][c][lane]) + red[3][c][lane];
+        }
+        __syncthreads();
     }
 }
 
@@ -344,7 +398,6 @@ __global__ __launch_bounds__(256) void k_direct(const double* __restrict__ sxyz,
     }
 }
 
-static inline int blocks(long long n, int b = 256) { return (int)((n + b - 1) / b); }
 
 void launch_charges(const int* slots, int n, const double* xyz, const double* w, const double* J, double* sxyz,
                     double* sq, double* sJ, hipStream_t st) {
@@ -379,18 +432,29 @@ void launch_l2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, cons
                                                T.level.as<int>(), sig, txyz, (const float*)Lexp, p, Lp, Bfar);
     FMM_HIP(hipGetLastError());
 }
+void launch_pack_sources(bool f64, int n, const double* origin, const double* sxyz, const double* sq, void* P0,
+                         void* P1, void* P2, hipStream_t st) {
+    if (!n) return;
+    if (f64)
+        k_pack_sources<double><<<blocks(n), 256, 0, st>>>(n, origin, sxyz, sq, (double4*)P0, (double4*)P1, (double*)P2);
+    else
+        k_pack_sources<float><<<blocks(n), 256, 0, st>>>(n, origin, sxyz, sq, (float4*)P0, (float4*)P1, (float*)P2);
+    FMM_HIP(hipGetLastError());
+}
 void launch_p2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
-                const int* close_end, const int* close_s, const double* sxyz, const double* sq, const double* txyz,
-                double* Bnear, hipStream_t st) {
+                const int* close_end, const int* close_s, const void* P0, const void* P1, const void* P2,
+                const double* txyz, double* Bnear, hipStream_t st) {
     if (!nleaves) return;
     if (f64)
-        k_p2p<double><<<nleaves, 64, 0, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
-                                              close_begin, close_end, close_s, S.begin.as<int>(), S.end.as<int>(), sxyz,
-                                              sq, txyz, Bnear);
+        k_p2p<double><<<nleaves, 256, 0, st>>>(leaves, T.begin.as<int>(), T.end.as<int>(),
+                                               close_begin, close_end, close_s, S.center.as<double>(),
+                                               S.begin.as<int>(), S.end.as<int>(), (const double4*)P0,
+                                               (const double4*)P1, (const double*)P2, txyz, Bnear);
     else
-        k_p2p<float><<<nleaves, 64, 0, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
-                                             close_begin, close_end, close_s, S.begin.as<int>(), S.end.as<int>(), sxyz,
-                                             sq, txyz, Bnear);
+        k_p2p<float><<<nleaves, 256, 0, st>>>(leaves, T.begin.as<int>(), T.end.as<int>(),
+                                              close_begin, close_end, close_s, S.center.as<double>(),
+                                              S.begin.as<int>(), S.end.as<int>(), (const float4*)P0,
+                                              (const float4*)P1, (const float*)P2, txyz, Bnear);
     FMM_HIP(hipGetLastError());
 }
 void launch_assemble(const int* slots, int n, const double* Bfar, const double* Bnear, double scale, double* B,

@@ -10,9 +10,13 @@ void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, cons
                 const double* sq, int p, int Lp, void* M, hipStream_t st);
 void launch_l2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig, const double* txyz,
                 const void* Lexp, int p, int Lp, double* Bfar, hipStream_t st);
+// near-field sources in slot order relative to origin[3] (a device pointer); P0, P1: 4 x R per
+// slot, P2: R per slot, R = double if f64 else float
+void launch_pack_sources(bool f64, int n, const double* origin, const double* sxyz, const double* sq, void* P0,
+                         void* P1, void* P2, hipStream_t st);
 void launch_p2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
-                const int* close_end, const int* close_s, const double* sxyz, const double* sq, const double* txyz,
-                double* Bnear, hipStream_t st);
+                const int* close_end, const int* close_s, const void* P0, const void* P1, const void* P2,
+                const double* txyz, double* Bnear, hipStream_t st);
 void launch_assemble(const int* slots, int n, const double* Bfar, const double* Bnear, double scale, double* B,
                      hipStream_t st);
 void launch_direct(const double* sxyz, const double* sq, int ns, const double* txyz, int nt, double scale, double* B,

@@ -19,6 +19,7 @@
 //            B = Bh + Bl (split on load): C += Al Bh + Ah Bl + Ah Bh, FP32 accumulate.
 // TRANS applies T^T (the adjoint passes) by reading the operator column-wise.
 #include "fmm_gemm.hpp"
+#include "fmm_prims.hpp"
 
 namespace fmm {
 
@@ -29,12 +30,12 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int BN = 64;
 
 // hardware float atomics (global_atomic_add_f32/f64) on coarse-grained device memory
-__device__ __forceinline__ void add_out(float* p, float v, int atomic) {
-    if (atomic) unsafeAtomicAdd(p, v);
+__device__ __forceinline__ void add_out(float* p, float v, int epi) {
+    if (epi == EPI_ATOMIC) unsafeAtomicAdd(p, v);
     else *p += v;
 }
-__device__ __forceinline__ void add_out(double* p, double v, int atomic) {
-    if (atomic) unsafeAtomicAdd(p, v);
+__device__ __forceinline__ void add_out(double* p, double v, int epi) {
+    if (epi == EPI_ATOMIC) unsafeAtomicAdd(p, v);
     else *p += v;
 }
 
@@ -54,7 +55,7 @@ template <> struct Vec4<double> {
 template <class T, int RB, bool TRANS>
 __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const GemmTile* __restrict__ tiles,
                                               const int* __restrict__ pair_src, const int* __restrict__ pair_dst,
-                                              const T* __restrict__ src_exp, T* __restrict__ dst_exp, int atomic) {
+                                              const T* __restrict__ src_exp, T* __restrict__ dst_exp, int epi) {
     constexpr int Lp = 64 * RB, BK = 16, SA = Lp + 4, SB = BN + 4, NK = Lp / BK;
     typedef Vec4<T> V;
     typedef typename V::t v4;
@@ -141,6 +142,20 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
         int cl = b * 16 + (lane & 15);
         if (cl >= tile.ncol) continue;
         int col = tile.col0 + cl;
+        if (epi == EPI_STORE) {
+            T* out = dst_exp + (size_t)col * Lp + wrow;
+#pragma unroll
+            for (int a = 0; a < RB; a++) {
+                if constexpr (sizeof(T) == 8) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) out[a * 16 + 4 * r + (lane >> 4)] = acc[a][b][r];
+                } else {
+                    *(float4*)(out + a * 16 + 4 * (lane >> 4)) =
+                        make_float4(acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]);
+                }
+            }
+            continue;
+        }
         int pair = col / 3, comp = col - 3 * pair;
         T* dst = dst_exp + ((size_t)pair_dst[pair] * 3 + comp) * Lp + wrow;
 #pragma unroll
@@ -148,7 +163,7 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 int rr = (sizeof(T) == 8) ? (4 * r + (lane >> 4)) : (4 * (lane >> 4) + r);
-                add_out(dst + a * 16 + rr, (T)acc[a][b][r], atomic);
+                add_out(dst + a * 16 + rr, (T)acc[a][b][r], epi);
             }
     }
 }
@@ -159,7 +174,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
                                                      const GemmTile* __restrict__ tiles,
                                                      const int* __restrict__ pair_src, const int* __restrict__ pair_dst,
                                                      const float* __restrict__ src_exp, float* __restrict__ dst_exp,
-                                                     int atomic) {
+                                                     int epi) {
     constexpr int Lp = 64 * RB, BK = 32, S = BK + 8, NK = Lp / BK;
     // k-contiguous rows: each lane reads its 8 consecutive k as one 16-byte vector
     __shared__ __attribute__((aligned(16))) __bf16 Ah[2][Lp][S];
@@ -266,37 +281,45 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
         int cl = b * 16 + (lane & 15);
         if (cl >= tile.ncol) continue;
         int col = tile.col0 + cl;
+        if (epi == EPI_STORE) {
+            float* out = dst_exp + (size_t)col * Lp + wrow;
+#pragma unroll
+            for (int a = 0; a < RB; a++)
+                *(float4*)(out + a * 16 + 4 * (lane >> 4)) =
+                    make_float4(acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]);
+            continue;
+        }
         int pair = col / 3, comp = col - 3 * pair;
         float* dst = dst_exp + ((size_t)pair_dst[pair] * 3 + comp) * Lp + wrow;
 #pragma unroll
         for (int a = 0; a < RB; a++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) add_out(dst + a * 16 + 4 * (lane >> 4) + r, acc[a][b][r], atomic);
+            for (int r = 0; r < 4; r++) add_out(dst + a * 16 + 4 * (lane >> 4) + r, acc[a][b][r], epi);
     }
 }
 
 template <int RB, bool TR>
 static void launch_rb(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
-                      const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, bool atomic,
+                      const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int epi,
                       hipStream_t st) {
     if (prec == Precision::F64)
         k_gemm<double, RB, TR><<<ntiles, 256, 0, st>>>((const double*)ops, d_tiles, pair_src, pair_dst,
-                                                      (const double*)src_exp, (double*)dst_exp, atomic ? 1 : 0);
+                                                      (const double*)src_exp, (double*)dst_exp, epi);
     else if (prec == Precision::F32)
         k_gemm<float, RB, TR><<<ntiles, 256, 0, st>>>((const float*)ops, d_tiles, pair_src, pair_dst,
-                                                     (const float*)src_exp, (float*)dst_exp, atomic ? 1 : 0);
+                                                     (const float*)src_exp, (float*)dst_exp, epi);
     else
         k_gemm_bf16x3<RB, TR><<<ntiles, 256, 0, st>>>((const __bf16*)ops, (const __bf16*)ops_lo, d_tiles, pair_src,
-                                                     pair_dst, (const float*)src_exp, (float*)dst_exp, atomic ? 1 : 0);
+                                                     pair_dst, (const float*)src_exp, (float*)dst_exp, epi);
     FMM_HIP(hipGetLastError());
 }
 
 template <bool TR>
 static void launch(int RB, Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
-                   const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, bool atomic,
+                   const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int epi,
                    hipStream_t st) {
 #define FMM_RB(R) \
-    case R: launch_rb<R, TR>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, atomic, st); break;
+    case R: launch_rb<R, TR>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epi, st); break;
     switch (RB) {
         FMM_RB(1) FMM_RB(2) FMM_RB(3) FMM_RB(4) FMM_RB(5) FMM_RB(6) FMM_RB(7)
         default: throw Error(4, "padded expansion length above 448 (order > 20)");
@@ -305,13 +328,16 @@ static void launch(int RB, Precision prec, const void* ops, const void* ops_lo, 
 }
 
 void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
-                const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, bool atomic,
+                const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, int epilogue,
                 hipStream_t st, bool transpose) {
     if (ntiles == 0) return;
     if (Lp % 64 != 0) throw Error(1, "padded expansion length must be a multiple of 64");
     int RB = Lp / 64;
-    if (transpose) launch<true>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, atomic, st);
-    else launch<false>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, atomic, st);
+    if (epilogue < EPI_ADD || epilogue > EPI_STORE) throw Error(1, "unknown GEMM epilogue");
+    if (transpose)
+        launch<true>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epilogue, st);
+    else
+        launch<false>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epilogue, st);
 }
 
 // host-side tiling of class column ranges: class c owns pairs [begin[c], begin[c+1])
@@ -322,6 +348,85 @@ void plan_tiles(const std::vector<int>& class_begin, const std::vector<int>& cla
         int c0 = 3 * class_begin[c], c1 = 3 * class_begin[c + 1];
         for (int j = c0; j < c1; j += BN) tiles.push_back(GemmTile{class_op[c], j, std::min(BN, c1 - j)});
     }
+}
+
+// ---------------------------------------------------------------- per-node reduction of pair slabs
+__global__ void k_iota(int* v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = i;
+}
+__global__ void k_zero_ranges(int* b, int* e, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = e[i] = 0;
+}
+__global__ void k_key_ranges(const uint32_t* __restrict__ sk, int n, int* __restrict__ b, int* __restrict__ e) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = sk[i];
+    if (i == 0 || sk[i - 1] != k) b[k] = i;
+    if (i == n - 1 || sk[i + 1] != k) e[k] = i + 1;
+}
+
+static int key_bits(int nnodes) {
+    int b = 1;
+    while (b < 32 && (1u << b) < (unsigned)nnodes) b++;
+    return b;
+}
+
+void build_pair_index(const int* key, int npairs, int nnodes, PairIndex& ix, hipStream_t st) {
+    ix.npairs = npairs;
+    ix.nnodes = nnodes;
+    int* b = ix.begin.get<int>(std::max(nnodes, 1));
+    int* e = ix.end.get<int>(std::max(nnodes, 1));
+    if (nnodes) k_zero_ranges<<<blocks(nnodes), 256, 0, st>>>(b, e, nnodes);
+    if (!npairs) return;
+    int* io = ix.iota.get<int>(npairs);
+    k_iota<<<blocks(npairs), 256, 0, st>>>(io, npairs);
+    uint32_t* ks = ix.keys_sorted.get<uint32_t>(npairs);
+    int* ord = ix.order.get<int>(npairs);
+    // stable radix sort: pairs of one node stay in ascending pair order
+    sort_pairs_u32((const uint32_t*)key, ks, io, ord, npairs, key_bits(nnodes), ix.tmp, st);
+    k_key_ranges<<<blocks(npairs), 256, 0, st>>>(ks, npairs, b, e);
+    FMM_HIP(hipGetLastError());
+}
+
+// one wave per node, 16-byte vectors across the node's 3 Lp values, pairs added in order
+template <class V>
+__global__ __launch_bounds__(64) void k_reduce_pairs(const int* __restrict__ begin, const int* __restrict__ end,
+                                                     const int* __restrict__ order, const V* __restrict__ slab, int nv,
+                                                     V* __restrict__ out) {
+    const int v = blockIdx.x;
+    const int b = begin[v], e = end[v];
+    if (b >= e) return;
+    V* o = out + (size_t)v * nv;
+    for (int i = threadIdx.x; i < nv; i += 64) {
+        V acc = o[i];
+        int k = b;
+        for (; k + 4 <= e; k += 4) {
+            V x0 = slab[(size_t)order[k] * nv + i], x1 = slab[(size_t)order[k + 1] * nv + i];
+            V x2 = slab[(size_t)order[k + 2] * nv + i], x3 = slab[(size_t)order[k + 3] * nv + i];
+            acc = acc + x0;
+            acc = acc + x1;
+            acc = acc + x2;
+            acc = acc + x3;
+        }
+        for (; k < e; k++) acc = acc + slab[(size_t)order[k] * nv + i];
+        o[i] = acc;
+    }
+}
+
+void reduce_pairs(bool f64, const PairIndex& ix, const void* slab, int Lp, void* out, hipStream_t st) {
+    if (!ix.npairs || !ix.nnodes) return;
+    if (f64) {
+        int nv = 3 * Lp / 2;
+        k_reduce_pairs<double2><<<ix.nnodes, 64, 0, st>>>(ix.begin.as<int>(), ix.end.as<int>(), ix.order.as<int>(),
+                                                          (const double2*)slab, nv, (double2*)out);
+    } else {
+        int nv = 3 * Lp / 4;
+        k_reduce_pairs<float4><<<ix.nnodes, 64, 0, st>>>(ix.begin.as<int>(), ix.end.as<int>(), ix.order.as<int>(),
+                                                         (const float4*)slab, nv, (float4*)out);
+    }
+    FMM_HIP(hipGetLastError());
 }
 
 }  // namespace fmm

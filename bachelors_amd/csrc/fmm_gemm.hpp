@@ -11,12 +11,31 @@ struct GemmTile {
     int ncol;    // valid columns in this tile (<= 64)
 };
 
+// How a class GEMM delivers column j = 3 * pair + component of T B:
+enum Epilogue {
+    EPI_ADD = 0,      // dst_exp[pair_dst[pair]][component] += (plain; each destination once per launch)
+    EPI_ATOMIC = 1,   // the same with hardware FP atomics (destinations repeat)
+    EPI_STORE = 2     // written to dst_exp + j * Lp (a per-pair slab; pair_dst unused)
+};
+
 // ops: operator tables (double / float / bf16 hi), ops_lo: bf16 lo parts (BF16X3 only).
 // transpose: apply T^T (adjoint passes).
 // src_exp / dst_exp: [node][3][Lp] in double (F64) or float (F32, BF16X3).
 void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
-                const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, bool atomic,
+                const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, int epilogue,
                 hipStream_t st, bool transpose = false);
+
+// Pairs grouped by the node that receives them: order[begin[v] .. end[v]) are the pair indices
+// (ascending) whose key is v; nodes without pairs have begin = end = 0.
+struct PairIndex {
+    int npairs = 0, nnodes = 0;
+    DevBuf keys, keys_sorted, iota, order, begin, end, tmp;
+};
+void build_pair_index(const int* key, int npairs, int nnodes, PairIndex& ix, hipStream_t st);
+
+// out[v][0 .. 3 Lp) += sum over the pairs of v, in ascending pair order, of slab[pair][0 .. 3 Lp)
+// (slab = an EPI_STORE result); elements are double if f64 else float
+void reduce_pairs(bool f64, const PairIndex& ix, const void* slab, int Lp, void* out, hipStream_t st);
 
 void plan_tiles(const std::vector<int>& class_begin, const std::vector<int>& class_op, std::vector<GemmTile>& tiles);
 

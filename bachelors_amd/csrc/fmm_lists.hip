@@ -176,7 +176,6 @@ __global__ void k_fill_int(int* p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-static inline int blocks(long long n, int b = 256) { return (int)((n + b - 1) / b); }
 
 static TreeView view(const TreeDev& T) {
     return TreeView{T.center.as<double>(), T.first_child.as<int>(), T.npts.as<int>(), T.level.as<int>()};

@@ -194,7 +194,6 @@ static void grow_keep(DevBuf& b, size_t need, size_t used, hipStream_t st) {
     b.bytes = nb;
 }
 
-static inline int blocks(long long n, int b = 256) { return (int)((n + b - 1) / b); }
 
 void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const double root_c[3],
                 const double root_h[3], int thr, int max_depth, bool cubic_scale, hipStream_t st) {
