@@ -81,6 +81,20 @@ def time_fmm(fm, torch, prec, d_src, d_w, d_tgt, p, cap, steps, warmup, subset, 
     return ms, st, err, finite
 
 
+def measured_traffic(args):
+    """HBM bytes per M2L GEMM launch from the committed rocprofv3 counter summary of the default
+    workload (FETCH_SIZE + WRITE_SIZE, calibrated by a 1 GiB copy); None for other workloads."""
+    default = (args.sources, args.targets, args.order, args.cap, args.precision) == (1 << 20, 1 << 20, 12, 64, "fp32")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "m2l_gemm_hbm_traffic_1M_shell_p12_fp32.md")
+    if not default or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(seed, n, p, cap):
     from oracle import oracle as O
     src, w, tgt = workload(n, n, seed + 100)
@@ -107,6 +121,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="first copy 1 GiB device-to-device (the calibration kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (sharding by target Morton "
@@ -118,6 +134,12 @@ def main():
     seed = 20261202 + 2
     src, w, tgt = workload(args.sources, args.targets, seed)
     dev = torch.device("cuda:0")
+    if args.traffic_probe:
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     d_src = torch.from_numpy(src).to(dev)
     d_w = torch.from_numpy(w).to(dev)
     d_tgt = torch.from_numpy(tgt).to(dev)
@@ -162,7 +184,7 @@ def main():
         "stage_ms": {k[2:]: round(v, 4) for k, v in st.items() if k.startswith("t_")},
         "counts": {k: st[k] for k in ("source_nodes", "target_nodes", "far_pairs", "close_pairs", "m2l_classes")},
         "roofline": {"bound": "mfma", "kernel": "M2L class-batched GEMM", "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(args)},
     }
     if not args.no_fp64:
         ms64, st64, err64, fin64 = time_fmm(fm, torch, "fp64", d_src, d_w, d_tgt, args.order, args.cap, args.steps,
