@@ -91,6 +91,7 @@ struct TreeDev {
     std::vector<int> h_first_child, h_npts, h_begin, h_end, h_level, h_parent;
     std::vector<double> h_center;
     int dropped = 0;                   // input points outside the root
+    bool sorted_build = false;         // built by the one-sort path (else level by level)
 };
 
 }  // namespace fmm

@@ -195,10 +195,7 @@ static void grow_keep(DevBuf& b, size_t need, size_t used, hipStream_t st) {
 }
 
 
-void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const double root_c[3],
-                const double root_h[3], int thr, int max_depth, bool cubic_scale, hipStream_t st) {
-    if (thr < 2) throw Error(1, "leaf capacity too small: a node would subdivide forever");
-    if (max_depth < 0 || max_depth >= MAX_LEVELS) max_depth = MAX_LEVELS - 1;
+static void set_level_sizes(TreeDev& T, const double root_h[3], bool cubic_scale) {
     // per-level half sizes: child half = (float)h / 2 (quadrature_octree.cpp:83-89)
     for (int a = 0; a < 3; a++) T.half[0][a] = root_h[a];
     for (int k = 1; k < MAX_LEVELS; k++)
@@ -208,6 +205,32 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
         T.radius[k] = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
         T.scale[k] = cubic_scale ? h[0] : T.radius[k] / std::sqrt(3.0);
     }
+}
+
+static void copy_host_mirrors(TreeDev& T, hipStream_t st) {
+    int num_nodes = T.num_nodes;
+    T.h_first_child.resize(num_nodes);
+    T.h_npts.resize(num_nodes);
+    T.h_begin.resize(num_nodes);
+    T.h_end.resize(num_nodes);
+    T.h_level.resize(num_nodes);
+    T.h_parent.resize(num_nodes);
+    T.h_center.resize(3 * (size_t)num_nodes);
+    FMM_HIP(hipMemcpyAsync(T.h_first_child.data(), T.first_child.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_npts.data(), T.npts.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_begin.data(), T.begin.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_end.data(), T.end.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_level.data(), T.level.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_parent.data(), T.parent.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_center.data(), T.center.ptr, 3 * (size_t)num_nodes * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+}
+
+// Level-by-level build: every level partitions the points still being refined into the
+// children by testing all 8 child boxes, so duplicates (overlaps) and drops (gaps) are exact.
+static void build_tree_levels(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const double root_c[3],
+                              int thr, int max_depth, hipStream_t st) {
     // root
     size_t node_cap = 1024;
     grow_keep(T.center, node_cap * 3 * sizeof(double), 0, st);
@@ -223,7 +246,7 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
     }
     S.lvl_n.assign(MAX_LEVELS, 0);
     double3 rc = make_double3(root_c[0], root_c[1], root_c[2]);
-    double3 rh = make_double3(root_h[0], root_h[1], root_h[2]);
+    double3 rh = make_double3(T.half[0][0], T.half[0][1], T.half[0][2]);
     int n0 = 0;
     if (n > 0) {
         k_root_filter<<<blocks(n), 256, 0, st>>>(d_xyz, n, rc, rh, flag);
@@ -350,23 +373,341 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
     }
     k_leaf_npts<<<blocks(num_nodes), 256, 0, st>>>(T.first_child.as<int>(), T.arrivals.as<int>(), num_nodes,
                                                    T.npts.as<int>());
-    // host mirrors of the node tables (small; used to plan later stages)
-    T.h_first_child.resize(num_nodes);
-    T.h_npts.resize(num_nodes);
-    T.h_begin.resize(num_nodes);
-    T.h_end.resize(num_nodes);
-    T.h_level.resize(num_nodes);
-    T.h_parent.resize(num_nodes);
-    T.h_center.resize(3 * (size_t)num_nodes);
-    FMM_HIP(hipMemcpyAsync(T.h_first_child.data(), T.first_child.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_npts.data(), T.npts.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_begin.data(), T.begin.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_end.data(), T.end.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_level.data(), T.level.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_parent.data(), T.parent.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_center.data(), T.center.ptr, 3 * (size_t)num_nodes * sizeof(double),
-                           hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------- one-sort build
+// Each point's path (octant digit per level, reference child order) is computed once from
+// the same float-rounded child boxes and half-open tests; one radix sort of the root's
+// points by path puts them in DFS order, so every node is a contiguous range of the sorted
+// array.  Levels are then built top-down point-parallel: each sorted point carries its
+// node, moves to child (first_child + digit) when the node subdivides, and a child's range
+// starts/ends where the neighbour's path prefix differs.  A point whose child at some level
+// is not unique (a gap or an overlap of the float-rounded boxes) is marked with that level;
+// if such a level is reached by the point's leaf the tree is rebuilt level by level, which
+// keeps duplicates and drops exact.
+constexpr int PATH_LEVELS = 21;   // 3 bits per level in a 63-bit key (level 1 in the top bits)
+constexpr int SHORT_LEVELS = 10;  // first attempt: 30-bit keys
+
+struct LevelHalves {
+    float h[PATH_LEVELS + 1][3];   // half size of level k boxes (float-rounded, as the tree's)
+};
+
+__device__ __forceinline__ int octant_of(bool sx, bool sy, bool sz) {
+    // inverse of CHILD_SIGN: (-,-)->0, (+,-)->1, (+,+)->2, (-,+)->3, +4 for +z
+    return (sz ? 4 : 0) + (sx ? (sy ? 2 : 1) : (sy ? 3 : 0));
+}
+
+__global__ void k_paths(const double* __restrict__ P, int n, double3 rc, double3 rh, LevelHalves H, int depth,
+                        uint64_t* __restrict__ key, int* __restrict__ irregular, uint32_t* __restrict__ inroot) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = P[3 * i], y = P[3 * i + 1], z = P[3 * i + 2];
+    bool in = in_range(x, rc.x, rh.x) && in_range(y, rc.y, rh.y) && in_range(z, rc.z, rh.z);
+    inroot[i] = in ? 1u : 0u;
+    if (!in) {
+        key[i] = 0;
+        irregular[i] = 0;
+        return;
+    }
+    float cx = (float)rc.x, cy = (float)rc.y, cz = (float)rc.z;
+    uint64_t k = 0;
+    int irr = PATH_LEVELS + 1;
+    for (int l = 1; l <= depth; l++) {
+        const float hx = H.h[l][0], hy = H.h[l][1], hz = H.h[l][2];
+        const float lx = __fsub_rn(cx, hx), ux = __fadd_rn(cx, hx);
+        const float ly = __fsub_rn(cy, hy), uy = __fadd_rn(cy, hy);
+        const float lz = __fsub_rn(cz, hz), uz = __fadd_rn(cz, hz);
+        const double dhx = hx, dhy = hy, dhz = hz;
+        const bool x0 = in_range(x, lx, dhx), x1 = in_range(x, ux, dhx);
+        const bool y0 = in_range(y, ly, dhy), y1 = in_range(y, uy, dhy);
+        const bool z0 = in_range(z, lz, dhz), z1 = in_range(z, uz, dhz);
+        if (x0 == x1 || y0 == y1 || z0 == z1) {   // in two children or in none
+            irr = l;
+            k <<= 3 * (depth - l + 1);
+            break;
+        }
+        k = (k << 3) | (uint64_t)octant_of(x1, y1, z1);
+        cx = x1 ? ux : lx;
+        cy = y1 ? uy : ly;
+        cz = z1 ? uz : lz;
+    }
+    key[i] = k;
+    irregular[i] = irr;
+}
+
+// the root's points, compacted in input order, keyed by the top `levels` levels of their path
+template <class K>
+__global__ void k_compact_paths(const uint64_t* __restrict__ key, const uint32_t* __restrict__ inroot,
+                                const uint32_t* __restrict__ rank, int n, int shift, K* __restrict__ kc,
+                                int* __restrict__ idx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !inroot[i]) return;
+    kc[rank[i]] = (K)(key[i] >> shift);
+    idx[rank[i]] = i;
+}
+
+__global__ void k_sorted_flags(const int* __restrict__ sb, const int* __restrict__ se, int lb, int nnodes, int thr,
+                               int allow, int deeper, uint32_t* __restrict__ flag, int* __restrict__ need_deeper) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    int node = lb + i;
+    bool sub = se[node] - sb[node] >= thr;
+    flag[i] = (allow && sub) ? 1u : 0u;
+    if (deeper && sub) *need_deeper = 1;
+}
+
+// number of subdividing nodes and the need-deeper flag in one readback
+__global__ void k_level_counts(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank, int nn,
+                               const int* __restrict__ need, int* __restrict__ out) {
+    out[0] = (int)(rank[nn - 1] + flag[nn - 1]);
+    out[1] = *need;
+}
+
+// the 8 children of every subdivided node: float centres as the reference; ranges start empty
+__global__ void k_make_children_sorted(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank, int lb,
+                                       int nnodes, int child_lb, int child_level, double3 ph,
+                                       double* __restrict__ center, int* __restrict__ first_child,
+                                       int* __restrict__ parent, int* __restrict__ level, int* __restrict__ sb,
+                                       int* __restrict__ se) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    int node = lb + i;
+    if (!flag[i]) { first_child[node] = -1; return; }
+    int fc = child_lb + 8 * (int)rank[i];
+    first_child[node] = fc;
+    float x = (float)center[3 * node], y = (float)center[3 * node + 1], z = (float)center[3 * node + 2];
+    float w = (float)ph.x, h = (float)ph.y, d = (float)ph.z;
+    float w2 = w / 2.0f, h2 = h / 2.0f, d2 = d / 2.0f;
+    for (int o = 0; o < 8; o++) {
+        int c = fc + o;
+        center[3 * c] = c_child_sign[o][0] < 0 ? __fsub_rn(x, w2) : __fadd_rn(x, w2);
+        center[3 * c + 1] = c_child_sign[o][1] < 0 ? __fsub_rn(y, h2) : __fadd_rn(y, h2);
+        center[3 * c + 2] = c_child_sign[o][2] < 0 ? __fsub_rn(z, d2) : __fadd_rn(z, d2);
+        parent[c] = node;
+        level[c] = child_level;
+        first_child[c] = -1;
+        sb[c] = -1;
+        se[c] = -1;
+    }
+}
+
+// every sorted point of a subdivided node moves to its child; runs of equal path prefix
+// give the children's ranges
+template <class K>
+__global__ void k_assign_level(const K* __restrict__ sk, int n0, int child_level, int depth,
+                               const int* __restrict__ first_child, int* __restrict__ node_of, int* __restrict__ sb,
+                               int* __restrict__ se) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n0) return;
+    int fc = first_child[node_of[j]];
+    if (fc < 0) return;
+    const int sh = 3 * (depth - child_level);
+    const K pre = sk[j] >> sh;
+    const int c = fc + (int)(pre & (K)7);
+    node_of[j] = c;
+    if (j == 0 || (sk[j - 1] >> sh) != pre) sb[c] = j;
+    if (j == n0 - 1 || (sk[j + 1] >> sh) != pre) se[c] = j + 1;
+}
+
+// empty children: begin = end = where they sit between their siblings (DFS position)
+__global__ void k_fix_empty(const int* __restrict__ first_child, int lb, int nnodes, int* __restrict__ sb,
+                            int* __restrict__ se) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    int node = lb + i;
+    int fc = first_child[node];
+    if (fc < 0) return;
+    int pos = sb[node];
+    for (int o = 0; o < 8; o++) {
+        int c = fc + o;
+        if (sb[c] < 0) sb[c] = se[c] = pos;
+        else pos = se[c];
+    }
+}
+
+__global__ void k_node_ranges(const int* __restrict__ first_child, const int* __restrict__ sb,
+                              const int* __restrict__ se, int nnodes, int* __restrict__ npts, int* __restrict__ begin,
+                              int* __restrict__ end) {
+    int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nnodes) return;
+    begin[node] = sb[node];
+    end[node] = se[node];
+    npts[node] = first_child[node] < 0 ? se[node] - sb[node] : 0;
+}
+
+// per sorted point: the key of its leaf for the in-leaf (input order) sort, and the check
+// that no point's path was ambiguous at or above its leaf
+__global__ void k_leaf_keys(const int* __restrict__ node_of, const int* __restrict__ order, int n0,
+                            const uint32_t* __restrict__ rank, const int* __restrict__ sb, const int* __restrict__ lvl,
+                            const int* __restrict__ irregular, uint32_t* __restrict__ leaf_key, int* __restrict__ bad) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n0) return;
+    int nd = node_of[j], q = order[j];
+    leaf_key[rank[q]] = (uint32_t)sb[nd];
+    if (irregular[q] <= lvl[nd]) *bad = 1;
+}
+
+__global__ void k_count_root(const uint32_t* __restrict__ inroot, const uint32_t* __restrict__ rank, int n,
+                             int* __restrict__ out) {
+    out[0] = (int)(rank[n - 1] + inroot[n - 1]);
+}
+
+__global__ void k_init_root(double3 c, int n0, double* __restrict__ center, int* __restrict__ parent,
+                            int* __restrict__ level, int* __restrict__ first_child, int* __restrict__ sb,
+                            int* __restrict__ se, int* __restrict__ flags2) {
+    center[0] = c.x;
+    center[1] = c.y;
+    center[2] = c.z;
+    parent[0] = -1;
+    level[0] = 0;
+    first_child[0] = -1;
+    sb[0] = 0;
+    se[0] = n0;
+    flags2[0] = flags2[1] = 0;
+}
+__global__ void k_zero_i32(int* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// returns false when the key is too short for the tree (a node below `depth` subdivides)
+template <class K>
+static bool build_from_sorted(TreeDev& T, TreeScratch& S, const K* sk, int n0, int depth, int thr, int max_depth,
+                              const double root_c[3], hipStream_t st) {
+    size_t node_cap = 1024;
+    grow_keep(T.center, node_cap * 3 * sizeof(double), 0, st);
+    grow_keep(T.first_child, node_cap * sizeof(int), 0, st);
+    grow_keep(T.parent, node_cap * sizeof(int), 0, st);
+    grow_keep(T.level, node_cap * sizeof(int), 0, st);
+    grow_keep(S.sb, node_cap * sizeof(int), 0, st);
+    grow_keep(S.se, node_cap * sizeof(int), 0, st);
+    int* need = S.need.get<int>(4);
+    int* node_of = S.node_of.get<int>(std::max(n0, 1));
+    k_init_root<<<1, 1, 0, st>>>(make_double3(root_c[0], root_c[1], root_c[2]), n0, T.center.as<double>(),
+                                 T.parent.as<int>(), T.level.as<int>(), T.first_child.as<int>(), S.sb.as<int>(),
+                                 S.se.as<int>(), need);
+    if (n0) k_zero_i32<<<blocks(n0), 256, 0, st>>>(node_of, n0);
+    T.level_begin[0] = 0;
+    T.level_begin[1] = 1;
+    int num_nodes = 1, k = 0;
+    for (;; k++) {
+        int lb = T.level_begin[k], le = T.level_begin[k + 1], nn = le - lb;
+        uint32_t* sflag = S.sflag.get<uint32_t>(nn + 1);
+        uint32_t* srank = S.srank.get<uint32_t>(nn + 1);
+        // the reference's rule (arrivals >= thr) below max_depth; a node that would subdivide
+        // below the key's depth needs the longer key
+        int allow = (k < max_depth && k < depth) ? 1 : 0;
+        int deeper = (k < max_depth && k >= depth) ? 1 : 0;
+        k_sorted_flags<<<blocks(nn), 256, 0, st>>>(S.sb.as<int>(), S.se.as<int>(), lb, nn, thr, allow, deeper, sflag,
+                                                   need);
+        exclusive_scan_u32(sflag, srank, nn, S.tmp, st);
+        k_level_counts<<<1, 1, 0, st>>>(sflag, srank, nn, need, need + 2);
+        FMM_HIP(hipMemcpyAsync(S.h_last, need + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipStreamSynchronize(st));
+        if (S.h_last[1]) return false;
+        int nsub = (int)S.h_last[0];
+        int child_lb = le;
+        int new_nodes = num_nodes + 8 * nsub;
+        size_t used = (size_t)num_nodes;
+        grow_keep(T.center, (size_t)new_nodes * 3 * sizeof(double), used * 3 * sizeof(double), st);
+        grow_keep(T.first_child, (size_t)new_nodes * sizeof(int), used * sizeof(int), st);
+        grow_keep(T.parent, (size_t)new_nodes * sizeof(int), used * sizeof(int), st);
+        grow_keep(T.level, (size_t)new_nodes * sizeof(int), used * sizeof(int), st);
+        grow_keep(S.sb, (size_t)new_nodes * sizeof(int), used * sizeof(int), st);
+        grow_keep(S.se, (size_t)new_nodes * sizeof(int), used * sizeof(int), st);
+        double3 ph = make_double3(T.half[k][0], T.half[k][1], T.half[k][2]);
+        k_make_children_sorted<<<blocks(nn), 256, 0, st>>>(sflag, srank, lb, nn, child_lb, k + 1, ph,
+                                                           T.center.as<double>(), T.first_child.as<int>(),
+                                                           T.parent.as<int>(), T.level.as<int>(), S.sb.as<int>(),
+                                                           S.se.as<int>());
+        num_nodes = new_nodes;
+        T.level_begin[k + 2] = new_nodes;
+        if (nsub == 0) break;
+        if (n0) k_assign_level<K><<<blocks(n0), 256, 0, st>>>(sk, n0, k + 1, depth, T.first_child.as<int>(), node_of,
+                                                               S.sb.as<int>(), S.se.as<int>());
+        k_fix_empty<<<blocks(nn), 256, 0, st>>>(T.first_child.as<int>(), lb, nn, S.sb.as<int>(), S.se.as<int>());
+    }
+    T.num_levels = k + 1;
+    T.num_nodes = num_nodes;
+    return true;
+}
+
+void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const double root_c[3],
+                const double root_h[3], int thr, int max_depth, bool cubic_scale, hipStream_t st) {
+    if (thr < 2) throw Error(1, "leaf capacity too small: a node would subdivide forever");
+    if (max_depth < 0 || max_depth >= MAX_LEVELS) max_depth = MAX_LEVELS - 1;
+    set_level_sizes(T, root_h, cubic_scale);
+    T.sorted_build = false;
+    if (n > 0 && max_depth <= PATH_LEVELS) {
+        const int depth = max_depth;
+        LevelHalves H;
+        for (int l = 0; l <= PATH_LEVELS; l++)
+            for (int a = 0; a < 3; a++) H.h[l][a] = (float)T.half[std::min(l, MAX_LEVELS - 1)][a];
+        uint64_t* key = S.k64.get<uint64_t>(n);
+        int* irr = S.irr.get<int>(n);
+        uint32_t* inroot = S.flag.get<uint32_t>(n + 1);
+        uint32_t* rank = S.pos.get<uint32_t>(n + 1);
+        double3 rc = make_double3(root_c[0], root_c[1], root_c[2]);
+        double3 rh = make_double3(root_h[0], root_h[1], root_h[2]);
+        k_paths<<<blocks(n), 256, 0, st>>>(d_xyz, n, rc, rh, H, depth, key, irr, inroot);
+        exclusive_scan_u32(inroot, rank, n, S.tmp, st);
+        int* cnt2 = S.need.get<int>(4);
+        k_count_root<<<1, 1, 0, st>>>(inroot, rank, n, cnt2);
+        FMM_HIP(hipMemcpyAsync(S.h_last, cnt2, sizeof(int), hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipStreamSynchronize(st));
+        const int n0 = (int)S.h_last[0];
+        int* idx = S.val.get<int>(std::max(n0, 1));       // root's points in input order
+        int* order = S.order.get<int>(std::max(n0, 1));   // ... sorted by path
+        bool ok = false;
+        const int short_depth = std::min(depth, SHORT_LEVELS);
+        if (n0 > 0) {
+            uint32_t* k32 = S.key.get<uint32_t>(n0);
+            uint32_t* k32s = S.key2.get<uint32_t>(n0);
+            k_compact_paths<uint32_t><<<blocks(n), 256, 0, st>>>(key, inroot, rank, n, 3 * (depth - short_depth), k32,
+                                                                 idx);
+            sort_pairs_u32(k32, k32s, idx, order, n0, 3 * short_depth, S.tmp, st);
+            ok = build_from_sorted<uint32_t>(T, S, k32s, n0, short_depth, thr, max_depth, root_c, st);
+            if (!ok) {
+                uint64_t* kc = S.k64c.get<uint64_t>(n0);
+                uint64_t* kcs = S.k64s.get<uint64_t>(n0);
+                k_compact_paths<uint64_t><<<blocks(n), 256, 0, st>>>(key, inroot, rank, n, 0, kc, idx);
+                sort_pairs_u64(kc, kcs, idx, order, n0, 3 * depth, S.tmp, st);
+                ok = build_from_sorted<uint64_t>(T, S, kcs, n0, depth, thr, max_depth, root_c, st);
+            }
+        } else {
+            ok = build_from_sorted<uint32_t>(T, S, (const uint32_t*)nullptr, 0, short_depth, thr, max_depth, root_c, st);
+        }
+        if (ok) {
+            const int nn = T.num_nodes;
+            T.npts.reserve((size_t)nn * sizeof(int));
+            T.begin.reserve((size_t)nn * sizeof(int));
+            T.end.reserve((size_t)nn * sizeof(int));
+            k_node_ranges<<<blocks(nn), 256, 0, st>>>(T.first_child.as<int>(), S.sb.as<int>(), S.se.as<int>(), nn,
+                                                      T.npts.as<int>(), T.begin.as<int>(), T.end.as<int>());
+            // in-leaf input order: stable sort of the (input-ordered) root points by leaf start
+            uint32_t* lkey = S.key.get<uint32_t>(std::max(n0, 1));
+            uint32_t* lkeys = S.key2.get<uint32_t>(std::max(n0, 1));
+            int* bad = S.need.get<int>(4) + 3;
+            FMM_HIP(hipMemsetAsync(bad, 0, sizeof(int), st));
+            if (n0) k_leaf_keys<<<blocks(n0), 256, 0, st>>>(S.node_of.as<int>(), order, n0, rank, S.sb.as<int>(),
+                                                            T.level.as<int>(), irr, lkey, bad);
+            FMM_HIP(hipMemcpyAsync(S.h_last, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+            FMM_HIP(hipStreamSynchronize(st));
+            if (S.h_last[0] == 0) {
+                int bits = 1;
+                while (bits < 32 && (1u << bits) <= (unsigned)n0) bits++;
+                int* slots = T.slots.get<int>(std::max(n0, 1));
+                if (n0) sort_pairs_u32(lkey, lkeys, idx, slots, n0, bits, S.tmp, st);
+                T.num_slots = n0;
+                T.dropped = n - n0;
+                T.sorted_build = true;
+                copy_host_mirrors(T, st);
+                return;
+            }
+        }
+    }
+    build_tree_levels(T, S, d_xyz, n, root_c, thr, max_depth, st);
+    copy_host_mirrors(T, st);
 }
 
 }  // namespace fmm

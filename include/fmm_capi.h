@@ -77,6 +77,8 @@ typedef struct fmm_stats {
     /* stage times in ms (HIP events on the context stream) */
     double t_tree, t_p2m, t_m2m, t_lists, t_tables, t_m2l, t_l2l, t_l2p, t_p2p, t_total;
     double t_m2l_gemm;                   /* the M2L translation GEMM alone (part of t_m2l) */
+    int source_tree_sorted, target_tree_sorted;   /* 1: built by one sort of point paths;
+                                                     0: level by level (gaps/overlaps of child boxes) */
 } fmm_stats;
 
 void fmm_config_init(fmm_config* cfg);

@@ -31,3 +31,25 @@ def shell_sources(rng, n, r0=0.8, r1=1.0, zh=0.5):
 
 def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def gap_overlap_targets(orc, tgt, cap, rng, nadd=60):
+    """Targets placed exactly in the one-ulp gaps and overlaps between float-rounded sibling
+    boxes of the reference (bounding-box) target tree, inside the targets' bounding box so the
+    root is unchanged: each lands in two children (inserted twice) or in none (dropped)."""
+    T = orc.target_tree(tgt, cap)
+    lo, hi = tgt.min(0), tgt.max(0)
+    cand = []
+    for b in np.nonzero(T.first_child >= 0)[0]:
+        fc = T.first_child[b]
+        for o_hi, ax in ((1, 0), (3, 1), (4, 2)):      # siblings of child 0 across x, y, z
+            c_lo, c_hi, h = T.center[fc], T.center[fc + o_hi], T.half[fc]
+            up, dn = c_lo[ax] + h[ax], c_hi[ax] - h[ax]
+            if up != dn:
+                p = c_lo.copy()
+                p[ax] = min(up, dn)
+                if np.all(p >= lo) and np.all(p <= hi):
+                    cand.append(p)
+    cand = np.array(cand)
+    pick = rng.choice(len(cand), size=min(nadd, len(cand)), replace=False)
+    return np.concatenate([tgt, cand[pick]]), len(pick)

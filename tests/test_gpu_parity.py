@@ -4,7 +4,7 @@ the tolerance written in each test."""
 import numpy as np
 import pytest
 
-from conftest import rel_l2, shell_sources
+from conftest import gap_overlap_targets, rel_l2, shell_sources
 
 pytestmark = pytest.mark.gpu
 
@@ -66,6 +66,45 @@ def test_trees_bit_exact(fm, orc, frame):
     F.build_trees()
     _cmp_tree(F.tree(0), orc.source_tree(src, 20))
     _cmp_tree(F.tree(1), orc.target_tree(tgt, 24, root=SHARED_ROOT if frame == 1 else None))
+    assert F.stats()["source_tree_sorted"] == 1
+
+
+def test_trees_bit_exact_float_gaps_and_overlaps(fm, orc):
+    """Reference frame: targets inside one-ulp gaps/overlaps of float-rounded sibling boxes are
+    dropped or inserted twice, as by the reference's pointer insertion (level-by-level build)."""
+    rng = np.random.default_rng(23)
+    src, w, tgt = _inputs(3, 8000, 6000)
+    tgt, nadd = gap_overlap_targets(orc, tgt, 24, rng)
+    F = fm.FMM(order=4, source_leaf_capacity=20, target_leaf_capacity=24, frame=0)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    F.build_trees()
+    _cmp_tree(F.tree(1), orc.target_tree(tgt, 24))
+    assert F.stats()["target_tree_sorted"] == 0
+
+
+def test_trees_bit_exact_lattice_faces_and_deep_cluster(fm, orc):
+    """Shared frame: targets exactly on child faces (the half-open lattice boxes tile exactly,
+    so the one-sort build applies) and a source cluster deeper than 10 levels (63-bit paths)."""
+    rng = np.random.default_rng(17)
+    src, w, tgt = _inputs(5, 12000, 3000)
+    src = np.concatenate([src, 0.3 + rng.uniform(-1e-5, 1e-5, (3000, 3))])
+    w = np.full(len(src), 1.0 / len(src))
+    h = 3.0 / 2 ** rng.integers(1, 7, 500)            # faces of the shared lattice, levels 1..6
+    faces = rng.uniform(-2, 2, (500, 3))
+    ax = rng.integers(3, size=500)
+    faces[np.arange(500), ax] = np.round(faces[np.arange(500), ax] / h) * h
+    tgt = np.concatenate([tgt, faces])
+    F = fm.FMM(order=4, source_leaf_capacity=20, target_leaf_capacity=24, frame=1)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    F.build_trees()
+    S = orc.source_tree(src, 20)
+    assert S.level.max() > 10
+    _cmp_tree(F.tree(0), S)
+    _cmp_tree(F.tree(1), orc.target_tree(tgt, 24, root=SHARED_ROOT))
+    st = F.stats()
+    assert st["source_tree_sorted"] == 1 and st["target_tree_sorted"] == 1
 
 
 @pytest.mark.parametrize("frame", [0, 1])

@@ -3,7 +3,7 @@ restated) and interaction maps (fast_multipole_solver.cpp:105-184) by brute forc
 import numpy as np
 import pytest
 
-from conftest import shell_sources
+from conftest import gap_overlap_targets, shell_sources
 
 CHILD_SIGN = np.array([(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (-1, -1, 1), (1, -1, 1), (1, 1, 1),
                        (-1, 1, 1)])
@@ -139,3 +139,17 @@ def test_interaction_lists_cover_every_pair_once(orc, shared):
         for s in close_set.get(b, []):
             hits += np.array([s in anc for anc in src_anc])
         assert np.all(hits == 1), (b, np.unique(hits))
+
+
+def test_gap_overlap_targets_are_duplicated_or_dropped(orc):
+    """The construction used by the GPU fallback test really hits float gaps/overlaps: the
+    reference tree holds some of those points twice or not at all."""
+    rng = np.random.default_rng(23)
+    tgt = np.concatenate([rng.uniform(-2, 2, (3000, 3)), shell_sources(rng, 3000)])
+    aug, nadd = gap_overlap_targets(orc, tgt, 24, rng)
+    assert nadd >= 20
+    T = orc.target_tree(aug, 24)
+    assert np.array_equal(T.center[0], orc.target_tree(tgt, 24).center[0])   # same root
+    counts = np.bincount(T.slots, minlength=len(aug))
+    extra = counts[len(tgt):]
+    assert np.any(extra != 1)
