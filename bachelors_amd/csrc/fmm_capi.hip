@@ -126,12 +126,16 @@ void upload_pass(Pass& P, const std::vector<int>& src, const std::vector<int>& d
                                hipMemcpyHostToDevice, st));
 }
 
-void finish_tables(fmm_ctx* c, OpTables& O) {
+// O.nops operators from device specs, written in the precision the GEMMs apply
+void build_tables(fmm_ctx* c, OpTables& O, const OpSpec* d_specs, hipStream_t st) {
     size_t n = (size_t)O.nops * c->Lp * c->Lp;
     if (!n) return;
-    if (c->prec == Precision::F32) operators_to_f32(O.f64.as<double>(), O.f32.get<float>(n), n, c->stream);
-    if (c->prec == Precision::BF16X3)
-        operators_to_bf16x2(O.f64.as<double>(), O.hi.get<__bf16>(n), O.lo.get<__bf16>(n), n, c->stream);
+    if (c->prec == Precision::F64)
+        make_operators(d_specs, O.nops, c->p, c->Lp, 0, O.f64.get<double>(n), nullptr, st);
+    else if (c->prec == Precision::F32)
+        make_operators(d_specs, O.nops, c->p, c->Lp, 1, O.f32.get<float>(n), nullptr, st);
+    else
+        make_operators(d_specs, O.nops, c->p, c->Lp, 2, O.hi.get<__bf16>(n), O.lo.get<__bf16>(n), st);
 }
 
 const void* ops_ptr(const fmm_ctx* c, const OpTables& O) {
@@ -249,8 +253,7 @@ void stage_upward(fmm_ctx* c) {
     if (O.nops) {
         OpSpec* d = O.specs.get<OpSpec>(O.nops);
         FMM_HIP(hipMemcpyAsync(d, specs.data(), O.nops * sizeof(OpSpec), hipMemcpyHostToDevice, c->stream));
-        make_operators(d, O.nops, c->p, c->Lp, O.f64.get<double>((size_t)O.nops * c->Lp * c->Lp), c->stream);
-        finish_tables(c, O);
+        build_tables(c, O, d, c->stream);
     }
     c->m2m_pass.resize(MAX_LEVELS);
     for (int k = nlev - 1; k >= 1; k--) {
@@ -295,8 +298,7 @@ void run_per_pair(fmm_ctx* c, int kind, const int* d_src, const int* d_dst, int 
         make_pair_specs(d_src + b, d_dst + b, n, Tsrc.center.as<double>(), Tdst.center.as<double>(),
                         Tsrc.level.as<int>(), Tdst.level.as<int>(), sg_src, sg_dst, kind, c->cfg.m2l_full, sp, st);
         O.nops = n;
-        make_operators(sp, n, c->p, c->Lp, O.f64.get<double>((size_t)n * c->Lp * c->Lp), st);
-        finish_tables(c, O);
+        build_tables(c, O, sp, st);
         tiles.resize(n);
         for (int i = 0; i < n; i++) tiles[i] = GemmTile{i, 3 * (b + i), 3};
         GemmTile* dt = c->m2l_pass.tiles.get<GemmTile>(n);
@@ -378,8 +380,7 @@ void stage_downward(fmm_ctx* c) {
                    S.center.as<double>(), T.center.as<double>(), S.level.as<int>(), T.level.as<int>(), c->sigS, c->sigT,
                    c->cfg.m2l_full, d + n_l2l, st);
     if (O.nops) {
-        make_operators(d, O.nops, c->p, c->Lp, O.f64.get<double>((size_t)O.nops * c->Lp * c->Lp), st);
-        finish_tables(c, O);
+        build_tables(c, O, d, st);
     }
     size_t n = (size_t)T.num_nodes * 3 * c->Lp;
     void* Lx = c->Lx.reserve(n * c->esize());
@@ -879,7 +880,7 @@ int fmm_make_operator(int kind, int order, const double* t, double sig_in, doubl
         OpSpec* ds = d.get<OpSpec>(1);
         FMM_HIP(hipMemcpy(ds, &s, sizeof(s), hipMemcpyHostToDevice));
         double* dout = o.get<double>((size_t)Lp * Lp);
-        make_operators(ds, 1, order, Lp, dout, 0);
+        make_operators(ds, 1, order, Lp, 0, dout, nullptr, 0);
         std::vector<double> h((size_t)Lp * Lp);
         FMM_HIP(hipMemcpy(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int i = 0; i < L; i++)

@@ -51,54 +51,63 @@ __global__ void k_gather_xyz(const int* __restrict__ slots, int n, const double*
 
 // ---------------------------------------------------------------- P2M
 // One wave per useful source leaf.  Lane (m, g) owns column m of the harmonic table for
-// the points g, g+G, ...; per-group partial sums live in LDS and are reduced at the end.
-template <class T>
+// the points g, g+G, ... (G = 64 / (P+1)); its 3 x 2 x (P+1) partial sums stay in registers
+// (the order is a template parameter so they index statically) and the G groups are added
+// through LDS at the end.  M_l^{+-m} = sum_j q_j R_l^{+-m}(u_j), u = (y - c) / sigma, with
+// R(l, 0) = (-1)^l Phi_l^0, R(l, +-m) = (-1)^l sqrt2 {Re, Im} Phi_l^m (harmonics.cpp:281-320).
+template <class T, int P>
 __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, const double* __restrict__ center,
                                             const int* __restrict__ begin, const int* __restrict__ end,
                                             const int* __restrict__ level, ScaleTable sig,
-                                            const double* __restrict__ sxyz, const double* __restrict__ sq, int p,
-                                            int Lp, T* __restrict__ M) {
-    extern __shared__ double acc[];   // [G][3][L]
+                                            const double* __restrict__ sxyz, const double* __restrict__ sq, int Lp,
+                                            T* __restrict__ M) {
+    constexpr int P1 = P + 1, G = 64 / P1, L = P1 * P1;
+    __shared__ double part[G][3][L];
     const int leaf = leaves[blockIdx.x];
-    const int L = (p + 1) * (p + 1), P1 = p + 1, G = 64 / P1;
     const int lane = threadIdx.x, m = lane % P1, g = lane / P1;
-    for (int e = lane; e < G * 3 * L; e += 64) acc[e] = 0.0;
-    __syncthreads();
+    double ar[3][P1], ai[3][P1];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int l = 0; l < P1; l++) ar[c][l] = ai[c][l] = 0.0;
     const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
     const double inv = 1.0 / sig.s[level[leaf]];
     if (g < G) {
-        double* a = acc + (size_t)g * 3 * L;
         for (int j = begin[leaf] + g; j < end[leaf]; j += G) {
             double ux = (sxyz[3 * j] - cx) * inv, uy = (sxyz[3 * j + 1] - cy) * inv, uz = (sxyz[3 * j + 2] - cz) * inv;
             double r2 = ux * ux + uy * uy + uz * uz;
             double q0 = sq[3 * j], q1 = sq[3 * j + 1], q2 = sq[3 * j + 2];
-            double dr = 1.0, di = 0.0;   // Phi_m^m
+            double dr = 1.0, di = 0.0;   // Phi_m^m = (x + iy)^m / (2^m m!)
             for (int i = 1; i <= m; i++) {
                 double s = 0.5 / i;
                 double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
-                dr = nr; di = ni;
+                dr = nr;
+                di = ni;
             }
             double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
-            for (int l = m; l <= p; l++) {
-                double vr, vi;
-                if (l == m) { vr = dr; vi = di; }
-                else {
+#pragma unroll
+            for (int l = 0; l < P1; l++) {
+                if (l < m) continue;
+                double vr = dr, vi = di;
+                if (l > m) {   // Phi_l^m = ((2l-1) z Phi_{l-1}^m - r^2 Phi_{l-2}^m) / (l^2 - m^2)
                     double f = 1.0 / (double)(l * l - m * m);
                     vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
                     vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
                     p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
                 }
-                if (l == m) { p1r = vr; p1i = vi; p2r = 0.0; p2i = 0.0; }
-                double s = sgnl(l);
-                if (m == 0) {
-                    int k0 = lmi(l, 0);
-                    a[k0] += s * q0 * vr; a[L + k0] += s * q1 * vr; a[2 * L + k0] += s * q2 * vr;
-                } else {
-                    double sr = s * SQRT2 * vr, si = s * SQRT2 * vi;
-                    int kp = lmi(l, m), kn = lmi(l, -m);
-                    a[kp] += q0 * sr; a[L + kp] += q1 * sr; a[2 * L + kp] += q2 * sr;
-                    a[kn] += q0 * si; a[L + kn] += q1 * si; a[2 * L + kn] += q2 * si;
-                }
+                ar[0][l] += q0 * vr; ar[1][l] += q1 * vr; ar[2][l] += q2 * vr;
+                ai[0][l] += q0 * vi; ai[1][l] += q1 * vi; ai[2][l] += q2 * vi;
+            }
+        }
+        // real basis: (-1)^l for m = 0, (-1)^l sqrt2 (Re, Im) for +-m
+#pragma unroll
+        for (int l = 0; l < P1; l++) {
+            if (l < m) continue;
+            const double s = sgnl(l) * (m == 0 ? 1.0 : SQRT2);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                part[g][c][lmi(l, m)] = s * ar[c][l];
+                if (m > 0) part[g][c][lmi(l, -m)] = s * ai[c][l];
             }
         }
     }
@@ -108,13 +117,15 @@ __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, cons
         int c = e / Lp, k = e - c * Lp;
         double s = 0.0;
         if (k < L)
-            for (int gg = 0; gg < G; gg++) s += acc[(size_t)gg * 3 * L + c * L + k];
+#pragma unroll
+            for (int gg = 0; gg < G; gg++) s += part[gg][c][k];
         out[e] = (T)s;
     }
 }
 
 // ---------------------------------------------------------------- L2P
-// One wave per useful target leaf.  The local expansion is turned into the complex
+// One wave per useful target leaf; lane (m, g) evaluates column m for the targets g, g+G, ...
+// (G = 64 / p) and the p lanes of a target are summed through LDS.  The local expansion is turned into the complex
 // coefficients beta_c (degree <= p-1) of B = sigma^-2 Re sum beta_c(l,m) Phi_l^m(u), using
 // dz Phi_l^m = Phi_{l-1}^m, (dx - i dy) Phi_l^m = Phi_{l-1}^{m-1}, (dx + i dy) Phi_l^m = -Phi_{l-1}^{m+1}.
 __device__ __forceinline__ void alpha_of(const double* Ls, int L, int c, int l, int m, double& ar, double& ai) {
@@ -182,22 +193,30 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
     __syncthreads();
     const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
     const double inv = 1.0 / sig.s[level[leaf]];
-    for (int k = begin[leaf] + lane; k < end[leaf]; k += 64) {
-        double ux = (txyz[3 * k] - cx) * inv, uy = (txyz[3 * k + 1] - cy) * inv, uz = (txyz[3 * k + 2] - cz) * inv;
-        double r2 = ux * ux + uy * uy + uz * uz;
+    const double s2 = inv * inv;
+    // lane (m, g): column m (< p) of the degree <= p-1 table for targets g, g+G, ...
+    const int G = 64 / p, m = lane % p, g = lane / p;
+    double* part = beta + 6 * NB;   // [64][3] partial sums of the lanes
+    const int kb = begin[leaf], n = end[leaf] - kb;
+    for (int it = 0; it * G < n; it++) {
+        const int t = it * G + g;
+        const bool valid = g < G && t < n;
         double b0 = 0, b1 = 0, b2 = 0;
-        double dr = 1.0, di = 0.0;
-        for (int m = 0; m < p; m++) {
-            if (m > 0) {
-                double s = 0.5 / m;
+        if (valid) {
+            const int k = kb + t;
+            double ux = (txyz[3 * k] - cx) * inv, uy = (txyz[3 * k + 1] - cy) * inv, uz = (txyz[3 * k + 2] - cz) * inv;
+            double r2 = ux * ux + uy * uy + uz * uz;
+            double dr = 1.0, di = 0.0;
+            for (int i = 1; i <= m; i++) {
+                double s = 0.5 / i;
                 double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
-                dr = nr; di = ni;
+                dr = nr;
+                di = ni;
             }
             double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
             for (int l = m; l < p; l++) {
-                double vr, vi;
-                if (l == m) { vr = dr; vi = di; }
-                else {
+                double vr = dr, vi = di;
+                if (l > m) {
                     double f = 1.0 / (double)(l * l - m * m);
                     vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
                     vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
@@ -209,10 +228,23 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
                 b2 += beta[(2 * NB + e) * 2] * vr - beta[(2 * NB + e) * 2 + 1] * vi;
             }
         }
-        double s2 = inv * inv;
-        Bfar[3 * k] = b0 * s2;
-        Bfar[3 * k + 1] = b1 * s2;
-        Bfar[3 * k + 2] = b2 * s2;
+        part[3 * lane] = b0;
+        part[3 * lane + 1] = b1;
+        part[3 * lane + 2] = b2;
+        __syncthreads();
+        if (valid && m == 0) {
+            double s0 = 0, s1 = 0, sz = 0;
+            for (int mm = 0; mm < p; mm++) {
+                s0 += part[3 * (lane + mm)];
+                s1 += part[3 * (lane + mm) + 1];
+                sz += part[3 * (lane + mm) + 2];
+            }
+            const int k = kb + t;
+            Bfar[3 * k] = s0 * s2;
+            Bfar[3 * k + 1] = s1 * s2;
+            Bfar[3 * k + 2] = sz * s2;
+        }
+        __syncthreads();
     }
 }
 
@@ -406,24 +438,38 @@ void launch_charges(const int* slots, int n, const double* xyz, const double* w,
 void launch_gather_xyz(const int* slots, int n, const double* xyz, double* out, hipStream_t st) {
     if (n) k_gather_xyz<<<blocks(n), 256, 0, st>>>(slots, n, xyz, out);
 }
+template <class T, int P>
+static void p2m_order(const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
+                      const double* sq, int Lp, T* M, hipStream_t st) {
+    k_p2m<T, P><<<nleaves, 64, 0, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
+                                        S.level.as<int>(), sig, sxyz, sq, Lp, M);
+}
+template <class T>
+static void p2m_dispatch(int p, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig,
+                         const double* sxyz, const double* sq, int Lp, T* M, hipStream_t st) {
+#define FMM_P2M(P) \
+    case P: p2m_order<T, P>(leaves, nleaves, S, sig, sxyz, sq, Lp, M, st); break;
+    switch (p) {
+        FMM_P2M(0) FMM_P2M(1) FMM_P2M(2) FMM_P2M(3) FMM_P2M(4) FMM_P2M(5) FMM_P2M(6) FMM_P2M(7) FMM_P2M(8) FMM_P2M(9)
+        FMM_P2M(10) FMM_P2M(11) FMM_P2M(12) FMM_P2M(13) FMM_P2M(14) FMM_P2M(15) FMM_P2M(16) FMM_P2M(17) FMM_P2M(18)
+        FMM_P2M(19) FMM_P2M(20)
+        default: throw Error(4, "expansion order above 20");
+    }
+#undef FMM_P2M
+}
 void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
                 const double* sq, int p, int Lp, void* M, hipStream_t st) {
     if (!nleaves) return;
-    int G = 64 / (p + 1);
-    size_t lds = (size_t)G * 3 * (p + 1) * (p + 1) * sizeof(double);
-    if (f64)
-        k_p2m<double><<<nleaves, 64, lds, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
-                                                S.level.as<int>(), sig, sxyz, sq, p, Lp, (double*)M);
-    else
-        k_p2m<float><<<nleaves, 64, lds, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
-                                               S.level.as<int>(), sig, sxyz, sq, p, Lp, (float*)M);
+    if (f64) p2m_dispatch<double>(p, leaves, nleaves, S, sig, sxyz, sq, Lp, (double*)M, st);
+    else p2m_dispatch<float>(p, leaves, nleaves, S, sig, sxyz, sq, Lp, (float*)M, st);
     FMM_HIP(hipGetLastError());
 }
 void launch_l2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig, const double* txyz,
                 const void* Lexp, int p, int Lp, double* Bfar, hipStream_t st) {
     if (!nleaves) return;
     int L = (p + 1) * (p + 1), NB = p * (p + 1) / 2;
-    size_t lds = (size_t)(3 * L + 6 * NB) * sizeof(double);
+    size_t lds = (size_t)(3 * L + 6 * NB + 3 * 64) * sizeof(double);
+    if (p < 1) throw Error(1, "order must be at least 1");
     if (f64)
         k_l2p<double><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
                                                 T.level.as<int>(), sig, txyz, (const double*)Lexp, p, Lp, Bfar);

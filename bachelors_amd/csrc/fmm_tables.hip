@@ -56,13 +56,19 @@ __device__ __forceinline__ void lm_of(int i, int& l, int& m) {
     m = i - l * l - l;
 }
 
-// One workgroup per operator.
-__global__ void k_make_ops(const OpSpec* __restrict__ specs, int p, int pI_full, int Lp, double* __restrict__ out) {
+// One workgroup per operator.  LDS: the harmonic table X(n, k>=0), the degree/order of every
+// coefficient index and the per-degree scale factor of the operator kind.
+template <int MODE>   // output: 0 = double, 1 = float, 2 = bf16 hi + bf16 lo (of the float value)
+__global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ specs, int p, int pI_full, int Lp,
+                                                  void* __restrict__ out, __bf16* __restrict__ out_lo) {
     extern __shared__ cplx X[];   // packed (n, k>=0), n <= pH
     const OpSpec sp = specs[blockIdx.x];
     const int kind = sp.kind;
     const int pH = (kind == OPK_M2L) ? (sp.m2l_full ? pI_full : p) : p;
     const int pI = pH;
+    const int L = (p + 1) * (p + 1);
+    short2* lmt = (short2*)(X + (pI_full + 1) * (pI_full + 2) / 2);   // (l, m) of coefficient i
+    double* fac = (double*)(lmt + ((L + 1) & ~1));                     // scale factor per degree
     double su = (kind == OPK_M2M) ? sp.sig_in : sp.sig_out;
     double ux = sp.t[0] / su, uy = sp.t[1] / su, uz = sp.t[2] / su;
     double r2 = ux * ux + uy * uy + uz * uz;
@@ -93,68 +99,63 @@ __global__ void k_make_ops(const OpSpec* __restrict__ specs, int p, int pI_full,
             pm1 = v;
         }
     }
-    __syncthreads();
-    const int L = (p + 1) * (p + 1);
-    const double c = 0.70710678118654752440;
     const double ratio = sp.sig_in / sp.sig_out;
-    double* T = out + (size_t)blockIdx.x * Lp * Lp;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        int l, m;
+        lm_of(i, l, m);
+        lmt[i] = make_short2((short)l, (short)m);
+    }
+    // scaling of the expansions: M2M (sig_c / sig_p)^l (output degree), L2L (sig_c / sig_p)^(a+1)
+    // and M2L (sig_s / sig_t)^a (input degree a)
+    for (int n = threadIdx.x; n <= p; n += blockDim.x)
+        fac[n] = kind == OPK_M2M ? pow(ratio, n) : kind == OPK_L2L ? pow(1.0 / ratio, n + 1) : pow(ratio, n);
+    __syncthreads();
+    const double c = 0.70710678118654752440;
+    const size_t base = (size_t)blockIdx.x * Lp * Lp;
     for (int e = threadIdx.x; e < Lp * Lp; e += blockDim.x) {
-        int i = e / Lp, j = e % Lp;
+        int i = e / Lp, j = e - (e / Lp) * Lp;
         double val = 0;
         if (i < L && j < L) {
-            int l, mr, a, br;
-            lm_of(i, l, mr);
-            lm_of(j, a, br);
-            // complex input v(b) for the real unit vector j
-            int nb = 0;
-            int bs[2];
-            cplx vs[2];
-            if (br == 0) { bs[0] = 0; vs[0] = {1, 0}; nb = 1; }
-            else if (br > 0) { bs[0] = br; vs[0] = {c, 0}; bs[1] = -br; vs[1] = {c, 0}; nb = 2; }
-            else { bs[0] = -br; vs[0] = {0, c}; bs[1] = br; vs[1] = {0, -c}; nb = 2; }
+            const short2 oi = lmt[i], oj = lmt[j];
+            const int l = oi.x, mr = oi.y, a = oj.x, br = oj.y;
+            // complex input v(b) for the real unit vector j: one or two complex orders b0, b1
+            const bool two = br != 0;
+            const int b0 = br >= 0 ? br : -br, b1 = -b0;
+            const cplx v0 = br == 0 ? cplx{1, 0} : br > 0 ? cplx{c, 0} : cplx{0, c};
+            const cplx v1 = br > 0 ? cplx{c, 0} : cplx{0, -c};
             auto out_c = [&](int m) {
-                cplx s = {0, 0};
-                for (int q = 0; q < nb; q++) s = cadd(s, cmul(kernel_entry(kind, X, pI, l, m, a, bs[q]), vs[q]));
+                cplx s = cmul(kernel_entry(kind, X, pI, l, m, a, b0), v0);
+                if (two) s = cadd(s, cmul(kernel_entry(kind, X, pI, l, m, a, b1), v1));
                 return s;
             };
             if (mr == 0) val = out_c(0).re;
             else if (mr > 0) { cplx o1 = out_c(mr), o2 = out_c(-mr); val = c * (o1.re + o2.re); }
             else { cplx o1 = out_c(-mr), o2 = out_c(mr); val = c * (o1.im - o2.im); }
-            // scaling of the expansions
-            if (kind == OPK_M2M) val *= pow(ratio, l);             // (sig_c / sig_p)^l
-            else if (kind == OPK_L2L) val *= pow(1.0 / ratio, a + 1); // (sig_c / sig_p)^(a+1)
-            else val *= pow(ratio, a);                             // (sig_s / sig_t)^a
+            val *= fac[kind == OPK_M2M ? l : a];
         }
-        T[e] = val;
+        if (MODE == 0) {
+            ((double*)out)[base + e] = val;
+        } else if (MODE == 1) {
+            ((float*)out)[base + e] = (float)val;
+        } else {
+            float v = (float)val;
+            __bf16 h = (__bf16)v;
+            ((__bf16*)out)[base + e] = h;
+            out_lo[base + e] = (__bf16)(v - (float)h);
+        }
     }
 }
 
-__global__ void k_ops_to_f32(const double* __restrict__ in, float* __restrict__ out, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (float)in[i];
-}
-__global__ void k_ops_to_bf16x2(const double* __restrict__ in, __bf16* __restrict__ hi, __bf16* __restrict__ lo,
-                                size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float v = (float)in[i];
-    __bf16 h = (__bf16)v;
-    hi[i] = h;
-    lo[i] = (__bf16)(v - (float)h);
-}
-
-void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, double* d_out, hipStream_t st) {
+void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, int out_mode, void* d_out, __bf16* d_out_lo,
+                    hipStream_t st) {
     if (nops == 0) return;
-    int pI = 2 * p;
-    size_t lds = (size_t)(pI + 1) * (pI + 2) / 2 * sizeof(cplx);
-    k_make_ops<<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out);
+    int pI = 2 * p, L = (p + 1) * (p + 1);
+    size_t lds = (size_t)(pI + 1) * (pI + 2) / 2 * sizeof(cplx) + (size_t)((L + 1) & ~1) * sizeof(short2) +
+                 (size_t)(p + 1) * sizeof(double);
+    if (out_mode == 0) k_make_ops<0><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr);
+    else if (out_mode == 1) k_make_ops<1><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr);
+    else k_make_ops<2><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, d_out_lo);
     FMM_HIP(hipGetLastError());
-}
-void operators_to_f32(const double* in, float* out, size_t n, hipStream_t st) {
-    if (n) k_ops_to_f32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(in, out, n);
-}
-void operators_to_bf16x2(const double* in, __bf16* hi, __bf16* lo, size_t n, hipStream_t st) {
-    if (n) k_ops_to_bf16x2<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(in, hi, lo, n);
 }
 
 }  // namespace fmm

@@ -15,10 +15,11 @@ struct OpSpec {
     int m2l_full;   // M2L keeps l + lambda <= 2p (else the reference's l + lambda <= p)
 };
 
-// d_out: nops x Lp x Lp doubles, row-major [out][in], zero padded.
-void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, double* d_out, hipStream_t st);
-void operators_to_f32(const double* in, float* out, size_t n, hipStream_t st);
-void operators_to_bf16x2(const double* in, __bf16* hi, __bf16* lo, size_t n, hipStream_t st);
+// nops operators of Lp x Lp, row-major [out][in], zero padded, computed in FP64 and written as
+// out_mode 0: double (d_out), 1: float (d_out), 2: bf16 hi (d_out) + bf16 lo (d_out_lo) of the
+// float value.
+void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, int out_mode, void* d_out, __bf16* d_out_lo,
+                    hipStream_t st);
 void make_m2l_specs(const int* class_begin, int nc, const int* far_t, const int* far_s, const double* sc,
                     const double* tc, const int* slev, const int* tlev, const ScaleTable& sigS, const ScaleTable& sigT,
                     int m2l_full, OpSpec* out, hipStream_t st);
