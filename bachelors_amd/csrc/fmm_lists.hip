@@ -10,28 +10,44 @@
 
 namespace fmm {
 
-struct TreeView {
-    const double* center;
-    const int* first_child;
-    const int* npts;
-    const int* level;
+// One 32-byte record per node (centre, first child, level and emptiness) so a node is one
+// sector read in the traversal.
+struct NodeRec {
+    double c[3];
+    int fc;     // first child or -1
+    int info;   // level | 32 if the node holds points (leaf with npts > 0)
 };
 struct RadiusTable {
     double r[MAX_LEVELS];
 };
 
-__device__ __forceinline__ bool nonempty(const TreeView& T, int n) { return T.first_child[n] >= 0 || T.npts[n] > 0; }
-__device__ __forceinline__ bool useful(const TreeView& T, int n) { return T.first_child[n] < 0 && T.npts[n] > 0; }
+__global__ void k_node_records(const double* __restrict__ center, const int* __restrict__ first_child,
+                               const int* __restrict__ npts, const int* __restrict__ level, int n,
+                               NodeRec* __restrict__ rec) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    NodeRec r;
+    r.c[0] = center[3 * i];
+    r.c[1] = center[3 * i + 1];
+    r.c[2] = center[3 * i + 2];
+    r.fc = first_child[i];
+    r.info = level[i] | (npts[i] > 0 ? 32 : 0);
+    rec[i] = r;
+}
 
-__device__ __forceinline__ bool far_enough(const TreeView& S, const RadiusTable& RS, int s, const TreeView& T,
-                                           const RadiusTable& RT, int t, double mac_scale) {
-    double r1 = RS.r[S.level[s]], r2 = RT.r[T.level[t]];
+__device__ __forceinline__ int lvl(const NodeRec& r) { return r.info & 31; }
+__device__ __forceinline__ bool nonempty(const NodeRec& r) { return r.fc >= 0 || (r.info & 32); }
+__device__ __forceinline__ bool useful(const NodeRec& r) { return r.fc < 0 && (r.info & 32); }
+
+__device__ __forceinline__ bool far_enough(const NodeRec& s, const RadiusTable& RS, const NodeRec& t,
+                                           const RadiusTable& RT, double mac_scale) {
+    double r1 = RS.r[lvl(s)], r2 = RT.r[lvl(t)];
     double mn = fmin(r1, r2), mx = fmax(r1, r2);
     double md = __dadd_rn(__dmul_rn(2.0, mx), mn);
     md = __dmul_rn(md, mac_scale);
-    double dx = __dsub_rn(S.center[3 * s], T.center[3 * t]);
-    double dy = __dsub_rn(S.center[3 * s + 1], T.center[3 * t + 1]);
-    double dz = __dsub_rn(S.center[3 * s + 2], T.center[3 * t + 2]);
+    double dx = __dsub_rn(s.c[0], t.c[0]);
+    double dy = __dsub_rn(s.c[1], t.c[1]);
+    double dz = __dsub_rn(s.c[2], t.c[2]);
     double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
     return __dmul_rn(md, md) < d2;
 }
@@ -51,76 +67,86 @@ __device__ __forceinline__ unsigned wave_reserve(unsigned* cnt, unsigned n) {
     return base + incl - n;
 }
 
-// frontier entries are (s, t); far/close outputs are (t, s).  Every lane of the wave
-// reaches the appends (threads past the end contribute zero items).
-__global__ __launch_bounds__(256) void k_lists_step(const int2* __restrict__ front, int nf, TreeView S, RadiusTable RS,
-                                                    TreeView T, RadiusTable RT, double mac_scale, int2* far_out,
-                                                    unsigned* far_cnt, unsigned far_cap, int2* close_out,
-                                                    unsigned* close_cnt, unsigned close_cap, int2* next,
-                                                    unsigned* next_cnt, unsigned next_cap) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int nfar = 0, nclose = 0, nnext = 0;
-    int s = 0, t = 0, fc = -1;
-    bool desc_s = false;
-    if (i < nf) {
-        s = front[i].x;
-        t = front[i].y;
-        if (far_enough(S, RS, s, T, RT, t, mac_scale)) {
-            nfar = 1;
-        } else if (useful(T, t)) {
-            bool close_enough = true;
-            fc = S.first_child[s];
-            if (fc >= 0)
-                for (int o = 0; o < 8; o++)
-                    if (far_enough(S, RS, fc + o, T, RT, t, mac_scale)) { close_enough = false; break; }
-            if (close_enough) nclose = 1;
-            else desc_s = true;
-        } else {
-            double rt = RT.r[T.level[t]], rs = RS.r[S.level[s]];
-            if ((rt < rs || useful(T, t)) && !useful(S, s)) { desc_s = true; fc = S.first_child[s]; }
-            else fc = T.first_child[t];
-        }
-        if (nfar == 0 && nclose == 0) {
-            for (int o = 0; o < 8; o++) nnext += desc_s ? nonempty(S, fc + o) : nonempty(T, fc + o);
-        }
-    }
-    unsigned pf = wave_reserve(far_cnt, nfar);
-    unsigned pc = wave_reserve(close_cnt, nclose);
-    unsigned pn = wave_reserve(next_cnt, nnext);
-    if (nfar && pf < far_cap) far_out[pf] = make_int2(t, s);
-    if (nclose && pc < close_cap) close_out[pc] = make_int2(t, s);
-    if (nnext) {
-        for (int o = 0; o < 8; o++) {
-            if (desc_s) {
-                if (nonempty(S, fc + o)) { if (pn < next_cap) next[pn] = make_int2(fc + o, t); pn++; }
+// One traversal step over the frontier (s, t) written by the previous step; its size is read
+// on the device (*front_cnt, capped at front_cap), so steps are queued without host syncs and
+// an exhausted frontier makes the remaining steps no-ops.  Far/close outputs are (t, s).
+// The grid strides over the frontier in whole workgroups, so every lane of a wave reaches
+// the appends (lanes past the end contribute zero items).
+__global__ __launch_bounds__(256) void k_lists_step(const int2* __restrict__ front, const unsigned* __restrict__ front_cnt,
+                                                    unsigned front_cap, const NodeRec* __restrict__ S, RadiusTable RS,
+                                                    const NodeRec* __restrict__ T, RadiusTable RT, double mac_scale,
+                                                    int2* far_out, unsigned* far_cnt, unsigned far_cap,
+                                                    int2* close_out, unsigned* close_cnt, unsigned close_cap,
+                                                    int2* next, unsigned* next_cnt, unsigned next_cap) {
+    const unsigned nf = min(*front_cnt, front_cap);
+    for (unsigned base = blockIdx.x * blockDim.x; base < nf; base += gridDim.x * blockDim.x) {
+        const unsigned i = base + threadIdx.x;
+        int nfar = 0, nclose = 0, nnext = 0;
+        int s = 0, t = 0, fc = -1;
+        bool desc_s = false;
+        if (i < nf) {
+            s = front[i].x;
+            t = front[i].y;
+            const NodeRec rs = S[s], rt = T[t];
+            if (far_enough(rs, RS, rt, RT, mac_scale)) {
+                nfar = 1;
+            } else if (useful(rt)) {
+                bool close_enough = true;
+                fc = rs.fc;
+                if (fc >= 0)
+                    for (int o = 0; o < 8; o++)
+                        if (far_enough(S[fc + o], RS, rt, RT, mac_scale)) { close_enough = false; break; }
+                if (close_enough) nclose = 1;
+                else desc_s = true;
             } else {
-                if (nonempty(T, fc + o)) { if (pn < next_cap) next[pn] = make_int2(s, fc + o); pn++; }
+                double radt = RT.r[lvl(rt)], rads = RS.r[lvl(rs)];
+                if ((radt < rads || useful(rt)) && !useful(rs)) { desc_s = true; fc = rs.fc; }
+                else fc = rt.fc;
+            }
+            if (nfar == 0 && nclose == 0)
+                for (int o = 0; o < 8; o++) nnext += desc_s ? nonempty(S[fc + o]) : nonempty(T[fc + o]);
+        }
+        unsigned pf = wave_reserve(far_cnt, nfar);
+        unsigned pc = wave_reserve(close_cnt, nclose);
+        unsigned pn = wave_reserve(next_cnt, nnext);
+        if (nfar && pf < far_cap) far_out[pf] = make_int2(t, s);
+        if (nclose && pc < close_cap) close_out[pc] = make_int2(t, s);
+        if (nnext) {
+            for (int o = 0; o < 8; o++) {
+                if (desc_s) {
+                    if (nonempty(S[fc + o])) { if (pn < next_cap) next[pn] = make_int2(fc + o, t); pn++; }
+                } else {
+                    if (nonempty(T[fc + o])) { if (pn < next_cap) next[pn] = make_int2(s, fc + o); pn++; }
+                }
             }
         }
     }
 }
 
-// 64-bit sort key: high word = class code (or 0), low word = target; value = source
-__global__ void k_far_keys(const int2* __restrict__ far, int n, TreeView S, TreeView T, const double* __restrict__ hs,
-                           const double* __restrict__ ht, int use_classes, uint64_t* __restrict__ key,
-                           int* __restrict__ val, unsigned* __restrict__ err) {
+// Far-pair sort key: class code (level difference and centre offset in units of the finer
+// half size) above the target index (target_bits wide); value = source
+__global__ void k_far_keys(const int2* __restrict__ far, int n, const NodeRec* __restrict__ S,
+                           const NodeRec* __restrict__ T, const double* __restrict__ hs, const double* __restrict__ ht,
+                           int use_classes, int target_bits, uint64_t* __restrict__ key, int* __restrict__ val,
+                           unsigned* __restrict__ err) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int t = far[i].x, s = far[i].y;
-    uint32_t code = 0;
+    uint64_t code = 0;
     if (use_classes) {
-        int ls = S.level[s], lt = T.level[t];
+        const NodeRec rs = S[s], rt = T[t];
+        int ls = lvl(rs), lt = lvl(rt);
         double hf = fmin(hs[ls], ht[lt]);   // finer of the two boxes (physical size, not tree level)
-        int nx = (int)rint((S.center[3 * s] - T.center[3 * t]) / hf);
-        int ny = (int)rint((S.center[3 * s + 1] - T.center[3 * t + 1]) / hf);
-        int nz = (int)rint((S.center[3 * s + 2] - T.center[3 * t + 2]) / hf);
+        int nx = (int)rint((rs.c[0] - rt.c[0]) / hf);
+        int ny = (int)rint((rs.c[1] - rt.c[1]) / hf);
+        int nz = (int)rint((rs.c[2] - rt.c[2]) / hf);
         int d = ls - lt;
         if (nx < -127 || nx > 127 || ny < -127 || ny > 127 || nz < -127 || nz > 127 || d < -31 || d > 31)
             atomicOr(err, 1u);
-        code = ((uint32_t)(d + 32) << 24) | ((uint32_t)(nx + 128) << 16) | ((uint32_t)(ny + 128) << 8) |
-               (uint32_t)(nz + 128);
+        code = ((uint64_t)(d + 32) << 24) | ((uint64_t)(nx + 128) << 16) | ((uint64_t)(ny + 128) << 8) |
+               (uint64_t)(nz + 128);
     }
-    key[i] = ((uint64_t)code << 32) | (uint32_t)t;
+    key[i] = (code << target_bits) | (uint32_t)t;
     val[i] = s;
 }
 
@@ -139,17 +165,17 @@ __global__ void k_unique_flags(const uint64_t* __restrict__ key, const int* __re
     flag[i] = (i == 0 || key[i] != key[i - 1] || val[i] != val[i - 1]) ? 1u : 0u;
 }
 
-// far keys carry the target in the low word (class code above), close keys in the high word
+// far keys carry the target in the low target_bits (class code above), close keys in the high word
 __global__ void k_compact_pairs(const uint64_t* __restrict__ key, const int* __restrict__ val,
                                 const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int n,
-                                int target_high, int* __restrict__ out_t, int* __restrict__ out_s,
+                                int target_high, int target_bits, int* __restrict__ out_t, int* __restrict__ out_s,
                                 uint32_t* __restrict__ out_code) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !flag[i]) return;
     uint32_t p = pos[i];
-    out_t[p] = target_high ? (int)(uint32_t)(key[i] >> 32) : (int)(uint32_t)(key[i] & 0xffffffffu);
+    out_t[p] = target_high ? (int)(uint32_t)(key[i] >> 32) : (int)(uint32_t)(key[i] & ((1ull << target_bits) - 1));
     out_s[p] = val[i];
-    if (out_code) out_code[p] = (uint32_t)(key[i] >> 32);
+    if (out_code) out_code[p] = (uint32_t)(key[i] >> target_bits);
 }
 
 // segment boundaries: class starts (far) / target starts (close)
@@ -177,9 +203,6 @@ __global__ void k_fill_int(int* p, int n, int v) {
 }
 
 
-static TreeView view(const TreeDev& T) {
-    return TreeView{T.center.as<double>(), T.first_child.as<int>(), T.npts.as<int>(), T.level.as<int>()};
-}
 static RadiusTable radii(const TreeDev& T) {
     RadiusTable r;
     for (int k = 0; k < MAX_LEVELS; k++) r.r[k] = T.radius[k];
@@ -193,9 +216,16 @@ static uint32_t read_u32(const void* p, hipStream_t st) {
     return v;
 }
 
-// sort by 64-bit key, drop duplicate (key, value) pairs; returns the unique count
-static int sort_unique(ListScratch& L, int n, int end_bit, int target_high, int* out_t, int* out_s, uint32_t* out_code,
-                       hipStream_t st) {
+static int bits_for(int n) {   // smallest b with 2^b >= n (at least 1)
+    int b = 1;
+    while (b < 31 && (1 << b) < n) b++;
+    return b;
+}
+
+// sort by the low end_bit bits of the 64-bit key, drop duplicate (key, value) pairs; returns
+// the unique count
+static int sort_unique(ListScratch& L, int n, int end_bit, int target_high, int target_bits, int* out_t, int* out_s,
+                       uint32_t* out_code, hipStream_t st) {
     if (n == 0) return 0;
     uint64_t* k2 = L.key2.get<uint64_t>(n);
     int* v2 = L.val2.get<int>(n);
@@ -205,15 +235,25 @@ static int sort_unique(ListScratch& L, int n, int end_bit, int target_high, int*
     k_unique_flags<<<blocks(n), 256, 0, st>>>(k2, v2, n, flag);
     exclusive_scan_u32(flag, pos, n, L.tmp, st);
     int nu = (int)(read_u32(pos + n - 1, st) + read_u32(flag + n - 1, st));
-    k_compact_pairs<<<blocks(n), 256, 0, st>>>(k2, v2, flag, pos, n, target_high, out_t, out_s, out_code);
+    k_compact_pairs<<<blocks(n), 256, 0, st>>>(k2, v2, flag, pos, n, target_high, target_bits, out_t, out_s, out_code);
     return nu;
 }
 
 void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T, double mac_scale, bool classes,
                  hipStream_t st) {
-    TreeView sv = view(S), tv = view(T);
     RadiusTable rs = radii(S), rt = radii(T);
-    unsigned* cnt = L.counters.get<unsigned>(4);
+    NodeRec* srec = L.srec.get<NodeRec>(std::max(S.num_nodes, 1));
+    NodeRec* trec = L.trec.get<NodeRec>(std::max(T.num_nodes, 1));
+    if (S.num_nodes)
+        k_node_records<<<blocks(S.num_nodes), 256, 0, st>>>(S.center.as<double>(), S.first_child.as<int>(),
+                                                            S.npts.as<int>(), S.level.as<int>(), S.num_nodes, srec);
+    if (T.num_nodes)
+        k_node_records<<<blocks(T.num_nodes), 256, 0, st>>>(T.center.as<double>(), T.first_child.as<int>(),
+                                                            T.npts.as<int>(), T.level.as<int>(), T.num_nodes, trec);
+    // every step descends one of the two trees by a level: the traversal ends within this many
+    const int max_steps = std::max(S.num_levels, 1) + std::max(T.num_levels, 1);
+    // counters: far, close, error flag, then one frontier count per step
+    unsigned* cnt = L.counters.get<unsigned>(4 + max_steps + 1);
     unsigned far_cap = std::max<unsigned>(L.far_cap, 1u << 16), close_cap = std::max<unsigned>(L.close_cap, 1u << 16),
              front_cap = std::max<unsigned>(L.front_cap, 1u << 16);
     for (int attempt = 0; attempt < 8; attempt++) {
@@ -221,65 +261,66 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
         int2* close = L.close.get<int2>(close_cap);
         int2* fa = L.frontA.get<int2>(front_cap);
         int2* fb = L.frontB.get<int2>(front_cap);
-        FMM_HIP(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned), st));
-        int2 root = make_int2(0, 0);
-        int nf = 0;
+        FMM_HIP(hipMemsetAsync(cnt, 0, (4 + max_steps + 1) * sizeof(unsigned), st));
         // the reference returns at once if either root is empty (fast_multipole_solver.cpp:206-210)
         bool s_ok = !S.h_first_child.empty() && (S.h_first_child[0] >= 0 || S.h_npts[0] > 0);
         bool t_ok = !T.h_first_child.empty() && (T.h_first_child[0] >= 0 || T.h_npts[0] > 0);
         if (s_ok && t_ok) {
-            FMM_HIP(hipMemcpyAsync(fa, &root, sizeof(int2), hipMemcpyHostToDevice, st));
-            nf = 1;
-        }
-        bool overflow = false;
-        int steps = 0;
-        while (nf > 0) {
-            FMM_HIP(hipMemsetAsync(cnt + 2, 0, sizeof(unsigned), st));
-            k_lists_step<<<blocks(nf), 256, 0, st>>>(fa, nf, sv, rs, tv, rt, mac_scale, far, cnt + 0, far_cap, close,
-                                                     cnt + 1, close_cap, fb, cnt + 2, front_cap);
-            unsigned h[3];
-            FMM_HIP(hipMemcpyAsync(h, cnt, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
-            FMM_HIP(hipStreamSynchronize(st));
-            steps++;
-            if (h[0] > far_cap || h[1] > close_cap || h[2] > front_cap) {
-                overflow = true;
-                far_cap = std::max(far_cap, 2 * h[0]);
-                close_cap = std::max(close_cap, 2 * h[1]);
-                front_cap = std::max(front_cap, 2 * h[2]);
-                break;
+            L.h_seed[0] = 0;
+            L.h_seed[1] = 0;
+            L.h_seed[2] = 1;
+            FMM_HIP(hipMemcpyAsync(fa, L.h_seed, sizeof(int2), hipMemcpyHostToDevice, st));
+            FMM_HIP(hipMemcpyAsync(cnt + 4, L.h_seed + 2, sizeof(unsigned), hipMemcpyHostToDevice, st));
+            // steps queue without host synchronisation; the grid strides over any frontier size
+            const int grid = 2048;
+            for (int k = 0; k < max_steps; k++) {
+                k_lists_step<<<grid, 256, 0, st>>>(fa, cnt + 4 + k, front_cap, srec, rs, trec, rt, mac_scale, far,
+                                                   cnt + 0, far_cap, close, cnt + 1, close_cap, fb, cnt + 4 + k + 1,
+                                                   front_cap);
+                std::swap(fa, fb);
             }
-            nf = (int)h[2];
-            std::swap(fa, fb);
         }
-        if (overflow) continue;
+        std::vector<unsigned> h(4 + max_steps + 1);
+        FMM_HIP(hipMemcpyAsync(h.data(), cnt, h.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipStreamSynchronize(st));
+        unsigned fmax = 0;
+        int steps = 0;
+        for (int k = 0; k <= max_steps; k++) {
+            fmax = std::max(fmax, h[4 + k]);
+            if (h[4 + k]) steps = k + 1;
+        }
+        if (h[4 + max_steps] != 0) throw Error(4, "interaction-list traversal did not terminate");
+        if (h[0] > far_cap || h[1] > close_cap || fmax > front_cap) {
+            far_cap = std::max(far_cap, 2 * h[0]);
+            close_cap = std::max(close_cap, 2 * h[1]);
+            front_cap = std::max(front_cap, 2 * fmax);
+            continue;
+        }
         L.far_cap = far_cap;
         L.close_cap = close_cap;
         L.front_cap = front_cap;
-        unsigned h[2];
-        FMM_HIP(hipMemcpyAsync(h, cnt, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
-        FMM_HIP(hipStreamSynchronize(st));
         int nfar = (int)h[0], nclose = (int)h[1];
         out.bfs_steps = steps;
         // ---- far pairs: sort by (class, target), de-duplicate, segment into classes
         {
             uint64_t* key = L.key.get<uint64_t>(std::max(nfar, 1));
             int* val = L.val.get<int>(std::max(nfar, 1));
-            unsigned* err = cnt + 3;
-            FMM_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), st));
-            DevBuf& hsb = L.hs;
-            double hsv[MAX_LEVELS], htv[MAX_LEVELS];
-            for (int k = 0; k < MAX_LEVELS; k++) { hsv[k] = S.half[k][0]; htv[k] = T.half[k][0]; }
-            double* dh = hsb.get<double>(2 * MAX_LEVELS);
-            FMM_HIP(hipMemcpyAsync(dh, hsv, sizeof(hsv), hipMemcpyHostToDevice, st));
-            FMM_HIP(hipMemcpyAsync(dh + MAX_LEVELS, htv, sizeof(htv), hipMemcpyHostToDevice, st));
+            unsigned* err = cnt + 2;
+            double* dh = L.hs.get<double>(2 * MAX_LEVELS);
+            for (int k = 0; k < MAX_LEVELS; k++) {
+                L.h_half[k] = S.half[k][0];
+                L.h_half[MAX_LEVELS + k] = T.half[k][0];
+            }
+            FMM_HIP(hipMemcpyAsync(dh, L.h_half, 2 * MAX_LEVELS * sizeof(double), hipMemcpyHostToDevice, st));
+            const int tbits = bits_for(T.num_nodes);
             if (nfar > 0)
-                k_far_keys<<<blocks(nfar), 256, 0, st>>>(far, nfar, sv, tv, dh, dh + MAX_LEVELS, classes ? 1 : 0, key,
-                                                         val, err);
+                k_far_keys<<<blocks(nfar), 256, 0, st>>>(far, nfar, srec, trec, dh, dh + MAX_LEVELS, classes ? 1 : 0,
+                                                         tbits, key, val, err);
             if (classes && read_u32(err, st)) throw Error(4, "interaction offset out of the class-key range");
             int* ft = out.far_t.get<int>(std::max(nfar, 1));
             int* fs = out.far_s.get<int>(std::max(nfar, 1));
             uint32_t* code = out.far_code.get<uint32_t>(std::max(nfar, 1));
-            int nu = sort_unique(L, nfar, classes ? 62 : 32, 0, ft, fs, code, st);
+            int nu = sort_unique(L, nfar, tbits + (classes ? 30 : 0), 0, tbits, ft, fs, code, st);
             out.num_far = nu;
             // classes
             out.num_classes = 0;
@@ -310,7 +351,7 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
             if (nclose > 0) k_close_keys<<<blocks(nclose), 256, 0, st>>>(close, nclose, key, val);
             int* ct = out.close_t.get<int>(std::max(nclose, 1));
             int* cs = out.close_s.get<int>(std::max(nclose, 1));
-            int nu = sort_unique(L, nclose, 64, 1, ct, cs, nullptr, st);
+            int nu = sort_unique(L, nclose, 64, 1, 32, ct, cs, nullptr, st);
             out.num_close = nu;
             int* cb = out.close_begin.get<int>(std::max(T.num_nodes, 1));
             int* ce = out.close_end.get<int>(std::max(T.num_nodes, 1));
