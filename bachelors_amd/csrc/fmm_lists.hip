@@ -52,74 +52,105 @@ __device__ __forceinline__ bool far_enough(const NodeRec& s, const RadiusTable& 
     return __dmul_rn(md, md) < d2;
 }
 
-// wave-aggregated append: one atomic per wave and buffer
-__device__ __forceinline__ unsigned wave_reserve(unsigned* cnt, unsigned n) {
-    const int lane = threadIdx.x & 63;
-    unsigned incl = n;
-    for (int d = 1; d < 64; d <<= 1) {
-        unsigned v = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += v;
+// fate of one (s, t) pair of the traversal (fast_multipole_solver.cpp:105-165): far, close, or
+// expanded into the non-empty children of s (desc_s) or of t (mask: which children)
+struct Fate {
+    int nfar, nclose, nnext, fc, mask;
+    bool desc_s;
+};
+__device__ __forceinline__ Fate pair_fate(int s, int t, const NodeRec* __restrict__ S, const RadiusTable& RS,
+                                          const NodeRec* __restrict__ T, const RadiusTable& RT, double mac_scale) {
+    Fate f = {0, 0, 0, -1, 0, false};
+    const NodeRec rs = S[s], rt = T[t];
+    if (far_enough(rs, RS, rt, RT, mac_scale)) {
+        f.nfar = 1;
+        return f;
     }
-    unsigned total = __shfl(incl, 63, 64);
-    unsigned base = 0;
-    if (lane == 63 && total) base = atomicAdd(cnt, total);
-    base = __shfl(base, 63, 64);
-    return base + incl - n;
+    if (useful(rt)) {
+        bool close_enough = true;
+        if (rs.fc >= 0)
+            for (int o = 0; o < 8; o++)
+                if (far_enough(S[rs.fc + o], RS, rt, RT, mac_scale)) { close_enough = false; break; }
+        if (close_enough) {
+            f.nclose = 1;
+            return f;
+        }
+        f.desc_s = true;
+        f.fc = rs.fc;
+    } else {
+        double radt = RT.r[lvl(rt)], rads = RS.r[lvl(rs)];
+        if ((radt < rads || useful(rt)) && !useful(rs)) { f.desc_s = true; f.fc = rs.fc; }
+        else f.fc = rt.fc;
+    }
+    for (int o = 0; o < 8; o++)
+        if (f.desc_s ? nonempty(S[f.fc + o]) : nonempty(T[f.fc + o])) f.mask |= 1 << o;
+    f.nnext = __popc(f.mask);
+    return f;
+}
+
+// exclusive scan of three counts over the workgroup (256 threads); returns the totals
+__device__ __forceinline__ void block_scan3(unsigned& a, unsigned& b, unsigned& c, unsigned tot[3]) {
+    __shared__ unsigned wsum[4][3];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned ia = a, ib = b, ic = c;
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned va = __shfl_up(ia, d, 64), vb = __shfl_up(ib, d, 64), vc = __shfl_up(ic, d, 64);
+        if (lane >= d) { ia += va; ib += vb; ic += vc; }
+    }
+    if (lane == 63) { wsum[w][0] = ia; wsum[w][1] = ib; wsum[w][2] = ic; }
+    __syncthreads();
+    unsigned oa = 0, ob = 0, oc = 0;
+    for (int k = 0; k < w; k++) { oa += wsum[k][0]; ob += wsum[k][1]; oc += wsum[k][2]; }
+    tot[0] = wsum[0][0] + wsum[1][0] + wsum[2][0] + wsum[3][0];
+    tot[1] = wsum[0][1] + wsum[1][1] + wsum[2][1] + wsum[3][1];
+    tot[2] = wsum[0][2] + wsum[1][2] + wsum[2][2] + wsum[3][2];
+    a = oa + ia - a;
+    b = ob + ib - b;
+    c = oc + ic - c;
 }
 
 // One traversal step over the frontier (s, t) written by the previous step; its size is read
 // on the device (*front_cnt, capped at front_cap), so steps are queued without host syncs and
-// an exhausted frontier makes the remaining steps no-ops.  Far/close outputs are (t, s).
-// The grid strides over the frontier in whole workgroups, so every lane of a wave reaches
-// the appends (lanes past the end contribute zero items).
+// an exhausted frontier makes the remaining steps no-ops.  Each workgroup takes a contiguous
+// chunk: pass 1 counts its far/close/next outputs, one atomic per counter reserves space,
+// pass 2 recomputes the fates and writes.  Far/close outputs are (t, s).
 __global__ __launch_bounds__(256) void k_lists_step(const int2* __restrict__ front, const unsigned* __restrict__ front_cnt,
                                                     unsigned front_cap, const NodeRec* __restrict__ S, RadiusTable RS,
                                                     const NodeRec* __restrict__ T, RadiusTable RT, double mac_scale,
                                                     int2* far_out, unsigned* far_cnt, unsigned far_cap,
                                                     int2* close_out, unsigned* close_cnt, unsigned close_cap,
                                                     int2* next, unsigned* next_cnt, unsigned next_cap) {
+    __shared__ unsigned base[3];
     const unsigned nf = min(*front_cnt, front_cap);
-    for (unsigned base = blockIdx.x * blockDim.x; base < nf; base += gridDim.x * blockDim.x) {
-        const unsigned i = base + threadIdx.x;
-        int nfar = 0, nclose = 0, nnext = 0;
-        int s = 0, t = 0, fc = -1;
-        bool desc_s = false;
-        if (i < nf) {
-            s = front[i].x;
-            t = front[i].y;
-            const NodeRec rs = S[s], rt = T[t];
-            if (far_enough(rs, RS, rt, RT, mac_scale)) {
-                nfar = 1;
-            } else if (useful(rt)) {
-                bool close_enough = true;
-                fc = rs.fc;
-                if (fc >= 0)
-                    for (int o = 0; o < 8; o++)
-                        if (far_enough(S[fc + o], RS, rt, RT, mac_scale)) { close_enough = false; break; }
-                if (close_enough) nclose = 1;
-                else desc_s = true;
-            } else {
-                double radt = RT.r[lvl(rt)], rads = RS.r[lvl(rs)];
-                if ((radt < rads || useful(rt)) && !useful(rs)) { desc_s = true; fc = rs.fc; }
-                else fc = rt.fc;
+    const unsigned per = (nf + gridDim.x - 1) / gridDim.x;
+    const unsigned b0 = blockIdx.x * per, b1 = min(nf, b0 + per);
+    if (b0 >= b1) return;   // uniform over the workgroup
+    unsigned cf = 0, cc = 0, cn = 0;
+    for (unsigned i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+        const Fate f = pair_fate(front[i].x, front[i].y, S, RS, T, RT, mac_scale);
+        cf += f.nfar;
+        cc += f.nclose;
+        cn += f.nnext;
+    }
+    unsigned tot[3];
+    block_scan3(cf, cc, cn, tot);
+    if (threadIdx.x == 0) {
+        base[0] = tot[0] ? atomicAdd(far_cnt, tot[0]) : 0;
+        base[1] = tot[1] ? atomicAdd(close_cnt, tot[1]) : 0;
+        base[2] = tot[2] ? atomicAdd(next_cnt, tot[2]) : 0;
+    }
+    __syncthreads();
+    unsigned pf = base[0] + cf, pc = base[1] + cc, pn = base[2] + cn;
+    for (unsigned i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+        const int s = front[i].x, t = front[i].y;
+        const Fate f = pair_fate(s, t, S, RS, T, RT, mac_scale);
+        if (f.nfar) { if (pf < far_cap) far_out[pf] = make_int2(t, s); pf++; }
+        if (f.nclose) { if (pc < close_cap) close_out[pc] = make_int2(t, s); pc++; }
+        for (int o = 0; o < 8; o++)
+            if (f.mask & (1 << o)) {
+                if (pn < next_cap) next[pn] = f.desc_s ? make_int2(f.fc + o, t) : make_int2(s, f.fc + o);
+                pn++;
             }
-            if (nfar == 0 && nclose == 0)
-                for (int o = 0; o < 8; o++) nnext += desc_s ? nonempty(S[fc + o]) : nonempty(T[fc + o]);
-        }
-        unsigned pf = wave_reserve(far_cnt, nfar);
-        unsigned pc = wave_reserve(close_cnt, nclose);
-        unsigned pn = wave_reserve(next_cnt, nnext);
-        if (nfar && pf < far_cap) far_out[pf] = make_int2(t, s);
-        if (nclose && pc < close_cap) close_out[pc] = make_int2(t, s);
-        if (nnext) {
-            for (int o = 0; o < 8; o++) {
-                if (desc_s) {
-                    if (nonempty(S[fc + o])) { if (pn < next_cap) next[pn] = make_int2(fc + o, t); pn++; }
-                } else {
-                    if (nonempty(T[fc + o])) { if (pn < next_cap) next[pn] = make_int2(s, fc + o); pn++; }
-                }
-            }
-        }
     }
 }
 
@@ -272,7 +303,7 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
             FMM_HIP(hipMemcpyAsync(fa, L.h_seed, sizeof(int2), hipMemcpyHostToDevice, st));
             FMM_HIP(hipMemcpyAsync(cnt + 4, L.h_seed + 2, sizeof(unsigned), hipMemcpyHostToDevice, st));
             // steps queue without host synchronisation; the grid strides over any frontier size
-            const int grid = 2048;
+            const int grid = 1024;
             for (int k = 0; k < max_steps; k++) {
                 k_lists_step<<<grid, 256, 0, st>>>(fa, cnt + 4 + k, front_cap, srec, rs, trec, rt, mac_scale, far,
                                                    cnt + 0, far_cap, close, cnt + 1, close_cap, fb, cnt + 4 + k + 1,
