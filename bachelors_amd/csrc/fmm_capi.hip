@@ -24,6 +24,9 @@ using namespace fmm;
 namespace {
 
 enum Stage { ST_EMPTY = 0, ST_TREES = 1, ST_LEAVES = 2, ST_UP = 3, ST_DOWN = 4, ST_EVAL = 5 };
+// M2M levels with at least this many parent columns use parent-major tiles
+constexpr int M2M_PARENT_MAJOR_COLUMNS = 64 * 256;
+
 enum Ev { E_START, E_TREE, E_P2M, E_M2M, E_LISTS, E_TABLES, E_M2L, E_L2L, E_L2P, E_P2P, E_END, E_M2L_GEMM, E_COUNT };
 
 struct Pass {               // one batched GEMM launch: tiles over (src, dst) pairs
@@ -62,7 +65,7 @@ struct fmm_ctx {
     DevBuf M, Lx;
     // translations
     OpTables up_ops, down_ops;
-    std::vector<Pass> m2m_pass, l2l_pass;
+    std::vector<Pass> m2m_pass, l2l_pass, m2m_adj_pass;   // m2m_adj_pass: per orientation, for M2M^T
     Pass m2l_pass;
     Lists lists;
     ListScratch lscr;
@@ -255,26 +258,38 @@ void stage_upward(fmm_ctx* c) {
         FMM_HIP(hipMemcpyAsync(d, specs.data(), O.nops * sizeof(OpSpec), hipMemcpyHostToDevice, c->stream));
         build_tables(c, O, d, c->stream);
     }
+    // per level: many parents -> parent-major tiles, each parent column summing its 8 children
+    // (one K segment per orientation operator; empty children hold zero multipoles), written
+    // once without atomics; few parents -> the 8 orientations as parallel classes of
+    // (child, parent) pairs with atomic accumulation (a long serial K loop would dominate)
     c->m2m_pass.resize(MAX_LEVELS);
     for (int k = nlev - 1; k >= 1; k--) {
-        std::vector<int> src[8], dst[8];
+        std::vector<int> first, parent;
         for (int n = S.level_begin[k - 1]; n < S.level_begin[k]; n++) {
             int fc = S.h_first_child[n];
-            if (fc < 0) continue;
-            for (int o = 0; o < 8; o++)
-                if (nonempty_host(S, fc + o)) { src[o].push_back(fc + o); dst[o].push_back(n); }
+            if (fc >= 0) { first.push_back(fc); parent.push_back(n); }
         }
-        std::vector<int> all_s, all_d, cb(1, 0), cop;
-        for (int o = 0; o < 8; o++) {
-            all_s.insert(all_s.end(), src[o].begin(), src[o].end());
-            all_d.insert(all_d.end(), dst[o].begin(), dst[o].end());
-            cb.push_back((int)all_s.size());
-            cop.push_back((k - 1) * 8 + o);
-        }
+        Pass& P = c->m2m_pass[k];
         std::vector<GemmTile> tiles;
-        plan_tiles(cb, cop, tiles);
-        upload_pass(c->m2m_pass[k], all_s, all_d, tiles, c->stream);
-        run_pass(c, O, c->m2m_pass[k], c->M.ptr, c->M.ptr, EPI_ATOMIC);
+        if (3 * first.size() >= (size_t)M2M_PARENT_MAJOR_COLUMNS) {
+            std::vector<int> cb = {0, (int)first.size()}, cop = {(k - 1) * 8};
+            plan_tiles(cb, cop, tiles);
+            upload_pass(P, first, parent, tiles, c->stream);
+            if (P.ntiles)
+                class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, P.src.as<int>(),
+                           P.dst.as<int>(), c->M.ptr, c->M.ptr, c->Lp, EPI_ADD, c->stream, false, 8);
+        } else {
+            std::vector<int> all_s, all_d, cb(1, 0), cop;
+            for (int o = 0; o < 8; o++) {
+                for (size_t i = 0; i < first.size(); i++)
+                    if (nonempty_host(S, first[i] + o)) { all_s.push_back(first[i] + o); all_d.push_back(parent[i]); }
+                cb.push_back((int)all_s.size());
+                cop.push_back((k - 1) * 8 + o);
+            }
+            plan_tiles(cb, cop, tiles);
+            upload_pass(P, all_s, all_d, tiles, c->stream);
+            run_pass(c, O, P, c->M.ptr, c->M.ptr, EPI_ATOMIC);
+        }
     }
     c->rec(E_M2M);
     c->stage = ST_UP;
@@ -691,8 +706,30 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
         a.up_ops_lo = c->up_ops.lo.ptr;
         a.down_ops = ops_ptr(c, c->down_ops);
         a.down_ops_lo = c->down_ops.lo.ptr;
+        // M2M^T needs (child, parent) pairs per orientation (the forward pass is parent-major)
+        const TreeDev& S = c->S;
+        c->m2m_adj_pass.resize(MAX_LEVELS);
+        for (int k = S.num_levels - 1; k >= 1; k--) {
+            std::vector<int> src[8], dst[8];
+            for (int n = S.level_begin[k - 1]; n < S.level_begin[k]; n++) {
+                int fc = S.h_first_child[n];
+                if (fc < 0) continue;
+                for (int o = 0; o < 8; o++)
+                    if (nonempty_host(S, fc + o)) { src[o].push_back(fc + o); dst[o].push_back(n); }
+            }
+            std::vector<int> all_s, all_d, cb(1, 0), cop;
+            for (int o = 0; o < 8; o++) {
+                all_s.insert(all_s.end(), src[o].begin(), src[o].end());
+                all_d.insert(all_d.end(), dst[o].begin(), dst[o].end());
+                cb.push_back((int)all_s.size());
+                cop.push_back((k - 1) * 8 + o);
+            }
+            std::vector<GemmTile> tiles;
+            plan_tiles(cb, cop, tiles);
+            upload_pass(c->m2m_adj_pass[k], all_s, all_d, tiles, c->stream);
+        }
         for (int k = 0; k < MAX_LEVELS; k++) {
-            const Pass* pm = k < (int)c->m2m_pass.size() ? &c->m2m_pass[k] : nullptr;
+            const Pass* pm = k < (int)c->m2m_adj_pass.size() ? &c->m2m_adj_pass[k] : nullptr;
             const Pass* pl = k < (int)c->l2l_pass.size() ? &c->l2l_pass[k] : nullptr;
             a.m2m_levels.push_back(pm ? PassView{pm->ntiles, pm->tiles.as<GemmTile>(), pm->src.as<int>(), pm->dst.as<int>()}
                                       : PassView{});

@@ -52,11 +52,13 @@ template <> struct Vec4<double> {
 };
 
 // ---------------------------------------------------------------- FP64 / FP32
-template <class T, int RB, bool TRANS>
+template <class T, int RB, bool TRANS, bool MULTI>
 __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const GemmTile* __restrict__ tiles,
                                               const int* __restrict__ pair_src, const int* __restrict__ pair_dst,
-                                              const T* __restrict__ src_exp, T* __restrict__ dst_exp, int epi) {
+                                              const T* __restrict__ src_exp, T* __restrict__ dst_exp, int epi,
+                                              int nseg) {
     constexpr int Lp = 64 * RB, BK = 16, SA = Lp + 4, SB = BN + 4, NK = Lp / BK;
+    const int NKT = MULTI ? nseg * NK : NK;   // single-segment launches keep a compile-time trip count
     typedef Vec4<T> V;
     typedef typename V::t v4;
     typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
@@ -64,8 +66,7 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
     __shared__ T Bs[2][BK][SB];
     const GemmTile tile = tiles[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const T* A = ops + (size_t)tile.op * Lp * Lp;
-    // B loader: one column and 4 consecutive k per thread
+    // B loader: one column and 4 consecutive k per thread; segment g reads node pair_src + g
     const int bc = tid >> 2, bq = (tid & 3) * 4;
     const T* bptr = nullptr;
     if (bc < tile.ncol) {
@@ -74,9 +75,12 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
         bptr = src_exp + ((size_t)pair_src[pair] * 3 + comp) * Lp;
     }
     // A loader: RB quads of 4 per thread.  Non-transposed: (row, 4 consecutive k);
-    // transposed: (4 consecutive rows, one k) of T^T = columns of T.
+    // transposed: (4 consecutive rows, one k) of T^T = columns of T.  Segment g applies
+    // operator tile.op + g.
     v4 ra[RB], rb;
-    auto gload = [&](int k0) {
+    auto gload = [&](int kt) {
+        const int seg = kt / NK, k0 = (kt - seg * NK) * BK;
+        const T* A = ops + (size_t)(tile.op + seg) * Lp * Lp;
 #pragma unroll
         for (int j = 0; j < RB; j++) {
             int qi = tid + 256 * j;
@@ -88,7 +92,7 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
                 ra[j] = V::load(A + (size_t)(k0 + k) * Lp + rq);
             }
         }
-        if (bptr) rb = V::load(bptr + k0 + bq);
+        if (bptr) rb = V::load(bptr + (size_t)seg * 3 * Lp + k0 + bq);
     };
     auto lstore = [&](int buf) {
 #pragma unroll
@@ -113,9 +117,9 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
     lstore(0);
     __syncthreads();
     const int wrow = wave * 16 * RB;
-    for (int kt = 0; kt < NK; kt++) {
+    for (int kt = 0; kt < NKT; kt++) {
         const int cur = kt & 1;
-        if (kt + 1 < NK) gload((kt + 1) * BK);
+        if (kt + 1 < NKT) gload(kt + 1);
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
             T af[RB], bf[4];
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
                         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
                 }
         }
-        if (kt + 1 < NK) lstore(cur ^ 1);
+        if (kt + 1 < NKT) lstore(cur ^ 1);
         __syncthreads();
     }
 #pragma unroll
@@ -169,13 +173,14 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
 }
 
 // ---------------------------------------------------------------- BF16x3
-template <int RB, bool TRANS>
+template <int RB, bool TRANS, bool MULTI>
 __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ ops_hi, const __bf16* __restrict__ ops_lo,
                                                      const GemmTile* __restrict__ tiles,
                                                      const int* __restrict__ pair_src, const int* __restrict__ pair_dst,
                                                      const float* __restrict__ src_exp, float* __restrict__ dst_exp,
-                                                     int epi) {
+                                                     int epi, int nseg) {
     constexpr int Lp = 64 * RB, BK = 32, S = BK + 8, NK = Lp / BK;
+    const int NKT = MULTI ? nseg * NK : NK;
     // k-contiguous rows: each lane reads its 8 consecutive k as one 16-byte vector
     __shared__ __attribute__((aligned(16))) __bf16 Ah[2][Lp][S];
     __shared__ __attribute__((aligned(16))) __bf16 Al[2][Lp][S];
@@ -183,9 +188,6 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
     __shared__ __attribute__((aligned(16))) __bf16 Bl[2][BN][S];
     const GemmTile tile = tiles[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const size_t opoff = (size_t)tile.op * Lp * Lp;
-    const __bf16* Ahg = ops_hi + opoff;
-    const __bf16* Alg = ops_lo + opoff;
     const int bc = tid >> 2, bq = (tid & 3) * 8;
     const float* bptr = nullptr;
     if (bc < tile.ncol) {
@@ -196,7 +198,11 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
     // A loader: RB units of 8 bf16 per thread and part
     bf16x8 rah[RB], ral[RB];
     float4 rb0 = {0, 0, 0, 0}, rb1 = {0, 0, 0, 0};
-    auto gload = [&](int k0) {
+    auto gload = [&](int kt) {
+        const int seg = kt / NK, k0 = (kt - seg * NK) * BK;
+        const size_t opoff = (size_t)(tile.op + seg) * Lp * Lp;
+        const __bf16* Ahg = ops_hi + opoff;
+        const __bf16* Alg = ops_lo + opoff;
 #pragma unroll
         for (int j = 0; j < RB; j++) {
             int qi = tid + 256 * j;
@@ -211,8 +217,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
             }
         }
         if (bptr) {
-            rb0 = *(const float4*)(bptr + k0 + bq);
-            rb1 = *(const float4*)(bptr + k0 + bq + 4);
+            const float* bp = bptr + (size_t)seg * 3 * Lp + k0 + bq;
+            rb0 = *(const float4*)bp;
+            rb1 = *(const float4*)(bp + 4);
         }
     };
     auto lstore = [&](int buf) {
@@ -252,9 +259,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
     __syncthreads();
     const int wrow = wave * 16 * RB;
     const int kk = 8 * (lane >> 4);
-    for (int kt = 0; kt < NK; kt++) {
+    for (int kt = 0; kt < NKT; kt++) {
         const int cur = kt & 1;
-        if (kt + 1 < NK) gload((kt + 1) * BK);
+        if (kt + 1 < NKT) gload(kt + 1);
         bf16x8 bh[4], bl[4];
 #pragma unroll
         for (int b = 0; b < 4; b++) {
@@ -273,7 +280,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[b], acc[a][b], 0, 0, 0);
             }
         }
-        if (kt + 1 < NK) lstore(cur ^ 1);
+        if (kt + 1 < NKT) lstore(cur ^ 1);
         __syncthreads();
     }
 #pragma unroll
@@ -298,28 +305,35 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
     }
 }
 
-template <int RB, bool TR>
+template <int RB, bool TR, bool MU>
 static void launch_rb(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
                       const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int epi,
-                      hipStream_t st) {
+                      int nseg, hipStream_t st) {
     if (prec == Precision::F64)
-        k_gemm<double, RB, TR><<<ntiles, 256, 0, st>>>((const double*)ops, d_tiles, pair_src, pair_dst,
-                                                      (const double*)src_exp, (double*)dst_exp, epi);
+        k_gemm<double, RB, TR, MU><<<ntiles, 256, 0, st>>>((const double*)ops, d_tiles, pair_src, pair_dst,
+                                                          (const double*)src_exp, (double*)dst_exp, epi, nseg);
     else if (prec == Precision::F32)
-        k_gemm<float, RB, TR><<<ntiles, 256, 0, st>>>((const float*)ops, d_tiles, pair_src, pair_dst,
-                                                     (const float*)src_exp, (float*)dst_exp, epi);
+        k_gemm<float, RB, TR, MU><<<ntiles, 256, 0, st>>>((const float*)ops, d_tiles, pair_src, pair_dst,
+                                                         (const float*)src_exp, (float*)dst_exp, epi, nseg);
     else
-        k_gemm_bf16x3<RB, TR><<<ntiles, 256, 0, st>>>((const __bf16*)ops, (const __bf16*)ops_lo, d_tiles, pair_src,
-                                                     pair_dst, (const float*)src_exp, (float*)dst_exp, epi);
+        k_gemm_bf16x3<RB, TR, MU><<<ntiles, 256, 0, st>>>((const __bf16*)ops, (const __bf16*)ops_lo, d_tiles, pair_src,
+                                                         pair_dst, (const float*)src_exp, (float*)dst_exp, epi, nseg);
     FMM_HIP(hipGetLastError());
 }
 
 template <bool TR>
 static void launch(int RB, Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
-                   const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int epi,
+                   const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int epi, int nseg,
                    hipStream_t st) {
-#define FMM_RB(R) \
-    case R: launch_rb<R, TR>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epi, st); break;
+#define FMM_RB(R)                                                                                             \
+    case R:                                                                                                   \
+        if (nseg > 1)                                                                                         \
+            launch_rb<R, TR, true>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epi, \
+                                   nseg, st);                                                                 \
+        else                                                                                                  \
+            launch_rb<R, TR, false>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epi, \
+                                    1, st);                                                                   \
+        break;
     switch (RB) {
         FMM_RB(1) FMM_RB(2) FMM_RB(3) FMM_RB(4) FMM_RB(5) FMM_RB(6) FMM_RB(7)
         default: throw Error(4, "padded expansion length above 448 (order > 20)");
@@ -329,15 +343,16 @@ static void launch(int RB, Precision prec, const void* ops, const void* ops_lo, 
 
 void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
                 const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, int epilogue,
-                hipStream_t st, bool transpose) {
+                hipStream_t st, bool transpose, int nseg) {
     if (ntiles == 0) return;
     if (Lp % 64 != 0) throw Error(1, "padded expansion length must be a multiple of 64");
     int RB = Lp / 64;
     if (epilogue < EPI_ADD || epilogue > EPI_STORE) throw Error(1, "unknown GEMM epilogue");
+    if (nseg < 1) throw Error(1, "a GEMM tile needs at least one segment");
     if (transpose)
-        launch<true>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epilogue, st);
+        launch<true>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epilogue, nseg, st);
     else
-        launch<false>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epilogue, st);
+        launch<false>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epilogue, nseg, st);
 }
 
 // host-side tiling of class column ranges: class c owns pairs [begin[c], begin[c+1])

@@ -21,9 +21,11 @@ enum Epilogue {
 // ops: operator tables (double / float / bf16 hi), ops_lo: bf16 lo parts (BF16X3 only).
 // transpose: apply T^T (adjoint passes).
 // src_exp / dst_exp: [node][3][Lp] in double (F64) or float (F32, BF16X3).
+// nseg > 1: every column sums nseg products, segment g applying operator tile.op + g to
+// node pair_src + g (the 8 children of a parent, first child in pair_src).
 void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
                 const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, int epilogue,
-                hipStream_t st, bool transpose = false);
+                hipStream_t st, bool transpose = false, int nseg = 1);
 
 // Pairs grouped by the node that receives them: order[begin[v] .. end[v]) are the pair indices
 // (ascending) whose key is v; nodes without pairs have begin = end = 0.
