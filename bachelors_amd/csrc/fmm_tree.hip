@@ -570,6 +570,188 @@ __global__ void k_zero_i32(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// A level with its boundaries kept on the device (no host sync), in two grid-wide kernels over
+// LEVEL_WGS contiguous chunks of the nodes [level_begin[k], level_begin[k+1]):
+// k_level_count counts the subdividing nodes of each chunk, k_level_children numbers them
+// (chunk prefix + block scan, i.e. parent order) and makes their 8 children (float centres as
+// the reference, empty sorted ranges).  Children beyond node_cap set flags[1] and none are made.
+constexpr int LEVEL_WGS = 256;
+
+__device__ __forceinline__ int block_excl_sum(int v, int& total) {   // 256 threads
+    __shared__ int wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        int u = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += u;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int off = 0;
+    for (int q = 0; q < w; q++) off += wsum[q];
+    total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return off + incl - v;
+}
+
+__device__ __forceinline__ void level_chunk(const int* level_begin, int k, int& a, int& b) {
+    const int lb = level_begin[k], le = level_begin[k + 1], nn = le - lb;
+    const int per_wg = (nn + LEVEL_WGS - 1) / LEVEL_WGS;
+    const int c0 = min(nn, blockIdx.x * per_wg), c1 = min(nn, c0 + per_wg);
+    const int per_t = (c1 - c0 + 255) / 256;
+    a = lb + min(c1, c0 + (int)threadIdx.x * per_t);
+    b = lb + min(c1, c0 + ((int)threadIdx.x + 1) * per_t);
+}
+
+__global__ __launch_bounds__(256) void k_level_count(int k, int depth, int max_depth, int thr,
+                                                     const int* __restrict__ level_begin, int* __restrict__ flags,
+                                                     const int* __restrict__ sb, const int* __restrict__ se,
+                                                     int* __restrict__ partial) {
+    const bool allow = k < max_depth && k < depth, deeper = k < max_depth && k >= depth;
+    int a, b;
+    level_chunk(level_begin, k, a, b);
+    int cnt = 0;
+    for (int i = a; i < b; i++) {
+        bool sub = se[i] - sb[i] >= thr;
+        if (sub && deeper) flags[0] = 1;
+        cnt += (allow && sub) ? 1 : 0;
+    }
+    int total;
+    block_excl_sum(cnt, total);
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_level_children(int k, int depth, int max_depth, int thr, double3 ph,
+                                                        int node_cap, int* __restrict__ level_begin,
+                                                        int* __restrict__ flags, const int* __restrict__ partial,
+                                                        double* __restrict__ center, int* __restrict__ first_child,
+                                                        int* __restrict__ parent, int* __restrict__ level,
+                                                        int* __restrict__ sb, int* __restrict__ se) {
+    const bool allow = k < max_depth && k < depth;
+    const int le = level_begin[k + 1];
+    // subdividing nodes before this chunk, and in the whole level
+    int before = 0, all = 0;
+    for (int q = threadIdx.x; q < LEVEL_WGS; q += 256) {
+        int v = partial[q];
+        all += v;
+        if (q < (int)blockIdx.x) before += v;
+    }
+    int tb, ta;
+    block_excl_sum(before, tb);
+    block_excl_sum(all, ta);
+    const bool fits = (long long)le + 8LL * ta <= (long long)node_cap;
+    int a, b;
+    level_chunk(level_begin, k, a, b);
+    int cnt = 0;
+    for (int i = a; i < b; i++) cnt += (allow && se[i] - sb[i] >= thr) ? 1 : 0;
+    int chunk_total;
+    int off = tb + block_excl_sum(cnt, chunk_total);
+    float w2 = (float)ph.x / 2.0f, h2 = (float)ph.y / 2.0f, d2 = (float)ph.z / 2.0f;
+    for (int i = a; i < b; i++) {
+        bool sub = allow && fits && se[i] - sb[i] >= thr;
+        if (!sub) { first_child[i] = -1; continue; }
+        int fc = le + 8 * off++;
+        first_child[i] = fc;
+        float x = (float)center[3 * i], y = (float)center[3 * i + 1], z = (float)center[3 * i + 2];
+        for (int o = 0; o < 8; o++) {
+            int c = fc + o;
+            center[3 * c] = c_child_sign[o][0] < 0 ? __fsub_rn(x, w2) : __fadd_rn(x, w2);
+            center[3 * c + 1] = c_child_sign[o][1] < 0 ? __fsub_rn(y, h2) : __fadd_rn(y, h2);
+            center[3 * c + 2] = c_child_sign[o][2] < 0 ? __fsub_rn(z, d2) : __fadd_rn(z, d2);
+            parent[c] = i;
+            level[c] = k + 1;
+            first_child[c] = -1;
+            sb[c] = -1;
+            se[c] = -1;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (!fits) flags[1] = 1;
+        level_begin[k + 2] = le + (fits ? 8 * ta : 0);
+    }
+}
+
+// empty children of the subdivided nodes of level k (grid-strided, bounds from the device)
+__global__ void k_fix_empty_dev(const int* __restrict__ first_child, const int* __restrict__ level_begin, int k,
+                                int* __restrict__ sb, int* __restrict__ se) {
+    const int lb = level_begin[k], le = level_begin[k + 1];
+    for (int node = lb + blockIdx.x * blockDim.x + threadIdx.x; node < le; node += gridDim.x * blockDim.x) {
+        int fc = first_child[node];
+        if (fc < 0) continue;
+        int pos = sb[node];
+        for (int o = 0; o < 8; o++) {
+            int c = fc + o;
+            if (sb[c] < 0) sb[c] = se[c] = pos;
+            else pos = se[c];
+        }
+    }
+}
+
+__global__ void k_init_root_dev(double3 c, int n0, double* __restrict__ center, int* __restrict__ parent,
+                                int* __restrict__ level, int* __restrict__ first_child, int* __restrict__ sb,
+                                int* __restrict__ se, int* __restrict__ level_begin, int nlb, int* __restrict__ flags) {
+    center[0] = c.x;
+    center[1] = c.y;
+    center[2] = c.z;
+    parent[0] = -1;
+    level[0] = 0;
+    first_child[0] = -1;
+    sb[0] = 0;
+    se[0] = n0;
+    level_begin[0] = 0;
+    for (int i = 1; i < nlb; i++) level_begin[i] = 1;
+    flags[0] = flags[1] = 0;
+}
+
+// Same tree as build_from_sorted, all levels queued at once: the node arrays are sized for
+// the bound (each subdividing node holds >= thr points, so a level adds <= 8 n0 / thr
+// nodes) and read back once.  Returns 0 on success, 1 if the key is too short (a node below
+// `depth` subdivides), 2 if the node bound was exceeded.
+template <class K>
+static int build_from_sorted_queued(TreeDev& T, TreeScratch& S, const K* sk, int n0, int depth, int thr,
+                                    int max_depth, const double root_c[3], size_t node_cap, hipStream_t st) {
+    grow_keep(T.center, node_cap * 3 * sizeof(double), 0, st);
+    grow_keep(T.first_child, node_cap * sizeof(int), 0, st);
+    grow_keep(T.parent, node_cap * sizeof(int), 0, st);
+    grow_keep(T.level, node_cap * sizeof(int), 0, st);
+    grow_keep(S.sb, node_cap * sizeof(int), 0, st);
+    grow_keep(S.se, node_cap * sizeof(int), 0, st);
+    const int nlb = MAX_LEVELS + 2;
+    int* dlb = S.dlevel.get<int>(nlb + 2);
+    int* flags = dlb + nlb;
+    int* node_of = S.node_of.get<int>(std::max(n0, 1));
+    int* partial = S.partial.get<int>(LEVEL_WGS);
+    k_init_root_dev<<<1, 1, 0, st>>>(make_double3(root_c[0], root_c[1], root_c[2]), n0, T.center.as<double>(),
+                                     T.parent.as<int>(), T.level.as<int>(), T.first_child.as<int>(), S.sb.as<int>(),
+                                     S.se.as<int>(), dlb, nlb, flags);
+    if (n0) k_zero_i32<<<blocks(n0), 256, 0, st>>>(node_of, n0);
+    const int last = std::min(max_depth, depth);   // levels 0..last can hold nodes
+    for (int k = 0; k <= last; k++) {
+        double3 ph = make_double3(T.half[k][0], T.half[k][1], T.half[k][2]);
+        k_level_count<<<LEVEL_WGS, 256, 0, st>>>(k, depth, max_depth, thr, dlb, flags, S.sb.as<int>(),
+                                                 S.se.as<int>(), partial);
+        k_level_children<<<LEVEL_WGS, 256, 0, st>>>(k, depth, max_depth, thr, ph, (int)node_cap, dlb, flags, partial,
+                                                    T.center.as<double>(), T.first_child.as<int>(),
+                                                    T.parent.as<int>(), T.level.as<int>(), S.sb.as<int>(),
+                                                    S.se.as<int>());
+        if (k == last) break;
+        if (n0) k_assign_level<K><<<blocks(n0), 256, 0, st>>>(sk, n0, k + 1, depth, T.first_child.as<int>(), node_of,
+                                                               S.sb.as<int>(), S.se.as<int>());
+        k_fix_empty_dev<<<256, 256, 0, st>>>(T.first_child.as<int>(), dlb, k, S.sb.as<int>(), S.se.as<int>());
+    }
+    std::vector<int> h(nlb + 2);
+    FMM_HIP(hipMemcpyAsync(h.data(), dlb, (nlb + 2) * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    if (h[nlb]) return 1;
+    if (h[nlb + 1]) return 2;
+    int k = 0;
+    while (k + 2 < nlb && h[k + 2] > h[k + 1]) k++;   // level k+1 holds nodes while its range is non-empty
+    T.num_levels = k + 1;
+    T.num_nodes = h[k + 1];
+    for (int i = 0; i < nlb; i++) T.level_begin[i] = std::min(h[i], T.num_nodes);
+    return 0;
+}
+
 // returns false when the key is too short for the tree (a node below `depth` subdivides)
 template <class K>
 static bool build_from_sorted(TreeDev& T, TreeScratch& S, const K* sk, int n0, int depth, int thr, int max_depth,
@@ -666,7 +848,15 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
             k_compact_paths<uint32_t><<<blocks(n), 256, 0, st>>>(key, inroot, rank, n, 3 * (depth - short_depth), k32,
                                                                  idx);
             sort_pairs_u32(k32, k32s, idx, order, n0, 3 * short_depth, S.tmp, st);
-            ok = build_from_sorted<uint32_t>(T, S, k32s, n0, short_depth, thr, max_depth, root_c, st);
+            // node bound of the queued build: 8 n0 / thr children per level
+            const size_t per_level = 8 * ((size_t)n0 / thr + 1);
+            const size_t cap_short = 1 + per_level * (size_t)short_depth;
+            const bool queued = cap_short * 44 <= ((size_t)1 << 31);
+            int r = queued ? build_from_sorted_queued<uint32_t>(T, S, k32s, n0, short_depth, thr, max_depth, root_c,
+                                                                cap_short, st)
+                           : (build_from_sorted<uint32_t>(T, S, k32s, n0, short_depth, thr, max_depth, root_c, st) ? 0 : 1);
+            if (r == 2) r = build_from_sorted<uint32_t>(T, S, k32s, n0, short_depth, thr, max_depth, root_c, st) ? 0 : 1;
+            ok = r == 0;
             if (!ok) {
                 uint64_t* kc = S.k64c.get<uint64_t>(n0);
                 uint64_t* kcs = S.k64s.get<uint64_t>(n0);

@@ -5,7 +5,7 @@ namespace fmm {
 
 struct TreeScratch {
     DevBuf flag, pos, mask, key, key2, val, sflag, srank, total, tmp;
-    DevBuf k64, k64c, k64s, irr, order, node_of, sb, se, need;   // one-sort build
+    DevBuf k64, k64c, k64s, irr, order, node_of, sb, se, need, dlevel, partial;   // one-sort build
     uint32_t h_last[4] = {0, 0, 0, 0};
     std::vector<DevBuf> lvl_idx, lvl_node;   // per level: point index and node of every arrival
     std::vector<int> lvl_n;
