@@ -53,3 +53,35 @@ def gap_overlap_targets(orc, tgt, cap, rng, nadd=60):
     cand = np.array(cand)
     pick = rng.choice(len(cand), size=min(nadd, len(cand)), replace=False)
     return np.concatenate([tgt, cand[pick]]), len(pick)
+
+
+def root_of(tree):
+    """(centre, half-sizes) of an exported tree's root, from the offsets of its children."""
+    c = tree["center"][0]
+    kids = np.flatnonzero(tree["parent"] == 0)
+    return c, 2.0 * np.abs(tree["center"][kids[0]] - c)
+
+
+def coaxial_shells(rng, n_per, radii=(0.5, 0.8, 1.1, 1.4), thickness=0.1, height=1.0):
+    """Coaxial cylindrical shells (uniform in volume), one random current sign per shell, equal
+    weights: J = s_k phi-hat expressed as w = s_k / N."""
+    P, W = [], []
+    signs = rng.choice([-1.0, 1.0], size=len(radii))
+    n = n_per * len(radii)
+    for R, s in zip(radii, signs):
+        u = rng.random((n_per, 3))
+        r0, r1 = R - thickness / 2, R + thickness / 2
+        r = np.sqrt(r0 * r0 + (r1 * r1 - r0 * r0) * u[:, 0])
+        ph = 2 * np.pi * u[:, 1]
+        z = (u[:, 2] - 0.5) * height
+        P.append(np.stack([r * np.cos(ph), r * np.sin(ph), z], 1))
+        W.append(np.full(n_per, s / n))
+    return np.concatenate(P), np.concatenate(W)
+
+
+def gaussian_clusters(rng, n, k=64):
+    """n points from k anisotropic Gaussian clusters: centres U[-1,1]^3, sigma per axis U[0.01, 0.1]."""
+    c = rng.uniform(-1, 1, (k, 3))
+    sg = rng.uniform(0.01, 0.1, (k, 3))
+    lab = rng.integers(k, size=n)
+    return c[lab] + rng.normal(size=(n, 3)) * sg[lab]

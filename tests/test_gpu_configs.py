@@ -1,0 +1,71 @@
+"""GPU: the FMM on inputs shaped like the BASELINE configurations, at sizes the oracle finishes in
+seconds -- config 1 in full (the CPU reference case), config 3 (4 coaxial shells with random-sign
+currents, p=16, split precision) and config 5 (anisotropic Gaussian clusters with random current
+directions, p=14, leaf capacity 48, depth <= 10) reduced.  Parity is against the oracle's FMM at the
+same order and tree (same frame, target root taken from the GPU tree); truncation against the direct
+sum with the bound measured for the oracle itself."""
+import numpy as np
+import pytest
+
+from conftest import coaxial_shells, gaussian_clusters, rel_l2, root_of, shell_sources
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(fm, src, w, tgt, J=None, **cfg):
+    F = fm.FMM(**cfg)
+    F.set_sources(src, w, J)
+    F.set_targets(tgt)
+    return F, F.run(1.0)
+
+
+@pytest.mark.parametrize("frame", [0, 1])
+def test_config1_reference_case_fp64(orc, frame):
+    import bachelors_amd as fm
+    rng = np.random.default_rng(20261202 + 1)
+    src = shell_sources(rng, 32768)
+    w = np.full(32768, 1.0 / 32768)
+    tgt = rng.uniform(-2, 2, (4096, 3))
+    F, B = _gpu(fm, src, w, tgt, order=10, source_leaf_capacity=32, target_leaf_capacity=32, precision="fp64",
+                frame=frame)
+    root = root_of(F.tree(1)) if frame == 1 else None
+    R = orc.FMMRun(src, w, tgt, p=10, cap_s=32, cap_t=32, flags=orc.CORRECTED, target_root=root)
+    assert rel_l2(B, R.B_input_order(len(tgt))) < 1e-10
+    # p = 10 truncation of this geometry: 1.55e-6 for the oracle itself
+    assert rel_l2(B, orc.direct_B(src, w, tgt)) < 3e-6
+
+
+def test_config3_coaxial_shells_p16(orc):
+    import bachelors_amd as fm
+    rng = np.random.default_rng(20261202 + 3)
+    src, w = coaxial_shells(rng, 1024)
+    tgt = rng.uniform(-2, 2, (2048, 3))
+    cfg = dict(order=16, source_leaf_capacity=64, target_leaf_capacity=64, frame=1)
+    F, B64 = _gpu(fm, src, w, tgt, precision="fp64", **cfg)
+    R = orc.FMMRun(src, w, tgt, p=16, cap_s=64, cap_t=64, flags=orc.CORRECTED, target_root=root_of(F.tree(1)))
+    assert rel_l2(B64, R.B_input_order(len(tgt))) < 1e-10
+    assert rel_l2(B64, orc.direct_B(src, w, tgt)) < 1e-8        # oracle's own truncation: 2.98e-9
+    _, B32 = _gpu(fm, src, w, tgt, precision="fp32", **cfg)
+    _, Bbf = _gpu(fm, src, w, tgt, precision="bf16x3", **cfg)
+    assert rel_l2(B32, B64) < 3e-6
+    assert rel_l2(Bbf, B64) < 2e-5
+
+
+def test_config5_gaussian_clusters_p14(orc):
+    import bachelors_amd as fm
+    rng = np.random.default_rng(20261202 + 5)
+    src = gaussian_clusters(rng, 2048)
+    J = rng.normal(size=(2048, 3))
+    J /= np.linalg.norm(J, axis=1)[:, None]
+    w = np.full(2048, 1.0 / 2048)
+    tgt = gaussian_clusters(np.random.default_rng(20261202 + 50), 2048)
+    cfg = dict(order=14, source_leaf_capacity=48, target_leaf_capacity=48, frame=1, max_depth=10)
+    F, B64 = _gpu(fm, src, w, tgt, J=J, precision="fp64", **cfg)
+    R = orc.FMMRun(src, w, tgt, p=14, cap_s=48, cap_t=48, flags=orc.CORRECTED, J=J, target_root=root_of(F.tree(1)),
+                   max_depth=10)
+    assert rel_l2(B64, R.B_input_order(len(tgt))) < 1e-10
+    assert rel_l2(B64, orc.direct_B(src, w, tgt, J=J)) < 3e-8   # oracle's own truncation: 6.6e-9
+    _, B32 = _gpu(fm, src, w, tgt, J=J, precision="fp32", **cfg)
+    _, Bbf = _gpu(fm, src, w, tgt, J=J, precision="bf16x3", **cfg)
+    assert rel_l2(B32, B64) < 3e-6
+    assert rel_l2(Bbf, B64) < 2e-5

@@ -6,17 +6,10 @@ box size and M2L classes must be keyed by physical box size."""
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import rel_l2, root_of
 from torus_geometry import torus_quadratures, telma
 
 pytestmark = pytest.mark.gpu
-
-
-def _root_of(tree):
-    """(centre, half-sizes) of an exported tree's root, from the offsets of its children."""
-    c = tree["center"][0]
-    kids = np.flatnonzero(tree["parent"] == 0)
-    return c, 2.0 * np.abs(tree["center"][kids[0]] - c)
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 3e-6)])
@@ -32,7 +25,7 @@ def test_fmm_torus_against_oracle_direct_and_telma(orc, precision, tol):
     s.calcLocalMultipoleExpansions(fm.M2LAlg.ComplexTranslation, fm.Device.CPU)
     B = s.calcB_array(5.0)
     T = s._fmm.tree(1)
-    root = _root_of(T)
+    root = root_of(T)
     assert np.all(root[1] > 0)
     # parity: the oracle's FMM on the same frame, order and leaf capacities
     R = orc.FMMRun(P, W, pts, p=10, cap_s=1000, cap_t=32, flags=orc.CORRECTED, target_root=root, current=5.0)
