@@ -7,7 +7,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fmm_capi.h"
@@ -27,7 +29,10 @@ enum Stage { ST_EMPTY = 0, ST_TREES = 1, ST_LEAVES = 2, ST_UP = 3, ST_DOWN = 4, 
 // M2M levels with at least this many parent columns use parent-major tiles
 constexpr int M2M_PARENT_MAJOR_COLUMNS = 64 * 256;
 
-enum Ev { E_START, E_TREE, E_P2M, E_M2M, E_LISTS, E_TABLES, E_M2L, E_L2L, E_L2P, E_P2P, E_END, E_M2L_GEMM, E_COUNT };
+// stage events: the stream-A chain, plus start marks of the stages that may run on the side
+// stream (lists, P2P) and of the translations after the lists (E_DOWN0)
+enum Ev { E_START, E_TREE, E_P2M, E_M2M, E_LISTS, E_TABLES, E_M2L, E_L2L, E_L2P, E_P2P, E_END, E_M2L_GEMM,
+          E_LISTS0, E_DOWN0, E_P2P0, E_COUNT };
 
 struct Pass {               // one batched GEMM launch: tiles over (src, dst) pairs
     int ntiles = 0, npairs = 0;
@@ -55,7 +60,7 @@ struct fmm_ctx {
     DevBuf src_xyz, src_w, src_J, tgt_xyz;
     // trees
     TreeDev S, T;
-    TreeScratch tscr;
+    TreeScratch tscr, tscr_t;                    // source / target tree scratch (built concurrently)
     ScaleTable sigS, sigT;
     DevBuf sxyz, sq, sJ, txyz;                   // slot order
     DevBuf nP0, nP1, nP2;                        // near-field sources packed per leaf (slot order)
@@ -79,12 +84,16 @@ struct fmm_ctx {
     int stage = ST_EMPTY;
     hipEvent_t ev[E_COUNT];
     bool ev_ok = false;
+    // side stream of the fused evaluation (lists and P2P beside the multipole chain) and its joins
+    hipStream_t side = nullptr;
+    hipEvent_t j_lists = nullptr, j_pack = nullptr, j_p2p = nullptr, j_gemm = nullptr;
     long long near_inter = -1;
     bool lists_ready = false;
 
     bool f64() const { return prec == Precision::F64; }
     size_t esize() const { return f64() ? sizeof(double) : sizeof(float); }
     void rec(int e) { if (ev_ok) FMM_HIP(hipEventRecord(ev[e], stream)); }
+    void rec_on(int e, hipStream_t s) { if (ev_ok) FMM_HIP(hipEventRecord(ev[e], s)); }
 };
 
 namespace {
@@ -158,8 +167,6 @@ void stage_trees(fmm_ctx* c) {
     hipStream_t st = c->stream;
     c->rec(E_START);
     const fmm_config& g = c->cfg;
-    build_tree(c->S, c->tscr, c->src_xyz.as<double>(), c->ns, g.source_root_center, g.source_root_half,
-               g.source_leaf_capacity, g.max_depth, true, st);
     double tc[3], th[3];
     if (g.frame == FMM_FRAME_SHARED) {
         // the source root, grown by doublings along the source lattice until it holds every
@@ -191,8 +198,26 @@ void stage_trees(fmm_ctx* c) {
             th[a] = hd * 1.1;
         }
     }
-    build_tree(c->T, c->tscr, c->tgt_xyz.as<double>(), c->nt, tc, th, g.target_leaf_capacity + 1, g.max_depth,
-               g.frame == FMM_FRAME_SHARED, st);
+    // the two trees are independent: the target tree is built on the side stream by a host
+    // thread (its own scratch) while this thread builds the source tree; both end synchronised
+    std::exception_ptr tgt_error;
+    std::thread tgt_thread([&] {
+        try {
+            build_tree(c->T, c->tscr_t, c->tgt_xyz.as<double>(), c->nt, tc, th, g.target_leaf_capacity + 1,
+                       g.max_depth, g.frame == FMM_FRAME_SHARED, c->side);
+        } catch (...) {
+            tgt_error = std::current_exception();
+        }
+    });
+    try {
+        build_tree(c->S, c->tscr, c->src_xyz.as<double>(), c->ns, g.source_root_center, g.source_root_half,
+                   g.source_leaf_capacity, g.max_depth, true, st);
+    } catch (...) {
+        tgt_thread.join();
+        throw;
+    }
+    tgt_thread.join();
+    if (tgt_error) std::rethrow_exception(tgt_error);
     for (int k = 0; k < MAX_LEVELS; k++) { c->sigS.s[k] = c->S.scale[k]; c->sigT.s[k] = c->T.scale[k]; }
     // sources and targets in slot (leaf DFS) order
     int nss = c->S.num_slots, nts = c->T.num_slots;
@@ -358,17 +383,27 @@ void downward_per_pair(fmm_ctx* c) {
     c->stage = ST_DOWN;
 }
 
-void stage_downward(fmm_ctx* c) {
+// interaction lists (and, in the shared frame, far pairs grouped by target node for the
+// atomic-free M2L accumulation) on stream ls; they depend on the trees only
+void stage_lists(fmm_ctx* c, hipStream_t ls) {
+    require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built");
+    const bool shared = c->cfg.frame == FMM_FRAME_SHARED;
+    c->rec_on(E_LISTS0, ls);
+    build_lists(c->lists, c->lscr, c->S, c->T, c->cfg.mac_scale, shared, ls);
+    if (shared) build_pair_index(c->lists.far_t.as<int>(), c->lists.num_far, c->T.num_nodes, c->m2l_ix, ls);
+    c->rec_on(E_LISTS, ls);
+    c->lists_ready = true;
+}
+
+// M2L and L2L (the lists are ready on c->stream's timeline)
+void stage_translate(fmm_ctx* c) {
     require(c->stage >= ST_UP, FMM_ERR_STATE, "Multipoles are not ready!");
+    require(c->lists_ready, FMM_ERR_STATE, "interaction lists are not built");
     hipStream_t st = c->stream;
     const TreeDev& S = c->S;
     const TreeDev& T = c->T;
     const bool shared = c->cfg.frame == FMM_FRAME_SHARED;
-    build_lists(c->lists, c->lscr, S, T, c->cfg.mac_scale, shared, st);
-    c->lists_ready = true;
-    // far pairs grouped by target node, for the atomic-free M2L accumulation
-    if (shared) build_pair_index(c->lists.far_t.as<int>(), c->lists.num_far, T.num_nodes, c->m2l_ix, st);
-    c->rec(E_LISTS);
+    c->rec(E_DOWN0);
     if (!shared) {
         downward_per_pair(c);
         return;
@@ -421,9 +456,11 @@ void stage_downward(fmm_ctx* c) {
         class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, c->lists.far_s.as<int>(),
                    c->lists.far_t.as<int>(), c->M.ptr, slab, c->Lp, EPI_STORE, st);
         c->rec(E_M2L_GEMM);
+        FMM_HIP(hipEventRecord(c->j_gemm, st));
         reduce_pairs(c->f64(), c->m2l_ix, slab, c->Lp, Lx, st);
     } else {
         c->rec(E_M2L_GEMM);
+        FMM_HIP(hipEventRecord(c->j_gemm, st));
     }
     c->rec(E_M2L);
     // L2L top-down (calculation_point_octree.cpp:277-299)
@@ -452,34 +489,103 @@ void stage_downward(fmm_ctx* c) {
     c->stage = ST_DOWN;
 }
 
+void stage_downward(fmm_ctx* c) {
+    require(c->stage >= ST_UP, FMM_ERR_STATE, "Multipoles are not ready!");
+    stage_lists(c, c->stream);
+    stage_translate(c);
+}
+
 __global__ void k_fill_nan(double* p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = __longlong_as_double(0x7ff8000000000000ll);
 }
 
-void stage_evaluate(fmm_ctx* c, double current, double* Bdev) {
-    require(c->stage >= ST_DOWN, FMM_ERR_STATE, "local expansions are not ready");
-    hipStream_t st = c->stream;
+// near field on stream ps (needs the lists and the packed sources)
+void stage_p2p(fmm_ctx* c, hipStream_t ps) {
     int nts = c->T.num_slots;
-    double* Bfar = c->Bfar.get<double>(3 * (size_t)std::max(nts, 1));
     double* Bnear = c->Bnear.get<double>(3 * (size_t)std::max(nts, 1));
-    launch_l2p(c->f64(), c->tgt_leaves.as<int>(), (int)c->h_tgt_leaves.size(), c->T, c->sigT, c->txyz.as<double>(),
-               c->Lx.ptr, c->p, c->Lp, Bfar, st);
-    c->rec(E_L2P);
+    c->rec_on(E_P2P0, ps);
     if (c->cfg.near_field) {
         launch_p2p(c->f64(), c->tgt_leaves.as<int>(), (int)c->h_tgt_leaves.size(), c->T, c->S, c->lists.close_begin.as<int>(),
                    c->lists.close_end.as<int>(), c->lists.close_s.as<int>(), c->nP0.ptr, c->nP1.ptr, c->nP2.ptr,
-                   c->txyz.as<double>(), Bnear, st);
+                   c->txyz.as<double>(), Bnear, ps);
     } else {
-        FMM_HIP(hipMemsetAsync(Bnear, 0, 3 * (size_t)std::max(nts, 1) * sizeof(double), st));
+        FMM_HIP(hipMemsetAsync(Bnear, 0, 3 * (size_t)std::max(nts, 1) * sizeof(double), ps));
     }
-    c->rec(E_P2P);
+    c->rec_on(E_P2P, ps);
+}
+
+void stage_l2p(fmm_ctx* c) {
+    require(c->stage >= ST_DOWN, FMM_ERR_STATE, "local expansions are not ready");
+    int nts = c->T.num_slots;
+    double* Bfar = c->Bfar.get<double>(3 * (size_t)std::max(nts, 1));
+    launch_l2p(c->f64(), c->tgt_leaves.as<int>(), (int)c->h_tgt_leaves.size(), c->T, c->sigT, c->txyz.as<double>(),
+               c->Lx.ptr, c->p, c->Lp, Bfar, c->stream);
+    c->rec(E_L2P);
+}
+
+void stage_assemble(fmm_ctx* c, double current, double* Bdev) {
+    hipStream_t st = c->stream;
+    int nts = c->T.num_slots;
     size_t nout = 3 * (size_t)c->nt;
     if (nout) k_fill_nan<<<(unsigned)((nout + 255) / 256), 256, 0, st>>>(Bdev, nout);
     c->current = current;
-    launch_assemble(c->T.slots.as<int>(), nts, Bfar, Bnear, current * REF_MU0 / (4.0 * REF_PI), Bdev, st);
+    launch_assemble(c->T.slots.as<int>(), nts, c->Bfar.as<double>(), c->Bnear.as<double>(),
+                    current * REF_MU0 / (4.0 * REF_PI), Bdev, st);
     c->rec(E_END);
     c->stage = ST_EVAL;
+}
+
+void stage_evaluate(fmm_ctx* c, double current, double* Bdev) {
+    require(c->stage >= ST_DOWN, FMM_ERR_STATE, "local expansions are not ready");
+    stage_l2p(c);
+    stage_p2p(c, c->stream);
+    stage_assemble(c, current, Bdev);
+}
+
+// The whole evaluation with the side stream: the lists are built (on a host thread, since
+// their compaction reads counts back) while P2M and M2M run, and the near field runs after the
+// M2L GEMM, beside the memory-bound slab reduction, L2L and L2P; the streams join before the
+// assembly.  The reference frame's per-pair M2L has no GEMM event split and runs staged.
+void run_fused(fmm_ctx* c, double current, double* Bdev) {
+    if (c->cfg.frame != FMM_FRAME_SHARED) {
+        stage_trees(c);
+        stage_leaves(c);
+        stage_upward(c);
+        stage_downward(c);
+        stage_evaluate(c, current, Bdev);
+        return;
+    }
+    hipStream_t A = c->stream, Bs = c->side;
+    stage_trees(c);
+    stage_leaves(c);
+    FMM_HIP(hipEventRecord(c->j_pack, A));
+    std::exception_ptr lists_error;
+    std::thread lists_thread([&] {
+        try {
+            stage_lists(c, Bs);   // the trees are complete (stage_trees reads them back)
+            FMM_HIP(hipEventRecord(c->j_lists, Bs));
+        } catch (...) {
+            lists_error = std::current_exception();
+        }
+    });
+    try {
+        stage_upward(c);
+    } catch (...) {
+        lists_thread.join();
+        throw;
+    }
+    lists_thread.join();
+    if (lists_error) std::rethrow_exception(lists_error);
+    FMM_HIP(hipStreamWaitEvent(A, c->j_lists, 0));
+    stage_translate(c);
+    FMM_HIP(hipStreamWaitEvent(Bs, c->j_pack, 0));
+    FMM_HIP(hipStreamWaitEvent(Bs, c->j_gemm, 0));
+    stage_p2p(c, Bs);
+    FMM_HIP(hipEventRecord(c->j_p2p, Bs));
+    stage_l2p(c);
+    FMM_HIP(hipStreamWaitEvent(A, c->j_p2p, 0));
+    stage_assemble(c, current, Bdev);
 }
 
 void set_sources_common(fmm_ctx* c, size_t n, const double* xyz, const double* w, const double* J, hipMemcpyKind kind) {
@@ -573,6 +679,11 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
         c->own_stream = true;
         for (int i = 0; i < E_COUNT; i++) FMM_HIP(hipEventCreate(&c->ev[i]));
         c->ev_ok = true;
+        FMM_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));   // default priority
+        FMM_HIP(hipEventCreateWithFlags(&c->j_lists, hipEventDisableTiming));
+        FMM_HIP(hipEventCreateWithFlags(&c->j_pack, hipEventDisableTiming));
+        FMM_HIP(hipEventCreateWithFlags(&c->j_p2p, hipEventDisableTiming));
+        FMM_HIP(hipEventCreateWithFlags(&c->j_gemm, hipEventDisableTiming));
     });
     if (rc != FMM_OK) {
         delete c;
@@ -585,8 +696,12 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
 void fmm_destroy(fmm_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_ok)
         for (int i = 0; i < E_COUNT; i++) (void)hipEventDestroy(c->ev[i]);
+    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm})
+        if (e) (void)hipEventDestroy(e);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -659,13 +774,7 @@ int fmm_evaluate_B(fmm_ctx* c, double current, double* B) {
     });
 }
 int fmm_run_B_dev(fmm_ctx* c, double current, double* B) {
-    return guard(c, [&] {
-        stage_trees(c);
-        stage_leaves(c);
-        stage_upward(c);
-        stage_downward(c);
-        stage_evaluate(c, current, B);
-    });
+    return guard(c, [&] { run_fused(c, current, B); });
 }
 
 int fmm_leaf_order(fmm_ctx* c, int64_t* order) {
@@ -797,12 +906,12 @@ int fmm_get_stats(fmm_ctx* c, fmm_stats* s) {
             s->t_tree = el(E_START, E_TREE);
             s->t_p2m = el(E_TREE, E_P2M);
             s->t_m2m = el(E_P2M, E_M2M);
-            s->t_lists = el(E_M2M, E_LISTS);
-            s->t_tables = el(E_LISTS, E_TABLES);
+            s->t_lists = el(E_LISTS0, E_LISTS);
+            s->t_tables = el(E_DOWN0, E_TABLES);
             s->t_m2l = el(E_TABLES, E_M2L);
             s->t_l2l = el(E_M2L, E_L2L);
             s->t_l2p = el(E_L2L, E_L2P);
-            s->t_p2p = el(E_L2P, E_P2P);
+            s->t_p2p = el(E_P2P0, E_P2P);
             s->t_total = el(E_START, E_END);
             s->t_m2l_gemm = c->cfg.frame == FMM_FRAME_SHARED ? el(E_TABLES, E_M2L_GEMM) : s->t_m2l;
         }

@@ -188,25 +188,24 @@ __global__ void k_close_keys(const int2* __restrict__ close, int n, uint64_t* __
     val[i] = close[i].y;
 }
 
-// keep the first of equal (key, value) runs
-__global__ void k_unique_flags(const uint64_t* __restrict__ key, const int* __restrict__ val, int n,
-                               uint32_t* __restrict__ flag) {
+
+// sorted keys back to (target, source[, class code]): far keys carry the target in the low
+// target_bits (class code above), close keys in the high word
+__global__ void k_unpack_pairs(const uint64_t* __restrict__ key, const int* __restrict__ val, int n, int target_high,
+                               int target_bits, int* __restrict__ out_t, int* __restrict__ out_s,
+                               uint32_t* __restrict__ out_code) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    flag[i] = (i == 0 || key[i] != key[i - 1] || val[i] != val[i - 1]) ? 1u : 0u;
+    out_t[i] = target_high ? (int)(uint32_t)(key[i] >> 32) : (int)(uint32_t)(key[i] & ((1ull << target_bits) - 1));
+    out_s[i] = val[i];
+    if (out_code) out_code[i] = (uint32_t)(key[i] >> target_bits);
 }
 
-// far keys carry the target in the low target_bits (class code above), close keys in the high word
-__global__ void k_compact_pairs(const uint64_t* __restrict__ key, const int* __restrict__ val,
-                                const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int n,
-                                int target_high, int target_bits, int* __restrict__ out_t, int* __restrict__ out_s,
-                                uint32_t* __restrict__ out_code) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flag[i]) return;
-    uint32_t p = pos[i];
-    out_t[p] = target_high ? (int)(uint32_t)(key[i] >> 32) : (int)(uint32_t)(key[i] & ((1ull << target_bits) - 1));
-    out_s[p] = val[i];
-    if (out_code) out_code[p] = (uint32_t)(key[i] >> target_bits);
+// the class count (scan total) and the key-range error flag in one readback
+__global__ void k_class_count(const uint32_t* __restrict__ pos, const uint32_t* __restrict__ flag, int n,
+                              const unsigned* __restrict__ err, unsigned* __restrict__ out) {
+    out[0] = pos[n - 1] + flag[n - 1];
+    out[1] = *err;
 }
 
 // segment boundaries: class starts (far) / target starts (close)
@@ -253,21 +252,16 @@ static int bits_for(int n) {   // smallest b with 2^b >= n (at least 1)
     return b;
 }
 
-// sort by the low end_bit bits of the 64-bit key, drop duplicate (key, value) pairs; returns
-// the unique count
-static int sort_unique(ListScratch& L, int n, int end_bit, int target_high, int target_bits, int* out_t, int* out_s,
-                       uint32_t* out_code, hipStream_t st) {
-    if (n == 0) return 0;
+// sort by the low end_bit bits of the 64-bit key and unpack.  The traversal reaches every
+// (source, target) pair at most once (each pair has a unique parent pair), so the sets need no
+// de-duplication, unlike the reference's std::set inserts.
+static void sort_unpack(ListScratch& L, int n, int end_bit, int target_high, int target_bits, int* out_t, int* out_s,
+                        uint32_t* out_code, hipStream_t st) {
+    if (n == 0) return;
     uint64_t* k2 = L.key2.get<uint64_t>(n);
     int* v2 = L.val2.get<int>(n);
     sort_pairs_u64(L.key.as<uint64_t>(), k2, L.val.as<int>(), v2, n, end_bit, L.tmp, st);
-    uint32_t* flag = L.flag.get<uint32_t>(n);
-    uint32_t* pos = L.pos.get<uint32_t>(n);
-    k_unique_flags<<<blocks(n), 256, 0, st>>>(k2, v2, n, flag);
-    exclusive_scan_u32(flag, pos, n, L.tmp, st);
-    int nu = (int)(read_u32(pos + n - 1, st) + read_u32(flag + n - 1, st));
-    k_compact_pairs<<<blocks(n), 256, 0, st>>>(k2, v2, flag, pos, n, target_high, target_bits, out_t, out_s, out_code);
-    return nu;
+    k_unpack_pairs<<<blocks(n), 256, 0, st>>>(k2, v2, n, target_high, target_bits, out_t, out_s, out_code);
 }
 
 void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T, double mac_scale, bool classes,
@@ -284,7 +278,7 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
     // every step descends one of the two trees by a level: the traversal ends within this many
     const int max_steps = std::max(S.num_levels, 1) + std::max(T.num_levels, 1);
     // counters: far, close, error flag, then one frontier count per step
-    unsigned* cnt = L.counters.get<unsigned>(4 + max_steps + 1);
+    unsigned* cnt = L.counters.get<unsigned>(4 + max_steps + 1 + 2);
     unsigned far_cap = std::max<unsigned>(L.far_cap, 1u << 16), close_cap = std::max<unsigned>(L.close_cap, 1u << 16),
              front_cap = std::max<unsigned>(L.front_cap, 1u << 16);
     for (int attempt = 0; attempt < 8; attempt++) {
@@ -347,48 +341,49 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
             if (nfar > 0)
                 k_far_keys<<<blocks(nfar), 256, 0, st>>>(far, nfar, srec, trec, dh, dh + MAX_LEVELS, classes ? 1 : 0,
                                                          tbits, key, val, err);
-            if (classes && read_u32(err, st)) throw Error(4, "interaction offset out of the class-key range");
             int* ft = out.far_t.get<int>(std::max(nfar, 1));
             int* fs = out.far_s.get<int>(std::max(nfar, 1));
             uint32_t* code = out.far_code.get<uint32_t>(std::max(nfar, 1));
-            int nu = sort_unique(L, nfar, tbits + (classes ? 30 : 0), 0, tbits, ft, fs, code, st);
-            out.num_far = nu;
+            sort_unpack(L, nfar, tbits + (classes ? 30 : 0), 0, tbits, ft, fs, code, st);
+            out.num_far = nfar;
             // classes
             out.num_classes = 0;
             out.h_class_begin.clear();
-            if (nu > 0) {
-                if (classes) {
-                    uint32_t* flag = L.flag.get<uint32_t>(nu);
-                    uint32_t* pos = L.pos.get<uint32_t>(nu);
-                    k_seg_flags<<<blocks(nu), 256, 0, st>>>(code, nu, flag);
-                    exclusive_scan_u32(flag, pos, nu, L.tmp, st);
-                    int nc = (int)(read_u32(pos + nu - 1, st) + read_u32(flag + nu - 1, st));
-                    int* starts = out.class_begin.get<int>(nc + 1);
-                    k_seg_starts<<<blocks(nu), 256, 0, st>>>(flag, pos, nu, starts);
-                    out.h_class_begin.resize(nc + 1);
-                    FMM_HIP(hipMemcpyAsync(out.h_class_begin.data(), starts, nc * sizeof(int), hipMemcpyDeviceToHost, st));
-                    FMM_HIP(hipStreamSynchronize(st));
-                    out.h_class_begin[nc] = nu;
-                    out.num_classes = nc;
-                } else {
-                    out.num_classes = 0;
-                }
+            if (nfar > 0 && classes) {
+                uint32_t* flag = L.flag.get<uint32_t>(nfar);
+                uint32_t* pos = L.pos.get<uint32_t>(nfar);
+                k_seg_flags<<<blocks(nfar), 256, 0, st>>>(code, nfar, flag);
+                exclusive_scan_u32(flag, pos, nfar, L.tmp, st);
+                unsigned* cc = cnt + 4 + max_steps + 1;
+                k_class_count<<<1, 1, 0, st>>>(pos, flag, nfar, err, cc);
+                unsigned hc[2];
+                FMM_HIP(hipMemcpyAsync(hc, cc, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+                FMM_HIP(hipStreamSynchronize(st));
+                if (hc[1]) throw Error(4, "interaction offset out of the class-key range");
+                int nc = (int)hc[0];
+                int* starts = out.class_begin.get<int>(nc + 1);
+                k_seg_starts<<<blocks(nfar), 256, 0, st>>>(flag, pos, nfar, starts);
+                out.h_class_begin.resize(nc + 1);
+                FMM_HIP(hipMemcpyAsync(out.h_class_begin.data(), starts, nc * sizeof(int), hipMemcpyDeviceToHost, st));
+                FMM_HIP(hipStreamSynchronize(st));
+                out.h_class_begin[nc] = nfar;
+                out.num_classes = nc;
             }
         }
-        // ---- close pairs: sort by (target, source), de-duplicate, per-target ranges
+        // ---- close pairs: sort by (target, source), per-target ranges
         {
             uint64_t* key = L.key.get<uint64_t>(std::max(nclose, 1));
             int* val = L.val.get<int>(std::max(nclose, 1));
             if (nclose > 0) k_close_keys<<<blocks(nclose), 256, 0, st>>>(close, nclose, key, val);
             int* ct = out.close_t.get<int>(std::max(nclose, 1));
             int* cs = out.close_s.get<int>(std::max(nclose, 1));
-            int nu = sort_unique(L, nclose, 64, 1, 32, ct, cs, nullptr, st);
-            out.num_close = nu;
+            sort_unpack(L, nclose, 64, 1, 32, ct, cs, nullptr, st);
+            out.num_close = nclose;
             int* cb = out.close_begin.get<int>(std::max(T.num_nodes, 1));
             int* ce = out.close_end.get<int>(std::max(T.num_nodes, 1));
             k_fill_int<<<blocks(T.num_nodes), 256, 0, st>>>(cb, T.num_nodes, 0);
             k_fill_int<<<blocks(T.num_nodes), 256, 0, st>>>(ce, T.num_nodes, 0);
-            if (nu > 0) k_target_ranges<<<blocks(nu), 256, 0, st>>>(ct, nu, cb, ce);
+            if (nclose > 0) k_target_ranges<<<blocks(nclose), 256, 0, st>>>(ct, nclose, cb, ce);
         }
         FMM_HIP(hipStreamSynchronize(st));
         return;

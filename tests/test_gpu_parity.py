@@ -240,3 +240,20 @@ def test_reference_frame_fmm_matches_oracle(fm, orc, precision, tol):
     ok = np.isfinite(Bo[:, 0])
     assert np.array_equal(np.isfinite(B[:, 0]), ok)
     assert rel_l2(B[ok], Bo[ok]) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+def test_fused_run_matches_staged_run(fm, precision, tol):
+    """fmm_run_B_dev (lists and near field on a side stream) equals the stage-by-stage calls; the
+    only difference allowed is the order of the atomic M2M additions on narrow levels."""
+    import torch
+    src, w, tgt = _inputs(21, 20000, 6000)
+    F = fm.FMM(order=8, source_leaf_capacity=32, target_leaf_capacity=32, precision=precision, frame=1)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    B_staged = F.run(1.0)
+    out = torch.empty((len(tgt), 3), dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        F.run_B_dev(1.0, out.data_ptr())
+        F.synchronize()
+        assert rel_l2(out.cpu().numpy(), B_staged) < tol
