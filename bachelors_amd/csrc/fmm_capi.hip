@@ -894,6 +894,8 @@ int fmm_get_stats(fmm_ctx* c, fmm_stats* s) {
                 tot += (long long)(c->T.h_end[ct[i]] - c->T.h_begin[ct[i]]) * (c->S.h_end[cs[i]] - c->S.h_begin[cs[i]]);
             c->near_inter = tot;
         }
+        s->source_leaves = (int)c->h_src_leaves.size();
+        s->target_leaves = (int)c->h_tgt_leaves.size();
         s->source_tree_sorted = c->S.sorted_build ? 1 : 0;
         s->target_tree_sorted = c->T.sorted_build ? 1 : 0;
         s->near_interactions = c->near_inter < 0 ? 0 : c->near_inter;

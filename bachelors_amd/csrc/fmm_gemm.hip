@@ -57,7 +57,9 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
                                               const int* __restrict__ pair_src, const int* __restrict__ pair_dst,
                                               const T* __restrict__ src_exp, T* __restrict__ dst_exp, int epi,
                                               int nseg) {
-    constexpr int Lp = 64 * RB, BK = 16, SA = Lp + 4, SB = BN + 4, NK = Lp / BK;
+    // LDS row strides: four K-rows are read by the 16-lane groups of a wave at once; strides of
+    // 16 (mod 64) words put the four rows on disjoint bank ranges (FP64: 8 doubles = 16 words)
+    constexpr int Lp = 64 * RB, BK = 16, PAD = sizeof(T) == 8 ? 8 : 16, SA = Lp + PAD, SB = BN + PAD, NK = Lp / BK;
     const int NKT = MULTI ? nseg * NK : NK;   // single-segment launches keep a compile-time trip count
     typedef Vec4<T> V;
     typedef typename V::t v4;

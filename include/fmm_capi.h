@@ -79,6 +79,7 @@ typedef struct fmm_stats {
     double t_m2l_gemm;                   /* the M2L translation GEMM alone (part of t_m2l) */
     int source_tree_sorted, target_tree_sorted;   /* 1: built by one sort of point paths;
                                                      0: level by level (gaps/overlaps of child boxes) */
+    int source_leaves, target_leaves;    /* leaves holding points */
 } fmm_stats;
 
 void fmm_config_init(fmm_config* cfg);
