@@ -255,7 +255,7 @@ void run_adjoint(AdjointState& S, const AdjointArgs& a, const double* dLdB, doub
         void* slab = S.slab.reserve((size_t)3 * npairs * a.Lp * es);
         class_gemm(a.prec, a.down_ops, a.down_ops_lo, a.m2l.tiles, a.m2l.ntiles, a.m2l.src, a.m2l.dst, lam, slab,
                    a.Lp, EPI_STORE, st, true);
-        reduce_pairs(f64, S.m2l_ix, slab, a.Lp, mu, st);
+        reduce_pairs(f64, S.m2l_ix, slab, a.Lp, a.Lp, mu, st);
     }
     // M2M^T top-down (forward child -> parent; here parent -> child, one write per child)
     for (int k = 1; k < Ts.num_levels; k++) {

@@ -452,12 +452,13 @@ void stage_translate(fmm_ctx* c) {
     }
     c->rec(E_TABLES);
     if (P.ntiles) {
-        void* slab = c->m2l_slab.reserve((size_t)3 * P.npairs * c->Lp * c->esize());
-        class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, c->lists.far_s.as<int>(),
-                   c->lists.far_t.as<int>(), c->M.ptr, slab, c->Lp, EPI_STORE, st);
+        const int R = m2l_slab_rows(c->prec, c->L, c->Lp);
+        void* slab = c->m2l_slab.reserve((size_t)3 * P.npairs * R * c->esize());
+        m2l_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, c->lists.far_s.as<int>(), c->M.ptr,
+                 slab, c->L, c->Lp, st);
         c->rec(E_M2L_GEMM);
         FMM_HIP(hipEventRecord(c->j_gemm, st));
-        reduce_pairs(c->f64(), c->m2l_ix, slab, c->Lp, Lx, st);
+        reduce_pairs(c->f64(), c->m2l_ix, slab, R, c->Lp, Lx, st);
     } else {
         c->rec(E_M2L_GEMM);
         FMM_HIP(hipEventRecord(c->j_gemm, st));

@@ -307,6 +307,143 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
     }
 }
 
+// ---------------------------------------------------------------- M2L, rows padded to 16
+// The M2L launch with rows and K padded only to R = 16 NB (>= L, <= Lp) instead of Lp: wave w
+// owns the 16 columns [16w, 16w + 16) of the 64-column tile and all NB row blocks, so an odd
+// block count leaves no wave idle.  Each column is written to the per-pair slab with row
+// stride R (rows >= L are zero: the operators are zero there).
+constexpr int lds_stride(int rows, int words_per_elem) {
+    // rows + pad with the stride = 16 (mod 64) 4-byte words: the four K-rows read at once by the
+    // 16-lane groups of a wave fall on disjoint LDS banks
+    return words_per_elem == 1 ? rows + ((16 - rows % 64) + 64) % 64 : rows + ((8 - rows % 32) + 32) % 32;
+}
+
+template <class T, int NB>
+__global__ __launch_bounds__(256) void k_gemm_m2l(const T* __restrict__ ops, const GemmTile* __restrict__ tiles,
+                                                  const int* __restrict__ pair_src, const T* __restrict__ src_exp,
+                                                  T* __restrict__ slab, int Lp) {
+    constexpr int R = 16 * NB, BK = 16, NK = NB, W = sizeof(T) / 4;
+    constexpr int SA = lds_stride(R, W), SB = lds_stride(BN, W);
+    constexpr int NQ = (NB * 64 + 255) / 256;   // A quads (4 elements) per thread and K step
+    typedef Vec4<T> V;
+    typedef typename V::t v4;
+    typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
+    __shared__ T As[2][BK][SA];
+    __shared__ T Bs[2][BK][SB];
+    const GemmTile tile = tiles[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const T* A = ops + (size_t)tile.op * Lp * Lp;
+    const int bc = tid >> 2, bq = (tid & 3) * 4;
+    const T* bptr = nullptr;
+    if (bc < tile.ncol) {
+        int col = tile.col0 + bc;
+        int pair = col / 3, comp = col - 3 * pair;
+        bptr = src_exp + ((size_t)pair_src[pair] * 3 + comp) * Lp;
+    }
+    v4 ra[NQ], rb;
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            int qi = tid + 256 * j;
+            if (qi < NB * 64) {
+                int row = qi >> 2, kq = (qi & 3) * 4;
+                ra[j] = V::load(A + (size_t)row * Lp + k0 + kq);
+            }
+        }
+        if (bptr) rb = V::load(bptr + k0 + bq);
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            int qi = tid + 256 * j;
+            if (qi < NB * 64) {
+                int row = qi >> 2, kq = (qi & 3) * 4;
+                for (int i = 0; i < 4; i++) As[buf][kq + i][row] = V::get(ra[j], i);
+            }
+        }
+        for (int i = 0; i < 4; i++) Bs[buf][bq + i][bc] = bptr ? V::get(rb, i) : T(0);
+    };
+    acc_t acc[NB];
+#pragma unroll
+    for (int a = 0; a < NB; a++) acc[a] = acc_t{0, 0, 0, 0};
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < NK; kt++) {
+        const int cur = kt & 1;
+        if (kt + 1 < NK) gload((kt + 1) * BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            const int kk = ks + (lane >> 4);
+            const T bf = Bs[cur][kk][wave * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < NB; a++) {
+                const T af = As[cur][kk][a * 16 + (lane & 15)];
+                if constexpr (sizeof(T) == 8)
+                    acc[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, acc[a], 0, 0, 0);
+                else
+                    acc[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(af, bf, acc[a], 0, 0, 0);
+            }
+        }
+        if (kt + 1 < NK) lstore(cur ^ 1);
+        __syncthreads();
+    }
+    const int cl = wave * 16 + (lane & 15);
+    if (cl >= tile.ncol) return;
+    T* out = slab + (size_t)(tile.col0 + cl) * R;
+#pragma unroll
+    for (int a = 0; a < NB; a++) {
+        if constexpr (sizeof(T) == 8) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) out[a * 16 + 4 * r + (lane >> 4)] = acc[a][r];
+        } else {
+            *(float4*)(out + a * 16 + 4 * (lane >> 4)) = make_float4(acc[a][0], acc[a][1], acc[a][2], acc[a][3]);
+        }
+    }
+}
+
+template <int NB>
+static void launch_m2l_nb(Precision prec, const void* ops, const GemmTile* d_tiles, int ntiles, const int* pair_src,
+                          const void* src_exp, void* slab, int Lp, hipStream_t st) {
+    if (prec == Precision::F64)
+        k_gemm_m2l<double, NB><<<ntiles, 256, 0, st>>>((const double*)ops, d_tiles, pair_src, (const double*)src_exp,
+                                                       (double*)slab, Lp);
+    else
+        k_gemm_m2l<float, NB><<<ntiles, 256, 0, st>>>((const float*)ops, d_tiles, pair_src, (const float*)src_exp,
+                                                      (float*)slab, Lp);
+    FMM_HIP(hipGetLastError());
+}
+
+int m2l_slab_rows(Precision prec, int L, int Lp) {
+    if (prec == Precision::BF16X3) return Lp;
+    int nb = (L + 15) / 16;
+    switch (nb) {
+        case 1: case 2: case 3: case 4: case 6: case 7: case 8: case 9: case 11: case 13: case 15: case 16: case 19:
+        case 21: case 23: case 25: case 28:
+            return 16 * nb;
+        default:
+            return Lp;
+    }
+}
+
+void m2l_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
+              const int* pair_src, const void* src_exp, void* slab, int L, int Lp, hipStream_t st) {
+    if (ntiles == 0) return;
+    const int R = m2l_slab_rows(prec, L, Lp);
+    if (R == Lp) {   // the whole-panel kernel (BF16x3, or an L without a padded-to-16 instance)
+        class_gemm(prec, ops, ops_lo, d_tiles, ntiles, pair_src, nullptr, src_exp, slab, Lp, EPI_STORE, st);
+        return;
+    }
+#define FMM_NB(N) \
+    case N: launch_m2l_nb<N>(prec, ops, d_tiles, ntiles, pair_src, src_exp, slab, Lp, st); break;
+    switch (R / 16) {
+        FMM_NB(1) FMM_NB(2) FMM_NB(3) FMM_NB(4) FMM_NB(6) FMM_NB(7) FMM_NB(8) FMM_NB(9) FMM_NB(11) FMM_NB(13)
+        FMM_NB(15) FMM_NB(16) FMM_NB(19) FMM_NB(21) FMM_NB(23) FMM_NB(25) FMM_NB(28)
+        default: throw Error(4, "no padded-row M2L kernel for this order");
+    }
+#undef FMM_NB
+}
+
 template <int RB, bool TR, bool MU>
 static void launch_rb(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
                       const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int epi,
@@ -407,17 +544,20 @@ void build_pair_index(const int* key, int npairs, int nnodes, PairIndex& ix, hip
     FMM_HIP(hipGetLastError());
 }
 
-// one wave per node, 16-byte vectors across the node's 3 Lp values, pairs added in order
+// one wave per node, 16-byte vectors across the node's 3 x R slab values (R = slab row length,
+// the first R of each component's Lp coefficients), pairs added in order
 template <class V>
 __global__ __launch_bounds__(64) void k_reduce_pairs(const int* __restrict__ begin, const int* __restrict__ end,
-                                                     const int* __restrict__ order, const V* __restrict__ slab, int nv,
-                                                     V* __restrict__ out) {
+                                                     const int* __restrict__ order, const V* __restrict__ slab, int vr,
+                                                     int vl, V* __restrict__ out) {
     const int v = blockIdx.x;
     const int b = begin[v], e = end[v];
     if (b >= e) return;
-    V* o = out + (size_t)v * nv;
+    const int nv = 3 * vr;   // vectors per slab row block (vr per component)
+    V* o = out + (size_t)v * 3 * vl;
     for (int i = threadIdx.x; i < nv; i += 64) {
-        V acc = o[i];
+        const int c = i / vr, q = i - c * vr;
+        V acc = o[c * vl + q];
         int k = b;
         for (; k + 4 <= e; k += 4) {
             V x0 = slab[(size_t)order[k] * nv + i], x1 = slab[(size_t)order[k + 1] * nv + i];
@@ -428,21 +568,18 @@ __global__ __launch_bounds__(64) void k_reduce_pairs(const int* __restrict__ beg
             acc = acc + x3;
         }
         for (; k < e; k++) acc = acc + slab[(size_t)order[k] * nv + i];
-        o[i] = acc;
+        o[c * vl + q] = acc;
     }
 }
 
-void reduce_pairs(bool f64, const PairIndex& ix, const void* slab, int Lp, void* out, hipStream_t st) {
+void reduce_pairs(bool f64, const PairIndex& ix, const void* slab, int R, int Lp, void* out, hipStream_t st) {
     if (!ix.npairs || !ix.nnodes) return;
-    if (f64) {
-        int nv = 3 * Lp / 2;
+    if (f64)
         k_reduce_pairs<double2><<<ix.nnodes, 64, 0, st>>>(ix.begin.as<int>(), ix.end.as<int>(), ix.order.as<int>(),
-                                                          (const double2*)slab, nv, (double2*)out);
-    } else {
-        int nv = 3 * Lp / 4;
+                                                          (const double2*)slab, R / 2, Lp / 2, (double2*)out);
+    else
         k_reduce_pairs<float4><<<ix.nnodes, 64, 0, st>>>(ix.begin.as<int>(), ix.end.as<int>(), ix.order.as<int>(),
-                                                         (const float4*)slab, nv, (float4*)out);
-    }
+                                                         (const float4*)slab, R / 4, Lp / 4, (float4*)out);
     FMM_HIP(hipGetLastError());
 }
 

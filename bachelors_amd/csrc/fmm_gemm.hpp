@@ -35,9 +35,16 @@ struct PairIndex {
 };
 void build_pair_index(const int* key, int npairs, int nnodes, PairIndex& ix, hipStream_t st);
 
-// out[v][0 .. 3 Lp) += sum over the pairs of v, in ascending pair order, of slab[pair][0 .. 3 Lp)
-// (slab = an EPI_STORE result); elements are double if f64 else float
-void reduce_pairs(bool f64, const PairIndex& ix, const void* slab, int Lp, void* out, hipStream_t st);
+// out[v][c][0 .. R) += sum over the pairs of v, in ascending pair order, of slab[pair][c][0 .. R)
+// (slab = a per-pair store with row length R <= Lp, multiple of 4); double if f64 else float
+void reduce_pairs(bool f64, const PairIndex& ix, const void* slab, int R, int Lp, void* out, hipStream_t st);
+
+// Row length of the M2L slab: L padded to 16 (MFMA block) where a kernel instance exists for it
+// (FP32/FP64), else Lp (BF16X3, whose K step is 32, and orders without an instance).
+int m2l_slab_rows(Precision prec, int L, int Lp);
+// M2L of all classes into the slab [3 * pairs][m2l_slab_rows] (column 3 pair + component).
+void m2l_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
+              const int* pair_src, const void* src_exp, void* slab, int L, int Lp, hipStream_t st);
 
 void plan_tiles(const std::vector<int>& class_begin, const std::vector<int>& class_op, std::vector<GemmTile>& tiles);
 

@@ -107,7 +107,10 @@ def other_stage_rooflines(st, L, args):
                    "achieved_TFs_at_20_flop": flops / (st["t_p2p"] * 1e-3) / 1e12 if st["t_p2p"] > 0 else 0.0,
                    "peak_TFs": PEAK_TFLOPS["fp32"]}
     red_ms = st["t_m2l"] - st["t_m2l_gemm"]
-    red_bytes = st["far_pairs"] * 3 * Lp * b_coef + 2 * st["target_nodes"] * 3 * Lp * b_coef
+    # slab rows: L padded to 16 for FP32/FP64 (the library's M2L kernel instances cover every
+    # order whose 16-padded length is below Lp), Lp for BF16x3
+    R = Lp if args.precision == "bf16x3" else min(Lp, (L + 15) // 16 * 16)
+    red_bytes = st["far_pairs"] * 3 * R * b_coef + 2 * st["target_nodes"] * 3 * Lp * b_coef
     rows["m2l_reduction"] = {"ms": red_ms, "bytes": red_bytes, "bound": "hbm",
                              "achieved_GBs": red_bytes / (red_ms * 1e-3) / 1e9 if red_ms > 0 else 0.0}
     for r in rows.values():
