@@ -86,7 +86,8 @@ struct fmm_ctx {
     bool ev_ok = false;
     // side stream of the fused evaluation (lists and P2P beside the multipole chain) and its joins
     hipStream_t side = nullptr;
-    hipEvent_t j_lists = nullptr, j_pack = nullptr, j_p2p = nullptr, j_gemm = nullptr;
+    hipEvent_t j_lists = nullptr, j_pack = nullptr, j_p2p = nullptr, j_gemm = nullptr, j_pairs = nullptr;
+    bool pairs_on_side = false;   // the M2L reduction waits for j_pairs (fused evaluation)
     long long near_inter = -1;
     bool lists_ready = false;
 
@@ -385,13 +386,16 @@ void downward_per_pair(fmm_ctx* c) {
 
 // interaction lists (and, in the shared frame, far pairs grouped by target node for the
 // atomic-free M2L accumulation) on stream ls; they depend on the trees only
-void stage_lists(fmm_ctx* c, hipStream_t ls) {
+// far_ready (may be null): recorded on ls once the far pairs and classes are complete; the
+// close pairs and the target grouping follow, then E_LISTS and j_pairs
+void stage_lists(fmm_ctx* c, hipStream_t ls, hipEvent_t far_ready) {
     require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built");
     const bool shared = c->cfg.frame == FMM_FRAME_SHARED;
     c->rec_on(E_LISTS0, ls);
-    build_lists(c->lists, c->lscr, c->S, c->T, c->cfg.mac_scale, shared, ls);
+    build_lists(c->lists, c->lscr, c->S, c->T, c->cfg.mac_scale, shared, ls, far_ready);
     if (shared) build_pair_index(c->lists.far_t.as<int>(), c->lists.num_far, c->T.num_nodes, c->m2l_ix, ls);
     c->rec_on(E_LISTS, ls);
+    FMM_HIP(hipEventRecord(c->j_pairs, ls));
     c->lists_ready = true;
 }
 
@@ -458,6 +462,7 @@ void stage_translate(fmm_ctx* c) {
                  slab, c->L, c->Lp, st);
         c->rec(E_M2L_GEMM);
         FMM_HIP(hipEventRecord(c->j_gemm, st));
+        if (c->pairs_on_side) FMM_HIP(hipStreamWaitEvent(st, c->j_pairs, 0));
         reduce_pairs(c->f64(), c->m2l_ix, slab, R, c->Lp, Lx, st);
     } else {
         c->rec(E_M2L_GEMM);
@@ -492,7 +497,7 @@ void stage_translate(fmm_ctx* c) {
 
 void stage_downward(fmm_ctx* c) {
     require(c->stage >= ST_UP, FMM_ERR_STATE, "Multipoles are not ready!");
-    stage_lists(c, c->stream);
+    stage_lists(c, c->stream, nullptr);
     stage_translate(c);
 }
 
@@ -564,8 +569,7 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     std::exception_ptr lists_error;
     std::thread lists_thread([&] {
         try {
-            stage_lists(c, Bs);   // the trees are complete (stage_trees reads them back)
-            FMM_HIP(hipEventRecord(c->j_lists, Bs));
+            stage_lists(c, Bs, c->j_lists);   // the trees are complete (stage_trees reads them back)
         } catch (...) {
             lists_error = std::current_exception();
         }
@@ -578,8 +582,15 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     }
     lists_thread.join();
     if (lists_error) std::rethrow_exception(lists_error);
-    FMM_HIP(hipStreamWaitEvent(A, c->j_lists, 0));
-    stage_translate(c);
+    FMM_HIP(hipStreamWaitEvent(A, c->j_lists, 0));   // far pairs and classes
+    c->pairs_on_side = true;                         // the reduction waits for the target grouping
+    try {
+        stage_translate(c);
+    } catch (...) {
+        c->pairs_on_side = false;
+        throw;
+    }
+    c->pairs_on_side = false;
     FMM_HIP(hipStreamWaitEvent(Bs, c->j_pack, 0));
     FMM_HIP(hipStreamWaitEvent(Bs, c->j_gemm, 0));
     stage_p2p(c, Bs);
@@ -685,6 +696,7 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
         FMM_HIP(hipEventCreateWithFlags(&c->j_pack, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_p2p, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_gemm, hipEventDisableTiming));
+        FMM_HIP(hipEventCreateWithFlags(&c->j_pairs, hipEventDisableTiming));
     });
     if (rc != FMM_OK) {
         delete c;
@@ -700,7 +712,7 @@ void fmm_destroy(fmm_ctx* c) {
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_ok)
         for (int i = 0; i < E_COUNT; i++) (void)hipEventDestroy(c->ev[i]);
-    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm})
+    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm, c->j_pairs})
         if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

@@ -267,7 +267,9 @@ template <> struct V4<double> {
     typedef double4 t;
     static __device__ double4 make(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
 };
-__device__ __forceinline__ float rsqrt_t(float x) { return rsqrtf(x); }
+// v_rsq_f32 directly (1 ulp): pair distances squared are far above the denormal range, so the
+// denormal-scaling wrapper of rsqrtf is not needed
+__device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
 
 // P0[j] = (hi(y_j - o), q_x), P1[j] = (lo(y_j - o), q_y), P2[j] = q_z for every source slot j

@@ -330,7 +330,11 @@ __global__ __launch_bounds__(256) void k_gemm_m2l(const T* __restrict__ ops, con
     typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
     __shared__ T As[2][BK][SA];
     __shared__ T Bs[2][BK][SB];
-    const GemmTile tile = tiles[blockIdx.x];
+    // workgroups are dealt round-robin to the 8 XCDs: give each XCD a contiguous eighth of the
+    // tiles, so the tiles of one class (one operator) share an L2 (a bijection in any case)
+    const int nt = gridDim.x, xcd = blockIdx.x & 7, q = nt >> 3, rem = nt & 7;
+    const int t = xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
+    const GemmTile tile = tiles[t];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const T* A = ops + (size_t)tile.op * Lp * Lp;
     const int bc = tid >> 2, bq = (tid & 3) * 4;

@@ -265,7 +265,7 @@ static void sort_unpack(ListScratch& L, int n, int end_bit, int target_high, int
 }
 
 void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T, double mac_scale, bool classes,
-                 hipStream_t st) {
+                 hipStream_t st, hipEvent_t far_ready) {
     RadiusTable rs = radii(S), rt = radii(T);
     NodeRec* srec = L.srec.get<NodeRec>(std::max(S.num_nodes, 1));
     NodeRec* trec = L.trec.get<NodeRec>(std::max(T.num_nodes, 1));
@@ -370,6 +370,7 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
                 out.num_classes = nc;
             }
         }
+        if (far_ready) FMM_HIP(hipEventRecord(far_ready, st));
         // ---- close pairs: sort by (target, source), per-target ranges
         {
             uint64_t* key = L.key.get<uint64_t>(std::max(nclose, 1));

@@ -22,7 +22,9 @@ struct Lists {
     DevBuf close_begin, close_end;          // per target node
 };
 
+// far_ready (may be null) is recorded on st once the far pairs and classes are complete, before
+// the close pairs are sorted
 void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T, double mac_scale, bool classes,
-                 hipStream_t st);
+                 hipStream_t st, hipEvent_t far_ready = nullptr);
 
 }  // namespace fmm
