@@ -44,11 +44,13 @@ template <> struct Vec4<float> {
     typedef float4 t;
     static __device__ __forceinline__ float4 load(const float* p) { return *(const float4*)p; }
     static __device__ __forceinline__ float get(const float4& v, int i) { return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w; }
+    static __device__ __forceinline__ float4 zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 };
 template <> struct Vec4<double> {
     struct t { double2 a, b; };
     static __device__ __forceinline__ t load(const double* p) { return t{*(const double2*)p, *(const double2*)(p + 2)}; }
     static __device__ __forceinline__ double get(const t& v, int i) { return i == 0 ? v.a.x : i == 1 ? v.a.y : i == 2 ? v.b.x : v.b.y; }
+    static __device__ __forceinline__ t zero() { return t{make_double2(0.0, 0.0), make_double2(0.0, 0.0)}; }
 };
 
 // ---------------------------------------------------------------- FP64 / FP32
@@ -322,20 +324,22 @@ template <class T, int NB>
 __global__ __launch_bounds__(256) void k_gemm_m2l(const T* __restrict__ ops, const GemmTile* __restrict__ tiles,
                                                   const int* __restrict__ pair_src, const T* __restrict__ src_exp,
                                                   T* __restrict__ slab, int Lp) {
-    constexpr int R = 16 * NB, BK = 16, NK = NB, W = sizeof(T) / 4;
-    constexpr int SA = lds_stride(R, W), SB = lds_stride(BN, W);
-    constexpr int NQ = (NB * 64 + 255) / 256;   // A quads (4 elements) per thread and K step
+    constexpr int R = 16 * NB, BK = 16, NK = NB;
+    // k-contiguous LDS rows: the 16-lane group g of a wave takes k = 4g .. 4g+3 of a K step, one
+    // per MFMA (the same order for A and B, so the sum is unchanged), read as one 4-vector
+    constexpr int SK = BK + 16 / (int)sizeof(T);   // row stride in elements (pad 16 bytes)
+    constexpr int NQ = (NB * 64 + 255) / 256;      // A quads (4 elements) per thread and K step
     typedef Vec4<T> V;
     typedef typename V::t v4;
     typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
-    __shared__ T As[2][BK][SA];
-    __shared__ T Bs[2][BK][SB];
+    __shared__ __attribute__((aligned(16))) T As[2][R][SK];
+    __shared__ __attribute__((aligned(16))) T Bs[2][BN][SK];
     // workgroups are dealt round-robin to the 8 XCDs: give each XCD a contiguous eighth of the
     // tiles, so the tiles of one class (one operator) share an L2 (a bijection in any case)
     const int nt = gridDim.x, xcd = blockIdx.x & 7, q = nt >> 3, rem = nt & 7;
     const int t = xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
     const GemmTile tile = tiles[t];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, li = lane & 15;
     const T* A = ops + (size_t)tile.op * Lp * Lp;
     const int bc = tid >> 2, bq = (tid & 3) * 4;
     const T* bptr = nullptr;
@@ -349,10 +353,7 @@ __global__ __launch_bounds__(256) void k_gemm_m2l(const T* __restrict__ ops, con
 #pragma unroll
         for (int j = 0; j < NQ; j++) {
             int qi = tid + 256 * j;
-            if (qi < NB * 64) {
-                int row = qi >> 2, kq = (qi & 3) * 4;
-                ra[j] = V::load(A + (size_t)row * Lp + k0 + kq);
-            }
+            if (qi < NB * 64) ra[j] = V::load(A + (size_t)(qi >> 2) * Lp + k0 + (qi & 3) * 4);
         }
         if (bptr) rb = V::load(bptr + k0 + bq);
     };
@@ -360,12 +361,10 @@ __global__ __launch_bounds__(256) void k_gemm_m2l(const T* __restrict__ ops, con
 #pragma unroll
         for (int j = 0; j < NQ; j++) {
             int qi = tid + 256 * j;
-            if (qi < NB * 64) {
-                int row = qi >> 2, kq = (qi & 3) * 4;
-                for (int i = 0; i < 4; i++) As[buf][kq + i][row] = V::get(ra[j], i);
-            }
+            if (qi < NB * 64) *(v4*)&As[buf][qi >> 2][(qi & 3) * 4] = ra[j];
         }
-        for (int i = 0; i < 4; i++) Bs[buf][bq + i][bc] = bptr ? V::get(rb, i) : T(0);
+        if (bptr) *(v4*)&Bs[buf][bc][bq] = rb;
+        else *(v4*)&Bs[buf][bc][bq] = V::zero();
     };
     acc_t acc[NB];
 #pragma unroll
@@ -376,32 +375,33 @@ __global__ __launch_bounds__(256) void k_gemm_m2l(const T* __restrict__ ops, con
     for (int kt = 0; kt < NK; kt++) {
         const int cur = kt & 1;
         if (kt + 1 < NK) gload((kt + 1) * BK);
+        const v4 bf = *(const v4*)&Bs[cur][wave * 16 + li][4 * g];
+        v4 af[NB];
 #pragma unroll
-        for (int ks = 0; ks < BK; ks += 4) {
-            const int kk = ks + (lane >> 4);
-            const T bf = Bs[cur][kk][wave * 16 + (lane & 15)];
+        for (int a = 0; a < NB; a++) af[a] = *(const v4*)&As[cur][a * 16 + li][4 * g];
+        // consecutive MFMAs update different accumulators (no back-to-back dependence)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int a = 0; a < NB; a++) {
-                const T af = As[cur][kk][a * 16 + (lane & 15)];
                 if constexpr (sizeof(T) == 8)
-                    acc[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, acc[a], 0, 0, 0);
+                    acc[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(V::get(af[a], j), V::get(bf, j), acc[a], 0, 0, 0);
                 else
-                    acc[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(af, bf, acc[a], 0, 0, 0);
+                    acc[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(V::get(af[a], j), V::get(bf, j), acc[a], 0, 0, 0);
             }
-        }
         if (kt + 1 < NK) lstore(cur ^ 1);
         __syncthreads();
     }
-    const int cl = wave * 16 + (lane & 15);
+    const int cl = wave * 16 + li;
     if (cl >= tile.ncol) return;
     T* out = slab + (size_t)(tile.col0 + cl) * R;
 #pragma unroll
     for (int a = 0; a < NB; a++) {
         if constexpr (sizeof(T) == 8) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) out[a * 16 + 4 * r + (lane >> 4)] = acc[a][r];
+            for (int r = 0; r < 4; r++) out[a * 16 + 4 * r + g] = acc[a][r];
         } else {
-            *(float4*)(out + a * 16 + 4 * (lane >> 4)) = make_float4(acc[a][0], acc[a][1], acc[a][2], acc[a][3]);
+            *(float4*)(out + a * 16 + 4 * g) = make_float4(acc[a][0], acc[a][1], acc[a][2], acc[a][3]);
         }
     }
 }
