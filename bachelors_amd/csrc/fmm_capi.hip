@@ -86,7 +86,8 @@ struct fmm_ctx {
     bool ev_ok = false;
     // side stream of the fused evaluation (lists and P2P beside the multipole chain) and its joins
     hipStream_t side = nullptr;
-    hipEvent_t j_lists = nullptr, j_pack = nullptr, j_p2p = nullptr, j_gemm = nullptr, j_pairs = nullptr;
+    hipEvent_t j_lists = nullptr, j_pack = nullptr, j_p2p = nullptr, j_gemm = nullptr, j_pairs = nullptr,
+               j_start = nullptr;
     bool pairs_on_side = false;   // the M2L reduction waits for j_pairs (fused evaluation)
     long long near_inter = -1;
     bool lists_ready = false;
@@ -200,7 +201,11 @@ void stage_trees(fmm_ctx* c) {
         }
     }
     // the two trees are independent: the target tree is built on the side stream by a host
-    // thread (its own scratch) while this thread builds the source tree; both end synchronised
+    // thread (its own scratch) while this thread builds the source tree; both end synchronised.
+    // The side stream first waits for everything queued on the main stream (a previous
+    // evaluation may still read the target tree there).
+    FMM_HIP(hipEventRecord(c->j_start, st));
+    FMM_HIP(hipStreamWaitEvent(c->side, c->j_start, 0));
     std::exception_ptr tgt_error;
     std::thread tgt_thread([&] {
         try {
@@ -697,6 +702,7 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
         FMM_HIP(hipEventCreateWithFlags(&c->j_p2p, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_gemm, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_pairs, hipEventDisableTiming));
+        FMM_HIP(hipEventCreateWithFlags(&c->j_start, hipEventDisableTiming));
     });
     if (rc != FMM_OK) {
         delete c;
@@ -712,7 +718,7 @@ void fmm_destroy(fmm_ctx* c) {
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_ok)
         for (int i = 0; i < E_COUNT; i++) (void)hipEventDestroy(c->ev[i]);
-    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm, c->j_pairs})
+    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm, c->j_pairs, c->j_start})
         if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
