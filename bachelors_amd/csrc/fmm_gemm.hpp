@@ -28,16 +28,20 @@ void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmT
                 hipStream_t st, bool transpose = false, int nseg = 1);
 
 // Pairs grouped by the node that receives them: order[begin[v] .. end[v]) are the pair indices
-// (ascending) whose key is v; nodes without pairs have begin = end = 0.
+// (ascending) whose key is v; nodes without pairs have begin = end = 0.  Each node's range is
+// cut into segments of at most PAIR_SEGMENT pairs, numbered node by node: node v owns segments
+// [seg_off[v], seg_off[v + 1]), seg_node maps a segment to its node (max_segments bounds their
+// number on the host without a read-back); partial holds one reduced block per segment.
+constexpr int PAIR_SEGMENT = 32;
 struct PairIndex {
-    int npairs = 0, nnodes = 0;
-    DevBuf keys, keys_sorted, iota, order, begin, end, tmp;
+    int npairs = 0, nnodes = 0, max_segments = 0;
+    DevBuf keys, keys_sorted, iota, order, begin, end, tmp, seg_count, seg_off, seg_node, partial;
 };
 void build_pair_index(const int* key, int npairs, int nnodes, PairIndex& ix, hipStream_t st);
 
 // out[v][c][0 .. R) += sum over the pairs of v, in ascending pair order, of slab[pair][c][0 .. R)
 // (slab = a per-pair store with row length R <= Lp, multiple of 4); double if f64 else float
-void reduce_pairs(bool f64, const PairIndex& ix, const void* slab, int R, int Lp, void* out, hipStream_t st);
+void reduce_pairs(bool f64, PairIndex& ix, const void* slab, int R, int Lp, void* out, hipStream_t st);
 
 // Row length of the M2L slab: L padded to 16 (MFMA block) where a kernel instance exists for it
 // (FP32/FP64), else Lp (BF16X3, whose K step is 32, and orders without an instance).
