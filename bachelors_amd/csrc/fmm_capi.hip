@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <functional>
+#include <future>
 #include <string>
 #include <thread>
 #include <vector>
@@ -392,31 +394,26 @@ void downward_per_pair(fmm_ctx* c) {
 // interaction lists (and, in the shared frame, far pairs grouped by target node for the
 // atomic-free M2L accumulation) on stream ls; they depend on the trees only
 // far_ready (may be null): recorded on ls once the far pairs and classes are complete; the
-// close pairs and the target grouping follow, then E_LISTS and j_pairs
-void stage_lists(fmm_ctx* c, hipStream_t ls, hipEvent_t far_ready) {
-    require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built");
+// close pairs and the target grouping follow, then E_LISTS and j_pairs.  far_host (may be
+// empty) is called when the host-side far data is final (the M2L can be planned).
+// The caller checks that the trees are built (this may run on a thread of its own).
+void stage_lists(fmm_ctx* c, hipStream_t ls, hipEvent_t far_ready, const std::function<void()>& far_host = {}) {
     const bool shared = c->cfg.frame == FMM_FRAME_SHARED;
     c->rec_on(E_LISTS0, ls);
-    build_lists(c->lists, c->lscr, c->S, c->T, c->cfg.mac_scale, shared, ls, far_ready);
+    build_lists(c->lists, c->lscr, c->S, c->T, c->cfg.mac_scale, shared, ls, far_ready, far_host);
     if (shared) build_pair_index(c->lists.far_t.as<int>(), c->lists.num_far, c->T.num_nodes, c->m2l_ix, ls);
     c->rec_on(E_LISTS, ls);
     FMM_HIP(hipEventRecord(c->j_pairs, ls));
     c->lists_ready = true;
 }
 
-// M2L and L2L (the lists are ready on c->stream's timeline)
-void stage_translate(fmm_ctx* c) {
-    require(c->stage >= ST_UP, FMM_ERR_STATE, "Multipoles are not ready!");
-    require(c->lists_ready, FMM_ERR_STATE, "interaction lists are not built");
+// Shared frame, first half of the downward pass: operator tables and the M2L GEMM into the
+// slab (needs the far pairs and classes only; the pair index may still be in flight)
+void translate_gemm(fmm_ctx* c) {
     hipStream_t st = c->stream;
     const TreeDev& S = c->S;
     const TreeDev& T = c->T;
-    const bool shared = c->cfg.frame == FMM_FRAME_SHARED;
     c->rec(E_DOWN0);
-    if (!shared) {
-        downward_per_pair(c);
-        return;
-    }
     // operator tables: L2L per (level, orientation), then one M2L operator per class
     OpTables& O = c->down_ops;
     int nlev = T.num_levels;
@@ -465,13 +462,21 @@ void stage_translate(fmm_ctx* c) {
         void* slab = c->m2l_slab.reserve((size_t)3 * P.npairs * R * c->esize());
         m2l_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, c->lists.far_s.as<int>(), c->M.ptr,
                  slab, c->L, c->Lp, st);
-        c->rec(E_M2L_GEMM);
-        FMM_HIP(hipEventRecord(c->j_gemm, st));
+    }
+    c->rec(E_M2L_GEMM);
+    FMM_HIP(hipEventRecord(c->j_gemm, st));
+}
+
+// Second half: per-target reduction of the slab (needs the pair index), then L2L
+void translate_finish(fmm_ctx* c) {
+    hipStream_t st = c->stream;
+    const TreeDev& T = c->T;
+    OpTables& O = c->down_ops;
+    const int nlev = T.num_levels;
+    void* Lx = c->Lx.ptr;
+    if (c->m2l_pass.ntiles) {
         if (c->pairs_on_side) FMM_HIP(hipStreamWaitEvent(st, c->j_pairs, 0));
-        reduce_pairs(c->f64(), c->m2l_ix, slab, R, c->Lp, Lx, st);
-    } else {
-        c->rec(E_M2L_GEMM);
-        FMM_HIP(hipEventRecord(c->j_gemm, st));
+        reduce_pairs(c->f64(), c->m2l_ix, c->m2l_slab.ptr, m2l_slab_rows(c->prec, c->L, c->Lp), c->Lp, Lx, st);
     }
     c->rec(E_M2L);
     // L2L top-down (calculation_point_octree.cpp:277-299)
@@ -500,9 +505,22 @@ void stage_translate(fmm_ctx* c) {
     c->stage = ST_DOWN;
 }
 
+// M2L and L2L (the lists are ready on c->stream's timeline)
+void stage_translate(fmm_ctx* c) {
+    require(c->stage >= ST_UP, FMM_ERR_STATE, "Multipoles are not ready!");
+    require(c->lists_ready, FMM_ERR_STATE, "interaction lists are not built");
+    if (c->cfg.frame != FMM_FRAME_SHARED) {
+        c->rec(E_DOWN0);
+        downward_per_pair(c);
+        return;
+    }
+    translate_gemm(c);
+    translate_finish(c);
+}
+
 void stage_downward(fmm_ctx* c) {
     require(c->stage >= ST_UP, FMM_ERR_STATE, "Multipoles are not ready!");
-    stage_lists(c, c->stream, nullptr);
+    stage_lists(c, c->stream, nullptr);   // ST_UP implies the trees
     stage_translate(c);
 }
 
@@ -571,26 +589,41 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     stage_trees(c);
     stage_leaves(c);
     FMM_HIP(hipEventRecord(c->j_pack, A));
+    // the lists thread signals far_host once the far pairs and classes are final on the host; it
+    // goes on with the close pairs and the pair index while this thread queues the M2L GEMM
     std::exception_ptr lists_error;
+    std::promise<void> far_host;
+    std::future<void> far_known = far_host.get_future();
+    bool far_signalled = false;   // touched by the lists thread only
     std::thread lists_thread([&] {
         try {
-            stage_lists(c, Bs, c->j_lists);   // the trees are complete (stage_trees reads them back)
+            stage_lists(c, Bs, c->j_lists, [&] {   // the trees are complete (stage_trees reads them back)
+                far_signalled = true;
+                far_host.set_value();
+            });
         } catch (...) {
             lists_error = std::current_exception();
+            if (!far_signalled) far_host.set_exception(lists_error);
         }
     });
+    c->pairs_on_side = true;   // the reduction waits for the pair index (j_pairs)
     try {
         stage_upward(c);
+        far_known.get();
+        FMM_HIP(hipStreamWaitEvent(A, c->j_lists, 0));   // far pairs and classes
+        translate_gemm(c);
     } catch (...) {
+        c->pairs_on_side = false;
         lists_thread.join();
         throw;
     }
     lists_thread.join();
-    if (lists_error) std::rethrow_exception(lists_error);
-    FMM_HIP(hipStreamWaitEvent(A, c->j_lists, 0));   // far pairs and classes
-    c->pairs_on_side = true;                         // the reduction waits for the target grouping
+    if (lists_error) {
+        c->pairs_on_side = false;
+        std::rethrow_exception(lists_error);
+    }
     try {
-        stage_translate(c);
+        translate_finish(c);
     } catch (...) {
         c->pairs_on_side = false;
         throw;

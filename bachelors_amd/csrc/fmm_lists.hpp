@@ -1,4 +1,6 @@
 #pragma once
+#include <functional>
+
 #include "fmm_common.hpp"
 
 namespace fmm {
@@ -23,8 +25,9 @@ struct Lists {
 };
 
 // far_ready (may be null) is recorded on st once the far pairs and classes are complete, before
-// the close pairs are sorted
+// the close pairs are sorted; far_host (may be empty) is called right after, when the host-side
+// far data (num_far, num_classes, h_class_begin) is final
 void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T, double mac_scale, bool classes,
-                 hipStream_t st, hipEvent_t far_ready = nullptr);
+                 hipStream_t st, hipEvent_t far_ready = nullptr, const std::function<void()>& far_host = {});
 
 }  // namespace fmm
