@@ -36,9 +36,59 @@ constexpr int M2M_PARENT_MAJOR_COLUMNS = 64 * 256;
 enum Ev { E_START, E_TREE, E_P2M, E_M2M, E_LISTS, E_TABLES, E_M2L, E_L2L, E_L2P, E_P2P, E_END, E_M2L_GEMM,
           E_LISTS0, E_DOWN0, E_P2P0, E_COUNT };
 
-struct Pass {               // one batched GEMM launch: tiles over (src, dst) pairs
+// Launch data built on the host for each evaluation (pair lists, tiles, operator specs) is
+// appended to a pinned buffer and sent with one copy per stage: per-level copies from pageable
+// memory each cost a blit launch on the stream.  begin() waits for the previous copy out of
+// the host buffer before it is overwritten.
+struct Staging {
+    char* h = nullptr;
+    size_t cap = 0, used = 0;
+    DevBuf d;
+    hipEvent_t copied = nullptr;
+    Staging() = default;
+    Staging(const Staging&) = delete;
+    Staging& operator=(const Staging&) = delete;
+    ~Staging() {
+        if (h) (void)hipHostFree(h);
+        if (copied) (void)hipEventDestroy(copied);
+    }
+    void begin() {
+        if (copied) FMM_HIP(hipEventSynchronize(copied));
+        used = 0;
+    }
+    // byte offset of a copy of bytes at p (16-byte aligned)
+    size_t put(const void* p, size_t bytes) {
+        size_t off = (used + 15) & ~(size_t)15;
+        if (off + bytes > cap) {
+            size_t nc = std::max(off + bytes, 2 * cap + (1 << 16));
+            char* nh = nullptr;
+            FMM_HIP(hipHostMalloc((void**)&nh, nc, hipHostMallocDefault));
+            if (used) std::memcpy(nh, h, used);
+            if (h) FMM_HIP(hipHostFree(h));
+            h = nh;
+            cap = nc;
+        }
+        if (bytes) std::memcpy(h + off, p, bytes);
+        used = off + bytes;
+        return off;
+    }
+    template <class T> size_t put(const std::vector<T>& v) { return put(v.data(), v.size() * sizeof(T)); }
+    // one copy of everything put since begin(); returns the device base of the offsets
+    char* send(hipStream_t st) {
+        char* dev = (char*)d.reserve(std::max<size_t>(used, 16));
+        if (used) FMM_HIP(hipMemcpyAsync(dev, h, used, hipMemcpyHostToDevice, st));
+        if (!copied) FMM_HIP(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
+        FMM_HIP(hipEventRecord(copied, st));
+        return dev;
+    }
+};
+
+struct Pass {               // one batched GEMM launch: tiles over (src, dst) pairs (staged data)
     int ntiles = 0, npairs = 0;
-    DevBuf tiles, src, dst;
+    size_t o_src = 0, o_dst = 0, o_tiles = 0;
+    const int* src = nullptr;
+    const int* dst = nullptr;
+    const GemmTile* tiles = nullptr;
 };
 
 struct OpTables {           // FP64 master + the precision actually applied
@@ -74,6 +124,10 @@ struct fmm_ctx {
     OpTables up_ops, down_ops;
     std::vector<Pass> m2m_pass, l2l_pass, m2m_adj_pass;   // m2m_adj_pass: per orientation, for M2M^T
     Pass m2l_pass;
+    // their launch data; operator specs go first in a copy of their own, so the table kernels
+    // run while the host plans the passes
+    Staging up_specs, up_stage, down_stage, m2l_specs, m2l_stage, adj_stage;
+    DevBuf pp_tiles;                             // per-pair path: tiles of one chunk
     Lists lists;
     ListScratch lscr;
     PairIndex m2l_ix;                            // far pairs by target node
@@ -129,17 +183,20 @@ bool nonempty_host(const TreeDev& T, int n) { return T.h_first_child[n] >= 0 || 
 const int SIGN[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                         {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
 
-void upload_pass(Pass& P, const std::vector<int>& src, const std::vector<int>& dst, const std::vector<GemmTile>& tiles,
-                 hipStream_t st) {
+void stage_pass(Staging& sg, Pass& P, const std::vector<int>& src, const std::vector<int>& dst,
+                const std::vector<GemmTile>& tiles) {
     P.npairs = (int)src.size();
     P.ntiles = (int)tiles.size();
-    if (P.npairs) {
-        FMM_HIP(hipMemcpyAsync(P.src.get<int>(P.npairs), src.data(), P.npairs * sizeof(int), hipMemcpyHostToDevice, st));
-        FMM_HIP(hipMemcpyAsync(P.dst.get<int>(P.npairs), dst.data(), P.npairs * sizeof(int), hipMemcpyHostToDevice, st));
-    }
-    if (P.ntiles)
-        FMM_HIP(hipMemcpyAsync(P.tiles.get<GemmTile>(P.ntiles), tiles.data(), P.ntiles * sizeof(GemmTile),
-                               hipMemcpyHostToDevice, st));
+    P.o_src = sg.put(src);
+    P.o_dst = sg.put(dst);
+    P.o_tiles = sg.put(tiles);
+    P.src = P.dst = nullptr;
+    P.tiles = nullptr;
+}
+void resolve_pass(Pass& P, const char* dev) {
+    P.src = (const int*)(dev + P.o_src);
+    P.dst = (const int*)(dev + P.o_dst);
+    P.tiles = (const GemmTile*)(dev + P.o_tiles);
 }
 
 // O.nops operators from device specs, written in the precision the GEMMs apply
@@ -162,8 +219,8 @@ const void* ops_ptr(const fmm_ctx* c, const OpTables& O) {
 
 void run_pass(fmm_ctx* c, const OpTables& O, const Pass& P, const void* src_exp, void* dst_exp, int epilogue) {
     if (!P.ntiles) return;
-    class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, P.src.as<int>(), P.dst.as<int>(),
-               src_exp, dst_exp, c->Lp, epilogue, c->stream);
+    class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, P.src, P.dst, src_exp, dst_exp, c->Lp, epilogue,
+               c->stream);
 }
 
 void stage_trees(fmm_ctx* c) {
@@ -286,16 +343,20 @@ void stage_upward(fmm_ctx* c) {
             s.kind = OPK_M2M;
             s.m2l_full = 0;
         }
-    if (O.nops) {
-        OpSpec* d = O.specs.get<OpSpec>(O.nops);
-        FMM_HIP(hipMemcpyAsync(d, specs.data(), O.nops * sizeof(OpSpec), hipMemcpyHostToDevice, c->stream));
-        build_tables(c, O, d, c->stream);
-    }
     // per level: many parents -> parent-major tiles, each parent column summing its 8 children
     // (one K segment per orientation operator; empty children hold zero multipoles), written
     // once without atomics; few parents -> the 8 orientations as parallel classes of
     // (child, parent) pairs with atomic accumulation (a long serial K loop would dominate)
-    c->m2m_pass.resize(MAX_LEVELS);
+    if (O.nops) {
+        Staging& sp = c->up_specs;
+        sp.begin();
+        const size_t o = sp.put(specs);
+        build_tables(c, O, (const OpSpec*)(sp.send(c->stream) + o), c->stream);
+    }
+    Staging& sg = c->up_stage;
+    sg.begin();
+    c->m2m_pass.assign(MAX_LEVELS, Pass());
+    std::vector<char> parent_major(MAX_LEVELS, 0);
     for (int k = nlev - 1; k >= 1; k--) {
         std::vector<int> first, parent;
         for (int n = S.level_begin[k - 1]; n < S.level_begin[k]; n++) {
@@ -307,10 +368,8 @@ void stage_upward(fmm_ctx* c) {
         if (3 * first.size() >= (size_t)M2M_PARENT_MAJOR_COLUMNS) {
             std::vector<int> cb = {0, (int)first.size()}, cop = {(k - 1) * 8};
             plan_tiles(cb, cop, tiles);
-            upload_pass(P, first, parent, tiles, c->stream);
-            if (P.ntiles)
-                class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, P.src.as<int>(),
-                           P.dst.as<int>(), c->M.ptr, c->M.ptr, c->Lp, EPI_ADD, c->stream, false, 8);
+            stage_pass(sg, P, first, parent, tiles);
+            parent_major[k] = 1;
         } else {
             std::vector<int> all_s, all_d, cb(1, 0), cop;
             for (int o = 0; o < 8; o++) {
@@ -320,9 +379,19 @@ void stage_upward(fmm_ctx* c) {
                 cop.push_back((k - 1) * 8 + o);
             }
             plan_tiles(cb, cop, tiles);
-            upload_pass(P, all_s, all_d, tiles, c->stream);
-            run_pass(c, O, P, c->M.ptr, c->M.ptr, EPI_ATOMIC);
+            stage_pass(sg, P, all_s, all_d, tiles);
         }
+    }
+    const char* dev = sg.send(c->stream);
+    for (int k = nlev - 1; k >= 1; k--) {
+        Pass& P = c->m2m_pass[k];
+        resolve_pass(P, dev);
+        if (!P.ntiles) continue;
+        if (parent_major[k])
+            class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, P.src, P.dst, c->M.ptr, c->M.ptr, c->Lp,
+                       EPI_ADD, c->stream, false, 8);
+        else
+            run_pass(c, O, P, c->M.ptr, c->M.ptr, EPI_ATOMIC);
     }
     c->rec(E_M2M);
     c->stage = ST_UP;
@@ -349,7 +418,7 @@ void run_per_pair(fmm_ctx* c, int kind, const int* d_src, const int* d_dst, int 
         build_tables(c, O, sp, st);
         tiles.resize(n);
         for (int i = 0; i < n; i++) tiles[i] = GemmTile{i, 3 * (b + i), 3};
-        GemmTile* dt = c->m2l_pass.tiles.get<GemmTile>(n);
+        GemmTile* dt = c->pp_tiles.get<GemmTile>(n);
         FMM_HIP(hipMemcpyAsync(dt, tiles.data(), n * sizeof(GemmTile), hipMemcpyHostToDevice, st));
         class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, dt, n, d_src, d_dst, src_exp, dst_exp, c->Lp,
                    atomic ? EPI_ATOMIC : EPI_ADD, st);
@@ -369,7 +438,9 @@ void downward_per_pair(fmm_ctx* c) {
                  c->sigT, c->M.ptr, Lx, true);
     c->rec(E_M2L);
     // L2L with each child's exact translation (calculation_point_octree.cpp:277-299)
-    c->l2l_pass.resize(MAX_LEVELS);
+    Staging& sg = c->down_stage;
+    sg.begin();
+    c->l2l_pass.assign(MAX_LEVELS, Pass());
     for (int k = 1; k < T.num_levels; k++) {
         std::vector<int> src, dst;
         for (int nd = T.level_begin[k - 1]; nd < T.level_begin[k]; nd++) {
@@ -378,12 +449,15 @@ void downward_per_pair(fmm_ctx* c) {
             for (int o = 0; o < 8; o++)
                 if (nonempty_host(T, fc + o)) { src.push_back(nd); dst.push_back(fc + o); }
         }
-        if (src.empty()) continue;
+        stage_pass(sg, c->l2l_pass[k], src, dst, std::vector<GemmTile>());
+    }
+    const char* dev = sg.send(st);
+    for (int k = 1; k < T.num_levels; k++) {
         Pass& P = c->l2l_pass[k];
-        upload_pass(P, src, dst, std::vector<GemmTile>(), st);
+        resolve_pass(P, dev);
         P.ntiles = 0;   // per-pair operators: this pass has no shared tiles (no adjoint)
-        run_per_pair(c, OPK_L2L, P.src.as<int>(), P.dst.as<int>(), (int)src.size(), T, T, c->sigT, c->sigT, Lx, Lx,
-                     false);
+        if (P.npairs)
+            run_per_pair(c, OPK_L2L, P.src, P.dst, P.npairs, T, T, c->sigT, c->sigT, Lx, Lx, false);
     }
     c->down_ops.nops = 0;
     c->m2l_pass.ntiles = 0;
@@ -420,8 +494,7 @@ void translate_gemm(fmm_ctx* c) {
     int n_l2l = 8 * std::max(nlev - 1, 0);
     int nc = c->lists.num_classes;
     O.nops = n_l2l + nc;
-    OpSpec* d = O.specs.get<OpSpec>(std::max(O.nops, 1));
-    std::vector<OpSpec> specs(n_l2l);
+    std::vector<OpSpec> specs(O.nops);   // the M2L specs are written on the device
     for (int k = 1; k < nlev; k++)
         for (int o = 0; o < 8; o++) {
             OpSpec& s = specs[(k - 1) * 8 + o];
@@ -431,12 +504,29 @@ void translate_gemm(fmm_ctx* c) {
             s.kind = OPK_L2L;
             s.m2l_full = 0;
         }
-    if (n_l2l) FMM_HIP(hipMemcpyAsync(d, specs.data(), n_l2l * sizeof(OpSpec), hipMemcpyHostToDevice, st));
+    Staging& sp = c->m2l_specs;
+    sp.begin();
+    const size_t o_specs = sp.put(specs);
+    OpSpec* d = (OpSpec*)(sp.send(st) + o_specs);
     make_m2l_specs(c->lists.class_begin.as<int>(), nc, c->lists.far_t.as<int>(), c->lists.far_s.as<int>(),
                    S.center.as<double>(), T.center.as<double>(), S.level.as<int>(), T.level.as<int>(), c->sigS, c->sigT,
                    c->cfg.m2l_full, d + n_l2l, st);
     if (O.nops) {
         build_tables(c, O, d, st);
+    }
+    // the M2L tiles (planned while the tables are built)
+    Pass& P = c->m2l_pass;
+    {
+        std::vector<int> cop(nc);
+        for (int i = 0; i < nc; i++) cop[i] = n_l2l + i;
+        std::vector<GemmTile> tiles;
+        plan_tiles(c->lists.h_class_begin, cop, tiles);
+        Staging& sg = c->m2l_stage;
+        sg.begin();
+        const size_t o_tiles = sg.put(tiles);
+        P.npairs = c->lists.num_far;
+        P.ntiles = (int)tiles.size();
+        P.tiles = (const GemmTile*)(sg.send(st) + o_tiles);
     }
     size_t n = (size_t)T.num_nodes * 3 * c->Lp;
     void* Lx = c->Lx.reserve(n * c->esize());
@@ -444,24 +534,12 @@ void translate_gemm(fmm_ctx* c) {
     // M2L: all classes in one launch (fast_multipole_solver.cpp:258-276) writing one slab per
     // (pair, component), then each target node sums its pairs' slabs in pair order (no atomics,
     // deterministic); E_TABLES .. E_M2L_GEMM brackets the GEMM alone
-    Pass& P = c->m2l_pass;
-    {
-        std::vector<int> cop(nc);
-        for (int i = 0; i < nc; i++) cop[i] = n_l2l + i;
-        std::vector<GemmTile> tiles;
-        plan_tiles(c->lists.h_class_begin, cop, tiles);
-        P.npairs = c->lists.num_far;
-        P.ntiles = (int)tiles.size();
-        if (P.ntiles)
-            FMM_HIP(hipMemcpyAsync(P.tiles.get<GemmTile>(P.ntiles), tiles.data(), P.ntiles * sizeof(GemmTile),
-                                   hipMemcpyHostToDevice, st));
-    }
     c->rec(E_TABLES);
     if (P.ntiles) {
         const int R = m2l_slab_rows(c->prec, c->L, c->Lp);
         void* slab = c->m2l_slab.reserve((size_t)3 * P.npairs * R * c->esize());
-        m2l_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles.as<GemmTile>(), P.ntiles, c->lists.far_s.as<int>(), c->M.ptr,
-                 slab, c->L, c->Lp, st);
+        m2l_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, c->lists.far_s.as<int>(), c->M.ptr, slab, c->L,
+                 c->Lp, st);
     }
     c->rec(E_M2L_GEMM);
     FMM_HIP(hipEventRecord(c->j_gemm, st));
@@ -480,7 +558,9 @@ void translate_finish(fmm_ctx* c) {
     }
     c->rec(E_M2L);
     // L2L top-down (calculation_point_octree.cpp:277-299)
-    c->l2l_pass.resize(MAX_LEVELS);
+    Staging& sg = c->down_stage;
+    sg.begin();
+    c->l2l_pass.assign(MAX_LEVELS, Pass());
     for (int k = 1; k < nlev; k++) {
         std::vector<int> src[8], dst[8];
         for (int nd = T.level_begin[k - 1]; nd < T.level_begin[k]; nd++) {
@@ -498,7 +578,11 @@ void translate_finish(fmm_ctx* c) {
         }
         std::vector<GemmTile> tiles;
         plan_tiles(cb, cop, tiles);
-        upload_pass(c->l2l_pass[k], all_s, all_d, tiles, st);
+        stage_pass(sg, c->l2l_pass[k], all_s, all_d, tiles);
+    }
+    const char* dev = sg.send(st);
+    for (int k = 1; k < nlev; k++) {
+        resolve_pass(c->l2l_pass[k], dev);
         run_pass(c, O, c->l2l_pass[k], Lx, Lx, EPI_ADD);
     }
     c->rec(E_L2L);
@@ -869,7 +953,9 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
         a.down_ops_lo = c->down_ops.lo.ptr;
         // M2M^T needs (child, parent) pairs per orientation (the forward pass is parent-major)
         const TreeDev& S = c->S;
-        c->m2m_adj_pass.resize(MAX_LEVELS);
+        Staging& sg = c->adj_stage;
+        sg.begin();
+        c->m2m_adj_pass.assign(MAX_LEVELS, Pass());
         for (int k = S.num_levels - 1; k >= 1; k--) {
             std::vector<int> src[8], dst[8];
             for (int n = S.level_begin[k - 1]; n < S.level_begin[k]; n++) {
@@ -887,17 +973,17 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
             }
             std::vector<GemmTile> tiles;
             plan_tiles(cb, cop, tiles);
-            upload_pass(c->m2m_adj_pass[k], all_s, all_d, tiles, c->stream);
+            stage_pass(sg, c->m2m_adj_pass[k], all_s, all_d, tiles);
         }
+        const char* dev = sg.send(c->stream);
+        for (int k = S.num_levels - 1; k >= 1; k--) resolve_pass(c->m2m_adj_pass[k], dev);
         for (int k = 0; k < MAX_LEVELS; k++) {
             const Pass* pm = k < (int)c->m2m_adj_pass.size() ? &c->m2m_adj_pass[k] : nullptr;
             const Pass* pl = k < (int)c->l2l_pass.size() ? &c->l2l_pass[k] : nullptr;
-            a.m2m_levels.push_back(pm ? PassView{pm->ntiles, pm->tiles.as<GemmTile>(), pm->src.as<int>(), pm->dst.as<int>()}
-                                      : PassView{});
-            a.l2l_levels.push_back(pl ? PassView{pl->ntiles, pl->tiles.as<GemmTile>(), pl->src.as<int>(), pl->dst.as<int>()}
-                                      : PassView{});
+            a.m2m_levels.push_back(pm ? PassView{pm->ntiles, pm->tiles, pm->src, pm->dst} : PassView{});
+            a.l2l_levels.push_back(pl ? PassView{pl->ntiles, pl->tiles, pl->src, pl->dst} : PassView{});
         }
-        a.m2l = PassView{c->m2l_pass.ntiles, c->m2l_pass.tiles.as<GemmTile>(), c->lists.far_t.as<int>(),
+        a.m2l = PassView{c->m2l_pass.ntiles, c->m2l_pass.tiles, c->lists.far_t.as<int>(),
                          c->lists.far_s.as<int>()};
         a.n_up_ops = c->up_ops.nops;
         a.n_down_ops = c->down_ops.nops;
