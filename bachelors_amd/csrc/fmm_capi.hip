@@ -9,7 +9,6 @@
 #include <cstring>
 #include <exception>
 #include <functional>
-#include <future>
 #include <string>
 #include <thread>
 #include <vector>
@@ -34,7 +33,7 @@ constexpr int M2M_PARENT_MAJOR_COLUMNS = 64 * 256;
 // stage events: the stream-A chain, plus start marks of the stages that may run on the side
 // stream (lists, P2P) and of the translations after the lists (E_DOWN0)
 enum Ev { E_START, E_TREE, E_P2M, E_M2M, E_LISTS, E_TABLES, E_M2L, E_L2L, E_L2P, E_P2P, E_END, E_M2L_GEMM,
-          E_LISTS0, E_DOWN0, E_P2P0, E_COUNT };
+          E_LISTS0, E_DOWN0, E_P2P0, E_GEMM0, E_COUNT };
 
 // Launch data built on the host for each evaluation (pair lists, tiles, operator specs) is
 // appended to a pinned buffer and sent with one copy per stage: per-level copies from pageable
@@ -142,9 +141,8 @@ struct fmm_ctx {
     bool ev_ok = false;
     // side stream of the fused evaluation (lists and P2P beside the multipole chain) and its joins
     hipStream_t side = nullptr;
-    hipEvent_t j_lists = nullptr, j_pack = nullptr, j_p2p = nullptr, j_gemm = nullptr, j_pairs = nullptr,
-               j_start = nullptr;
-    bool pairs_on_side = false;   // the M2L reduction waits for j_pairs (fused evaluation)
+    hipEvent_t j_lists = nullptr, j_pack = nullptr, j_p2p = nullptr, j_gemm = nullptr,
+               j_start = nullptr, j_tables = nullptr;   // j_tables: everything the side stream builds for M2L
     long long near_inter = -1;
     bool lists_ready = false;
 
@@ -467,27 +465,26 @@ void downward_per_pair(fmm_ctx* c) {
 
 // interaction lists (and, in the shared frame, far pairs grouped by target node for the
 // atomic-free M2L accumulation) on stream ls; they depend on the trees only
-// far_ready (may be null): recorded on ls once the far pairs and classes are complete; the
-// close pairs and the target grouping follow, then E_LISTS and j_pairs.  far_host (may be
-// empty) is called when the host-side far data is final (the M2L can be planned).
-// The caller checks that the trees are built (this may run on a thread of its own).
-void stage_lists(fmm_ctx* c, hipStream_t ls, hipEvent_t far_ready, const std::function<void()>& far_host = {}) {
+// far_ready (may be null): recorded on ls once the far pairs and classes are complete; then
+// after_far (may be empty) is called, and the close pairs and the target grouping follow, then
+// E_LISTS.  The caller checks that the trees are built (this may run on a thread of its own).
+void stage_lists(fmm_ctx* c, hipStream_t ls, hipEvent_t far_ready, const std::function<void()>& after_far = {}) {
     const bool shared = c->cfg.frame == FMM_FRAME_SHARED;
     c->rec_on(E_LISTS0, ls);
-    build_lists(c->lists, c->lscr, c->S, c->T, c->cfg.mac_scale, shared, ls, far_ready, far_host);
+    build_lists(c->lists, c->lscr, c->S, c->T, c->cfg.mac_scale, shared, ls, far_ready, after_far);
     if (shared) build_pair_index(c->lists.far_t.as<int>(), c->lists.num_far, c->T.num_nodes, c->m2l_ix, ls);
     c->rec_on(E_LISTS, ls);
-    FMM_HIP(hipEventRecord(c->j_pairs, ls));
     c->lists_ready = true;
 }
 
-// Shared frame, first half of the downward pass: operator tables and the M2L GEMM into the
-// slab (needs the far pairs and classes only; the pair index may still be in flight)
-void translate_gemm(fmm_ctx* c) {
-    hipStream_t st = c->stream;
+// Shared frame, downward pass part 1 on stream st: operator tables (L2L per level and
+// orientation, one M2L operator per class), the M2L tiles and the cleared local expansions.
+// Needs the far pairs and classes only (not the multipoles), so the fused evaluation runs it on
+// the lists' stream beside P2M and M2M.
+void translate_tables(fmm_ctx* c, hipStream_t st) {
     const TreeDev& S = c->S;
     const TreeDev& T = c->T;
-    c->rec(E_DOWN0);
+    c->rec_on(E_DOWN0, st);
     // operator tables: L2L per (level, orientation), then one M2L operator per class
     OpTables& O = c->down_ops;
     int nlev = T.num_levels;
@@ -531,10 +528,18 @@ void translate_gemm(fmm_ctx* c) {
     size_t n = (size_t)T.num_nodes * 3 * c->Lp;
     void* Lx = c->Lx.reserve(n * c->esize());
     FMM_HIP(hipMemsetAsync(Lx, 0, n * c->esize(), st));
-    // M2L: all classes in one launch (fast_multipole_solver.cpp:258-276) writing one slab per
-    // (pair, component), then each target node sums its pairs' slabs in pair order (no atomics,
-    // deterministic); E_TABLES .. E_M2L_GEMM brackets the GEMM alone
-    c->rec(E_TABLES);
+    c->rec_on(E_TABLES, st);
+}
+
+// Part 2 (stream A, after the tables and the multipoles): the M2L of all classes in one launch
+// (fast_multipole_solver.cpp:258-276) writing one slab per (pair, component); each target node
+// then sums its pairs' slabs in pair order (part 3: no atomics, deterministic).  E_GEMM0 ..
+// E_M2L_GEMM brackets the GEMM alone.
+void translate_gemm(fmm_ctx* c) {
+    hipStream_t st = c->stream;
+    const OpTables& O = c->down_ops;
+    const Pass& P = c->m2l_pass;
+    c->rec(E_GEMM0);
     if (P.ntiles) {
         const int R = m2l_slab_rows(c->prec, c->L, c->Lp);
         void* slab = c->m2l_slab.reserve((size_t)3 * P.npairs * R * c->esize());
@@ -545,7 +550,7 @@ void translate_gemm(fmm_ctx* c) {
     FMM_HIP(hipEventRecord(c->j_gemm, st));
 }
 
-// Second half: per-target reduction of the slab (needs the pair index), then L2L
+// Part 3: per-target reduction of the slab (needs the pair index), then L2L
 void translate_finish(fmm_ctx* c) {
     hipStream_t st = c->stream;
     const TreeDev& T = c->T;
@@ -553,7 +558,6 @@ void translate_finish(fmm_ctx* c) {
     const int nlev = T.num_levels;
     void* Lx = c->Lx.ptr;
     if (c->m2l_pass.ntiles) {
-        if (c->pairs_on_side) FMM_HIP(hipStreamWaitEvent(st, c->j_pairs, 0));
         reduce_pairs(c->f64(), c->m2l_ix, c->m2l_slab.ptr, m2l_slab_rows(c->prec, c->L, c->Lp), c->Lp, Lx, st);
     }
     c->rec(E_M2L);
@@ -598,6 +602,7 @@ void stage_translate(fmm_ctx* c) {
         downward_per_pair(c);
         return;
     }
+    translate_tables(c, c->stream);
     translate_gemm(c);
     translate_finish(c);
 }
@@ -673,46 +678,34 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     stage_trees(c);
     stage_leaves(c);
     FMM_HIP(hipEventRecord(c->j_pack, A));
-    // the lists thread signals far_host once the far pairs and classes are final on the host; it
-    // goes on with the close pairs and the pair index while this thread queues the M2L GEMM
+    // the lists thread queues the far lists, the operator tables, then the close lists and the
+    // pair index on the side stream (beside P2M and M2M); it only blocks on the list counts it
+    // reads back.  The M2L GEMM waits for the tables (j_tables), the reduction for the pair index
+    // (j_lists, re-recorded at the end).
     std::exception_ptr lists_error;
-    std::promise<void> far_host;
-    std::future<void> far_known = far_host.get_future();
-    bool far_signalled = false;   // touched by the lists thread only
     std::thread lists_thread([&] {
         try {
-            stage_lists(c, Bs, c->j_lists, [&] {   // the trees are complete (stage_trees reads them back)
-                far_signalled = true;
-                far_host.set_value();
+            stage_lists(c, Bs, nullptr, [&] {   // the trees are complete (stage_trees reads them back)
+                translate_tables(c, Bs);
+                FMM_HIP(hipEventRecord(c->j_tables, Bs));
             });
+            FMM_HIP(hipEventRecord(c->j_lists, Bs));
         } catch (...) {
             lists_error = std::current_exception();
-            if (!far_signalled) far_host.set_exception(lists_error);
         }
     });
-    c->pairs_on_side = true;   // the reduction waits for the pair index (j_pairs)
     try {
         stage_upward(c);
-        far_known.get();
-        FMM_HIP(hipStreamWaitEvent(A, c->j_lists, 0));   // far pairs and classes
-        translate_gemm(c);
     } catch (...) {
-        c->pairs_on_side = false;
         lists_thread.join();
         throw;
     }
     lists_thread.join();
-    if (lists_error) {
-        c->pairs_on_side = false;
-        std::rethrow_exception(lists_error);
-    }
-    try {
-        translate_finish(c);
-    } catch (...) {
-        c->pairs_on_side = false;
-        throw;
-    }
-    c->pairs_on_side = false;
+    if (lists_error) std::rethrow_exception(lists_error);
+    FMM_HIP(hipStreamWaitEvent(A, c->j_tables, 0));   // far lists, tables, tiles
+    translate_gemm(c);
+    FMM_HIP(hipStreamWaitEvent(A, c->j_lists, 0));    // close lists, pair index
+    translate_finish(c);
     FMM_HIP(hipStreamWaitEvent(Bs, c->j_pack, 0));
     FMM_HIP(hipStreamWaitEvent(Bs, c->j_gemm, 0));
     stage_p2p(c, Bs);
@@ -818,8 +811,8 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
         FMM_HIP(hipEventCreateWithFlags(&c->j_pack, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_p2p, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_gemm, hipEventDisableTiming));
-        FMM_HIP(hipEventCreateWithFlags(&c->j_pairs, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_start, hipEventDisableTiming));
+        FMM_HIP(hipEventCreateWithFlags(&c->j_tables, hipEventDisableTiming));
     });
     if (rc != FMM_OK) {
         delete c;
@@ -835,7 +828,7 @@ void fmm_destroy(fmm_ctx* c) {
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_ok)
         for (int i = 0; i < E_COUNT; i++) (void)hipEventDestroy(c->ev[i]);
-    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm, c->j_pairs, c->j_start})
+    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm, c->j_start, c->j_tables})
         if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -1048,12 +1041,12 @@ int fmm_get_stats(fmm_ctx* c, fmm_stats* s) {
             s->t_m2m = el(E_P2M, E_M2M);
             s->t_lists = el(E_LISTS0, E_LISTS);
             s->t_tables = el(E_DOWN0, E_TABLES);
-            s->t_m2l = el(E_TABLES, E_M2L);
+            s->t_m2l = c->cfg.frame == FMM_FRAME_SHARED ? el(E_GEMM0, E_M2L) : el(E_TABLES, E_M2L);
             s->t_l2l = el(E_M2L, E_L2L);
             s->t_l2p = el(E_L2L, E_L2P);
             s->t_p2p = el(E_P2P0, E_P2P);
             s->t_total = el(E_START, E_END);
-            s->t_m2l_gemm = c->cfg.frame == FMM_FRAME_SHARED ? el(E_TABLES, E_M2L_GEMM) : s->t_m2l;
+            s->t_m2l_gemm = c->cfg.frame == FMM_FRAME_SHARED ? el(E_GEMM0, E_M2L_GEMM) : s->t_m2l;
         }
     });
 }

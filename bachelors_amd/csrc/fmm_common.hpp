@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -67,6 +68,32 @@ struct DevBuf {
     template <class T> T* get(size_t n) { return (T*)reserve(n * sizeof(T)); }
 };
 
+// Grow-only pinned host array (the host mirrors of device arrays): device-to-host copies into
+// it are plain DMA, without the staging of pageable memory.  resize() keeps no contents.
+template <class T> struct PinnedVec {
+    T* p = nullptr;
+    size_t n = 0, cap = 0;
+    PinnedVec() = default;
+    PinnedVec(const PinnedVec&) = delete;
+    PinnedVec& operator=(const PinnedVec&) = delete;
+    ~PinnedVec() { if (p) (void)hipHostFree(p); }
+    void resize(size_t m) {
+        if (m > cap) {
+            if (p) FMM_HIP(hipHostFree(p));
+            p = nullptr;
+            FMM_HIP(hipHostMalloc((void**)&p, std::max<size_t>(m, 1) * sizeof(T), hipHostMallocDefault));
+            cap = m;
+        }
+        n = m;
+    }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+    T* data() { return p; }
+    const T* data() const { return p; }
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+};
+
 struct ScaleTable {
     double s[MAX_LEVELS];
 };
@@ -88,8 +115,8 @@ struct TreeDev {
     DevBuf center;                     // double3 per node
     DevBuf parent, first_child, arrivals, npts, begin, end, level;  // int per node
     DevBuf slots;                      // int per slot: input point index
-    std::vector<int> h_first_child, h_npts, h_begin, h_end, h_level, h_parent;
-    std::vector<double> h_center;
+    PinnedVec<int> h_first_child, h_npts, h_begin, h_end, h_level, h_parent;
+    PinnedVec<double> h_center;
     int dropped = 0;                   // input points outside the root
     bool sorted_build = false;         // built by the one-sort path (else level by level)
 };

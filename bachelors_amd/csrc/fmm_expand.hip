@@ -9,6 +9,7 @@
 // FP64 on the scaled vector u = (x - c) / sigma; only P2P runs in FP32 (with FP64
 // accumulation per target).
 #include "fmm_expand.hpp"
+#include "fmm_recip.hpp"
 
 namespace fmm {
 
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, cons
             double q0 = sq[3 * j], q1 = sq[3 * j + 1], q2 = sq[3 * j + 2];
             double dr = 1.0, di = 0.0;   // Phi_m^m = (x + iy)^m / (2^m m!)
             for (int i = 1; i <= m; i++) {
-                double s = 0.5 / i;
+                double s = recip(2 * i);
                 double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
                 dr = nr;
                 di = ni;
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, cons
                 if (l < m) continue;
                 double vr = dr, vi = di;
                 if (l > m) {   // Phi_l^m = ((2l-1) z Phi_{l-1}^m - r^2 Phi_{l-2}^m) / (l^2 - m^2)
-                    double f = 1.0 / (double)(l * l - m * m);
+                    double f = recip(l * l - m * m);
                     vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
                     vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
                     p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
             double r2 = ux * ux + uy * uy + uz * uz;
             double dr = 1.0, di = 0.0;
             for (int i = 1; i <= m; i++) {
-                double s = 0.5 / i;
+                double s = recip(2 * i);
                 double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
                 dr = nr;
                 di = ni;
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
             for (int l = m; l < p; l++) {
                 double vr = dr, vi = di;
                 if (l > m) {
-                    double f = 1.0 / (double)(l * l - m * m);
+                    double f = recip(l * l - m * m);
                     vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
                     vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
                     p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;

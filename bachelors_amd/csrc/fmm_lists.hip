@@ -265,7 +265,7 @@ static void sort_unpack(ListScratch& L, int n, int end_bit, int target_high, int
 }
 
 void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T, double mac_scale, bool classes,
-                 hipStream_t st, hipEvent_t far_ready, const std::function<void()>& far_host) {
+                 hipStream_t st, hipEvent_t far_ready, const std::function<void()>& after_far) {
     RadiusTable rs = radii(S), rt = radii(T);
     NodeRec* srec = L.srec.get<NodeRec>(std::max(S.num_nodes, 1));
     NodeRec* trec = L.trec.get<NodeRec>(std::max(T.num_nodes, 1));
@@ -371,7 +371,7 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
             }
         }
         if (far_ready) FMM_HIP(hipEventRecord(far_ready, st));
-        if (far_host) far_host();
+        if (after_far) after_far();
         // ---- close pairs: sort by (target, source), per-target ranges
         {
             uint64_t* key = L.key.get<uint64_t>(std::max(nclose, 1));
@@ -387,7 +387,6 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
             k_fill_int<<<blocks(T.num_nodes), 256, 0, st>>>(ce, T.num_nodes, 0);
             if (nclose > 0) k_target_ranges<<<blocks(nclose), 256, 0, st>>>(ct, nclose, cb, ce);
         }
-        FMM_HIP(hipStreamSynchronize(st));
         return;
     }
     throw Error(4, "interaction list buffers did not converge");

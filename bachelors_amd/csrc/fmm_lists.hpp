@@ -24,10 +24,11 @@ struct Lists {
     DevBuf close_begin, close_end;          // per target node
 };
 
-// far_ready (may be null) is recorded on st once the far pairs and classes are complete, before
-// the close pairs are sorted; far_host (may be empty) is called right after, when the host-side
-// far data (num_far, num_classes, h_class_begin) is final
+// far_ready (may be null) is recorded on st once the far pairs and classes are complete, then
+// after_far (may be empty) is called (the host-side far data, num_far, num_classes and
+// h_class_begin, is final), then the close pairs are sorted.  Returns once everything is queued
+// on st (the host waits only for the list counts it reads back).
 void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T, double mac_scale, bool classes,
-                 hipStream_t st, hipEvent_t far_ready = nullptr, const std::function<void()>& far_host = {});
+                 hipStream_t st, hipEvent_t far_ready = nullptr, const std::function<void()>& after_far = {});
 
 }  // namespace fmm

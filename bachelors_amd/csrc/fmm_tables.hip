@@ -9,6 +9,7 @@
 // multiplied by sigma^(degree+1)), which makes every operator of the shared frame
 // independent of the level and keeps FP32/BF16 magnitudes uniform across degrees.
 #include "fmm_tables.hpp"
+#include "fmm_recip.hpp"
 
 namespace fmm {
 
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
         } else {
             // Phi_k^k = (x+iy)^k / (2^k k!)
             d = {1, 0};
-            for (int i = 1; i <= k; i++) d = cscale(cmul(d, {ux, uy}), 1.0 / (2.0 * i));
+            for (int i = 1; i <= k; i++) d = cscale(cmul(d, {ux, uy}), recip(2 * i));
         }
         cplx pm2 = {0, 0}, pm1 = d;
         X[tri(k, k)] = cscale(d, sgn_pow(k));
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
                 v = cscale(cadd(cscale(pm1, (2.0 * n - 1) * uz), cscale(pm2, -(double)((n - 1) * (n - 1) - k * k))),
                            1.0 / r2);
             else
-                v = cscale(cadd(cscale(pm1, (2.0 * n - 1) * uz), cscale(pm2, -r2)), 1.0 / (double)(n * n - k * k));
+                v = cscale(cadd(cscale(pm1, (2.0 * n - 1) * uz), cscale(pm2, -r2)), recip(n * n - k * k));
             X[tri(n, k)] = cscale(v, sgn_pow(n));
             pm2 = pm1;
             pm1 = v;
@@ -112,8 +113,11 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
     __syncthreads();
     const double c = 0.70710678118654752440;
     const size_t base = (size_t)blockIdx.x * Lp * Lp;
-    for (int e = threadIdx.x; e < Lp * Lp; e += blockDim.x) {
-        int i = e / Lp, j = e - (e / Lp) * Lp;
+    // element e = i Lp + j; (i, j) advanced by the stride instead of divided out (blockDim < Lp * Lp)
+    int i = threadIdx.x / Lp, j = threadIdx.x - (threadIdx.x / Lp) * Lp;
+    const int di = blockDim.x / Lp, dj = blockDim.x - di * Lp;
+    for (int e = threadIdx.x; e < Lp * Lp; e += blockDim.x, i += di, j += dj) {
+        if (j >= Lp) { j -= Lp; i++; }
         double val = 0;
         if (i < L && j < L) {
             const short2 oi = lmt[i], oj = lmt[j];
