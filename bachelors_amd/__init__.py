@@ -65,7 +65,8 @@ class Stats(ctypes.Structure):
                [(n, ctypes.c_double) for n in ("t_tree", "t_p2m", "t_m2m", "t_lists", "t_tables", "t_m2l", "t_l2l",
                                                "t_l2p", "t_p2p", "t_total", "t_m2l_gemm")] + \
                [(n, ctypes.c_int) for n in ("source_tree_sorted", "target_tree_sorted", "source_leaves",
-                                            "target_leaves")]
+                                            "target_leaves")] + \
+               [(n, ctypes.c_double) for n in ("t_to_m2l_gemm", "t_after_m2l_gemm", "t_lists_far")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

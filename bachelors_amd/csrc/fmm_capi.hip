@@ -7,7 +7,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <condition_variable>
 #include <exception>
+#include <mutex>
 #include <functional>
 #include <string>
 #include <thread>
@@ -33,7 +35,7 @@ constexpr int M2M_PARENT_MAJOR_COLUMNS = 64 * 256;
 // stage events: the stream-A chain, plus start marks of the stages that may run on the side
 // stream (lists, P2P) and of the translations after the lists (E_DOWN0)
 enum Ev { E_START, E_TREE, E_P2M, E_M2M, E_LISTS, E_TABLES, E_M2L, E_L2L, E_L2P, E_P2P, E_END, E_M2L_GEMM,
-          E_LISTS0, E_DOWN0, E_P2P0, E_GEMM0, E_COUNT };
+          E_LISTS0, E_DOWN0, E_P2P0, E_GEMM0, E_FAR, E_COUNT };
 
 // Launch data built on the host for each evaluation (pair lists, tiles, operator specs) is
 // appended to a pinned buffer and sent with one copy per stage: per-level copies from pageable
@@ -115,7 +117,7 @@ struct fmm_ctx {
     ScaleTable sigS, sigT;
     DevBuf sxyz, sq, sJ, txyz;                   // slot order
     DevBuf nP0, nP1, nP2;                        // near-field sources packed per leaf (slot order)
-    std::vector<int> h_src_leaves, h_tgt_leaves;
+    PinnedVec<int> h_src_leaves, h_tgt_leaves;   // useful (non-empty) leaves
     DevBuf src_leaves, tgt_leaves;
     // expansions [node][3][Lp]
     DevBuf M, Lx;
@@ -142,7 +144,8 @@ struct fmm_ctx {
     // side stream of the fused evaluation (lists and P2P beside the multipole chain) and its joins
     hipStream_t side = nullptr;
     hipEvent_t j_lists = nullptr, j_pack = nullptr, j_p2p = nullptr, j_gemm = nullptr,
-               j_start = nullptr, j_tables = nullptr;   // j_tables: everything the side stream builds for M2L
+               j_start = nullptr, j_tables = nullptr,   // j_tables: everything the side stream builds for M2L
+               j_src = nullptr, j_tgt = nullptr;        // source tree on the main / target side on the side stream
     long long near_inter = -1;
     bool lists_ready = false;
 
@@ -221,12 +224,12 @@ void run_pass(fmm_ctx* c, const OpTables& O, const Pass& P, const void* src_exp,
                c->stream);
 }
 
-void stage_trees(fmm_ctx* c) {
+// Roots of both trees (tc, th: target root) and the start mark of an evaluation.
+void trees_begin(fmm_ctx* c, double tc[3], double th[3]) {
     require(c->ns > 0 || c->nt > 0, FMM_ERR_STATE, "sources and targets are not set");
     hipStream_t st = c->stream;
     c->rec(E_START);
     const fmm_config& g = c->cfg;
-    double tc[3], th[3];
     if (g.frame == FMM_FRAME_SHARED) {
         // the source root, grown by doublings along the source lattice until it holds every
         // target (half-open), so all boxes of both trees stay on one dyadic lattice
@@ -257,53 +260,73 @@ void stage_trees(fmm_ctx* c) {
             th[a] = hd * 1.1;
         }
     }
-    // the two trees are independent: the target tree is built on the side stream by a host
-    // thread (its own scratch) while this thread builds the source tree; both end synchronised.
-    // The side stream first waits for everything queued on the main stream (a previous
-    // evaluation may still read the target tree there).
+    c->near_inter = -1;
+    c->lists_ready = false;
+    // the target tree is built on the side stream, which first waits for everything queued on
+    // the main stream (a previous evaluation may still read the target tree there)
     FMM_HIP(hipEventRecord(c->j_start, st));
     FMM_HIP(hipStreamWaitEvent(c->side, c->j_start, 0));
+}
+
+// useful (non-empty) leaves of T into h (pinned) and d, copied on stream st
+void upload_leaves(const TreeDev& T, PinnedVec<int>& h, DevBuf& d, hipStream_t st) {
+    h.resize(std::max(T.num_nodes, 1));
+    size_t k = 0;
+    for (int n = 0; n < T.num_nodes; n++)
+        if (T.h_first_child[n] < 0 && T.h_npts[n] > 0) h[k++] = n;
+    h.resize(k);
+    if (k) FMM_HIP(hipMemcpyAsync(d.get<int>(k), h.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
+}
+
+// source tree, sources in slot (leaf DFS) order and the useful source leaves, on stream A
+void build_source_side(fmm_ctx* c) {
+    hipStream_t st = c->stream;
+    const fmm_config& g = c->cfg;
+    build_tree(c->S, c->tscr, c->src_xyz.as<double>(), c->ns, g.source_root_center, g.source_root_half,
+               g.source_leaf_capacity, g.max_depth, true, st);
+    for (int k = 0; k < MAX_LEVELS; k++) c->sigS.s[k] = c->S.scale[k];
+    int nss = c->S.num_slots;
+    launch_charges(c->S.slots.as<int>(), nss, c->src_xyz.as<double>(), c->src_w.as<double>(),
+                   c->have_J ? c->src_J.as<double>() : nullptr, c->sxyz.get<double>(3 * (size_t)std::max(nss, 1)),
+                   c->sq.get<double>(3 * (size_t)std::max(nss, 1)), c->sJ.get<double>(3 * (size_t)std::max(nss, 1)), st);
+    upload_leaves(c->S, c->h_src_leaves, c->src_leaves, st);
+}
+
+// target tree, targets in slot order and the useful target leaves, on stream ts
+void build_target_side(fmm_ctx* c, const double tc[3], const double th[3], hipStream_t ts) {
+    const fmm_config& g = c->cfg;
+    build_tree(c->T, c->tscr_t, c->tgt_xyz.as<double>(), c->nt, tc, th, g.target_leaf_capacity + 1, g.max_depth,
+               g.frame == FMM_FRAME_SHARED, ts);
+    for (int k = 0; k < MAX_LEVELS; k++) c->sigT.s[k] = c->T.scale[k];
+    int nts = c->T.num_slots;
+    launch_gather_xyz(c->T.slots.as<int>(), nts, c->tgt_xyz.as<double>(), c->txyz.get<double>(3 * (size_t)std::max(nts, 1)),
+                      ts);
+    upload_leaves(c->T, c->h_tgt_leaves, c->tgt_leaves, ts);
+}
+
+// Both trees: the target side on the side stream by a host thread of its own (its own scratch)
+// while this thread builds the source side; the main stream then waits for the side stream.
+void stage_trees(fmm_ctx* c) {
+    double tc[3], th[3];
+    trees_begin(c, tc, th);
     std::exception_ptr tgt_error;
     std::thread tgt_thread([&] {
         try {
-            build_tree(c->T, c->tscr_t, c->tgt_xyz.as<double>(), c->nt, tc, th, g.target_leaf_capacity + 1,
-                       g.max_depth, g.frame == FMM_FRAME_SHARED, c->side);
+            build_target_side(c, tc, th, c->side);
+            FMM_HIP(hipEventRecord(c->j_tgt, c->side));
         } catch (...) {
             tgt_error = std::current_exception();
         }
     });
     try {
-        build_tree(c->S, c->tscr, c->src_xyz.as<double>(), c->ns, g.source_root_center, g.source_root_half,
-                   g.source_leaf_capacity, g.max_depth, true, st);
+        build_source_side(c);
     } catch (...) {
         tgt_thread.join();
         throw;
     }
     tgt_thread.join();
     if (tgt_error) std::rethrow_exception(tgt_error);
-    for (int k = 0; k < MAX_LEVELS; k++) { c->sigS.s[k] = c->S.scale[k]; c->sigT.s[k] = c->T.scale[k]; }
-    // sources and targets in slot (leaf DFS) order
-    int nss = c->S.num_slots, nts = c->T.num_slots;
-    launch_charges(c->S.slots.as<int>(), nss, c->src_xyz.as<double>(), c->src_w.as<double>(),
-                   c->have_J ? c->src_J.as<double>() : nullptr, c->sxyz.get<double>(3 * (size_t)std::max(nss, 1)),
-                   c->sq.get<double>(3 * (size_t)std::max(nss, 1)), c->sJ.get<double>(3 * (size_t)std::max(nss, 1)), st);
-    launch_gather_xyz(c->T.slots.as<int>(), nts, c->tgt_xyz.as<double>(), c->txyz.get<double>(3 * (size_t)std::max(nts, 1)),
-                      st);
-    // useful leaves
-    c->h_src_leaves.clear();
-    c->h_tgt_leaves.clear();
-    for (int n = 0; n < c->S.num_nodes; n++)
-        if (c->S.h_first_child[n] < 0 && c->S.h_npts[n] > 0) c->h_src_leaves.push_back(n);
-    for (int n = 0; n < c->T.num_nodes; n++)
-        if (c->T.h_first_child[n] < 0 && c->T.h_npts[n] > 0) c->h_tgt_leaves.push_back(n);
-    if (!c->h_src_leaves.empty())
-        FMM_HIP(hipMemcpyAsync(c->src_leaves.get<int>(c->h_src_leaves.size()), c->h_src_leaves.data(),
-                               c->h_src_leaves.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    if (!c->h_tgt_leaves.empty())
-        FMM_HIP(hipMemcpyAsync(c->tgt_leaves.get<int>(c->h_tgt_leaves.size()), c->h_tgt_leaves.data(),
-                               c->h_tgt_leaves.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    c->near_inter = -1;
-    c->lists_ready = false;
+    FMM_HIP(hipStreamWaitEvent(c->stream, c->j_tgt, 0));
     c->rec(E_TREE);
     c->stage = ST_TREES;
 }
@@ -609,7 +632,7 @@ void stage_translate(fmm_ctx* c) {
 
 void stage_downward(fmm_ctx* c) {
     require(c->stage >= ST_UP, FMM_ERR_STATE, "Multipoles are not ready!");
-    stage_lists(c, c->stream, nullptr);   // ST_UP implies the trees
+    stage_lists(c, c->stream, c->ev_ok ? c->ev[E_FAR] : nullptr);   // ST_UP implies the trees
     stage_translate(c);
 }
 
@@ -675,35 +698,66 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
         return;
     }
     hipStream_t A = c->stream, Bs = c->side;
-    stage_trees(c);
-    stage_leaves(c);
-    FMM_HIP(hipEventRecord(c->j_pack, A));
-    // the lists thread queues the far lists, the operator tables, then the close lists and the
-    // pair index on the side stream (beside P2M and M2M); it only blocks on the list counts it
-    // reads back.  The M2L GEMM waits for the tables (j_tables), the reduction for the pair index
-    // (j_lists, re-recorded at the end).
-    std::exception_ptr lists_error;
-    std::thread lists_thread([&] {
-        try {
-            stage_lists(c, Bs, nullptr, [&] {   // the trees are complete (stage_trees reads them back)
-                translate_tables(c, Bs);
-                FMM_HIP(hipEventRecord(c->j_tables, Bs));
-            });
-            FMM_HIP(hipEventRecord(c->j_lists, Bs));
-        } catch (...) {
-            lists_error = std::current_exception();
+    // Side thread (stream Bs): target tree, targets in slot order, then -- once the source tree
+    // is built -- the far lists (E_FAR), the close lists and the pair index (j_lists).  This
+    // thread (stream A): source tree, charges, P2M, M2M, then -- once the far classes are known
+    // on the host -- the operator tables and the M2L GEMM.  Neither thread blocks on the other's
+    // GPU work except through events; the side thread only waits for the counts it reads back.
+    double tc[3], th[3];
+    trees_begin(c, tc, th);
+    std::mutex mu;
+    std::condition_variable cv;
+    int src_state = 0, far_state = 0;   // 0 pending, 1 ready, 2 failed
+    std::exception_ptr side_error;
+    auto post = [&](int& state, int v) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            state = v;
         }
+        cv.notify_all();
+    };
+    std::thread side_thread([&] {
+        try {
+            build_target_side(c, tc, th, Bs);
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return src_state != 0; });
+            }
+            if (src_state == 1) {
+                FMM_HIP(hipStreamWaitEvent(Bs, c->j_src, 0));   // the source tree on stream A
+                stage_lists(c, Bs, c->ev[E_FAR], [&] { post(far_state, 1); });   // E_FAR also joins
+                FMM_HIP(hipEventRecord(c->j_lists, Bs));
+            }
+        } catch (...) {
+            side_error = std::current_exception();
+        }
+        if (far_state == 0) post(far_state, 2);
     });
     try {
+        build_source_side(c);
+        FMM_HIP(hipEventRecord(c->j_src, A));
+        post(src_state, 1);
+        c->rec(E_TREE);
+        c->stage = ST_TREES;
+        stage_leaves(c);
+        FMM_HIP(hipEventRecord(c->j_pack, A));   // packed near-field sources, for the P2P
         stage_upward(c);
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return far_state != 0; });
+        }
+        if (far_state == 1) {
+            FMM_HIP(hipStreamWaitEvent(A, c->ev[E_FAR], 0));   // far pairs and classes
+            translate_tables(c, A);
+            translate_gemm(c);
+        }
     } catch (...) {
-        lists_thread.join();
+        if (src_state == 0) post(src_state, 2);
+        side_thread.join();
         throw;
     }
-    lists_thread.join();
-    if (lists_error) std::rethrow_exception(lists_error);
-    FMM_HIP(hipStreamWaitEvent(A, c->j_tables, 0));   // far lists, tables, tiles
-    translate_gemm(c);
+    side_thread.join();
+    if (side_error) std::rethrow_exception(side_error);
     FMM_HIP(hipStreamWaitEvent(A, c->j_lists, 0));    // close lists, pair index
     translate_finish(c);
     FMM_HIP(hipStreamWaitEvent(Bs, c->j_pack, 0));
@@ -813,6 +867,8 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
         FMM_HIP(hipEventCreateWithFlags(&c->j_gemm, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_start, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_tables, hipEventDisableTiming));
+        FMM_HIP(hipEventCreateWithFlags(&c->j_src, hipEventDisableTiming));
+        FMM_HIP(hipEventCreateWithFlags(&c->j_tgt, hipEventDisableTiming));
     });
     if (rc != FMM_OK) {
         delete c;
@@ -828,7 +884,7 @@ void fmm_destroy(fmm_ctx* c) {
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_ok)
         for (int i = 0; i < E_COUNT; i++) (void)hipEventDestroy(c->ev[i]);
-    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm, c->j_start, c->j_tables})
+    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm, c->j_start, c->j_tables, c->j_src, c->j_tgt})
         if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -1040,6 +1096,7 @@ int fmm_get_stats(fmm_ctx* c, fmm_stats* s) {
             s->t_p2m = el(E_TREE, E_P2M);
             s->t_m2m = el(E_P2M, E_M2M);
             s->t_lists = el(E_LISTS0, E_LISTS);
+            s->t_lists_far = el(E_LISTS0, E_FAR);
             s->t_tables = el(E_DOWN0, E_TABLES);
             s->t_m2l = c->cfg.frame == FMM_FRAME_SHARED ? el(E_GEMM0, E_M2L) : el(E_TABLES, E_M2L);
             s->t_l2l = el(E_M2L, E_L2L);
@@ -1047,6 +1104,10 @@ int fmm_get_stats(fmm_ctx* c, fmm_stats* s) {
             s->t_p2p = el(E_P2P0, E_P2P);
             s->t_total = el(E_START, E_END);
             s->t_m2l_gemm = c->cfg.frame == FMM_FRAME_SHARED ? el(E_GEMM0, E_M2L_GEMM) : s->t_m2l;
+            if (c->cfg.frame == FMM_FRAME_SHARED) {
+                s->t_to_m2l_gemm = el(E_START, E_GEMM0);
+                s->t_after_m2l_gemm = el(E_M2L_GEMM, E_END);
+            }
         }
     });
 }

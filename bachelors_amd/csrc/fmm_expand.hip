@@ -74,30 +74,46 @@ __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, cons
     const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
     const double inv = 1.0 / sig.s[level[leaf]];
     if (g < G) {
-        for (int j = begin[leaf] + g; j < end[leaf]; j += G) {
+        // two points per pass (j and j + G; a missing second point has zero charge): their
+        // recurrence chains are independent, so the two hide each other's latency; the sums
+        // still add the points in the order j, j + G, j + 2G, ...
+        const int jb = begin[leaf], je = end[leaf];
+        for (int j = jb + g; j < je; j += 2 * G) {
+            const bool two = j + G < je;
+            const int j1 = two ? j + G : j;
             double ux = (sxyz[3 * j] - cx) * inv, uy = (sxyz[3 * j + 1] - cy) * inv, uz = (sxyz[3 * j + 2] - cz) * inv;
-            double r2 = ux * ux + uy * uy + uz * uz;
+            double wx = (sxyz[3 * j1] - cx) * inv, wy = (sxyz[3 * j1 + 1] - cy) * inv, wz = (sxyz[3 * j1 + 2] - cz) * inv;
+            double r2 = ux * ux + uy * uy + uz * uz, s2 = wx * wx + wy * wy + wz * wz;
             double q0 = sq[3 * j], q1 = sq[3 * j + 1], q2 = sq[3 * j + 2];
-            double dr = 1.0, di = 0.0;   // Phi_m^m = (x + iy)^m / (2^m m!)
+            double t0 = two ? sq[3 * j1] : 0.0, t1 = two ? sq[3 * j1 + 1] : 0.0, t2 = two ? sq[3 * j1 + 2] : 0.0;
+            double dr = 1.0, di = 0.0, er = 1.0, ei = 0.0;   // Phi_m^m = (x + iy)^m / (2^m m!)
             for (int i = 1; i <= m; i++) {
-                double s = recip(2 * i);
+                double s = 0.5 / i;
                 double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
-                dr = nr;
-                di = ni;
+                double mr = (er * wx - ei * wy) * s, mi = (er * wy + ei * wx) * s;
+                dr = nr; di = ni; er = mr; ei = mi;
             }
             double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
+            double o1r = er, o1i = ei, o2r = 0.0, o2i = 0.0;
 #pragma unroll
             for (int l = 0; l < P1; l++) {
                 if (l < m) continue;
-                double vr = dr, vi = di;
+                double vr = dr, vi = di, yr = er, yi = ei;
                 if (l > m) {   // Phi_l^m = ((2l-1) z Phi_{l-1}^m - r^2 Phi_{l-2}^m) / (l^2 - m^2)
-                    double f = recip(l * l - m * m);
+                    double f = 1.0 / (double)(l * l - m * m);
                     vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
                     vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
+                    yr = ((2.0 * l - 1) * wz * o1r - s2 * o2r) * f;
+                    yi = ((2.0 * l - 1) * wz * o1i - s2 * o2i) * f;
                     p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
+                    o2r = o1r; o2i = o1i; o1r = yr; o1i = yi;
                 }
                 ar[0][l] += q0 * vr; ar[1][l] += q1 * vr; ar[2][l] += q2 * vr;
                 ai[0][l] += q0 * vi; ai[1][l] += q1 * vi; ai[2][l] += q2 * vi;
+                if (two) {
+                    ar[0][l] += t0 * yr; ar[1][l] += t1 * yr; ar[2][l] += t2 * yr;
+                    ai[0][l] += t0 * yi; ai[1][l] += t1 * yi; ai[2][l] += t2 * yi;
+                }
             }
         }
         // real basis: (-1)^l for m = 0, (-1)^l sqrt2 (Re, Im) for +-m

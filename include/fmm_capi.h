@@ -80,6 +80,9 @@ typedef struct fmm_stats {
     int source_tree_sorted, target_tree_sorted;   /* 1: built by one sort of point paths;
                                                      0: level by level (gaps/overlaps of child boxes) */
     int source_leaves, target_leaves;    /* leaves holding points */
+    double t_to_m2l_gemm;                /* evaluation start to the start of the M2L GEMM */
+    double t_after_m2l_gemm;             /* end of the M2L GEMM to the end of the evaluation */
+    double t_lists_far;                  /* the far pairs and classes (first part of t_lists) */
 } fmm_stats;
 
 void fmm_config_init(fmm_config* cfg);
