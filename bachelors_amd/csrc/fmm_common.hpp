@@ -104,6 +104,18 @@ struct ScaleTable {
 // grid size for n work items in workgroups of b
 inline int blocks(long long n, int b = 256) { return (int)((n + b - 1) / b); }
 
+// compute units of the current device (queried once per process)
+inline int num_cus() {
+    static const int n = [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        return v;
+    }();
+    return n;
+}
+
 struct TreeDev {
     int num_nodes = 0;
     int num_slots = 0;                 // points kept by the tree (leaf DFS order)

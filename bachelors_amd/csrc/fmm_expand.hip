@@ -8,6 +8,8 @@
 // (harmonics.cpp:20-56 with the sectoral term m = p included).  All recurrences run in
 // FP64 on the scaled vector u = (x - c) / sigma; only P2P runs in FP32 (with FP64
 // accumulation per target).
+#include <algorithm>
+
 #include "fmm_expand.hpp"
 #include "fmm_recip.hpp"
 
@@ -57,15 +59,17 @@ __global__ void k_gather_xyz(const int* __restrict__ slots, int n, const double*
 // through LDS at the end.  M_l^{+-m} = sum_j q_j R_l^{+-m}(u_j), u = (y - c) / sigma, with
 // R(l, 0) = (-1)^l Phi_l^0, R(l, +-m) = (-1)^l sqrt2 {Re, Im} Phi_l^m (harmonics.cpp:281-320).
 template <class T, int P>
-__global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, const double* __restrict__ center,
-                                            const int* __restrict__ begin, const int* __restrict__ end,
-                                            const int* __restrict__ level, ScaleTable sig,
+__global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, int nleaves,
+                                            const double* __restrict__ center, const int* __restrict__ begin,
+                                            const int* __restrict__ end, const int* __restrict__ level, ScaleTable sig,
                                             const double* __restrict__ sxyz, const double* __restrict__ sq, int Lp,
                                             T* __restrict__ M) {
     constexpr int P1 = P + 1, G = 64 / P1, L = P1 * P1;
     __shared__ double part[G][3][L];
-    const int leaf = leaves[blockIdx.x];
     const int lane = threadIdx.x, m = lane % P1, g = lane / P1;
+  // a grid smaller than the leaf count: the wave strides over leaves
+  for (int il = blockIdx.x; il < nleaves; il += gridDim.x) {
+    const int leaf = leaves[il];
     double ar[3][P1], ai[3][P1];
 #pragma unroll
     for (int c = 0; c < 3; c++)
@@ -138,6 +142,8 @@ __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, cons
             for (int gg = 0; gg < G; gg++) s += part[gg][c][k];
         out[e] = (T)s;
     }
+    __syncthreads();   // part is rewritten for the next leaf
+  }
 }
 
 // ---------------------------------------------------------------- L2P
@@ -460,8 +466,12 @@ void launch_gather_xyz(const int* slots, int n, const double* xyz, double* out, 
 template <class T, int P>
 static void p2m_order(const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
                       const double* sq, int Lp, T* M, hipStream_t st) {
-    k_p2m<T, P><<<nleaves, 64, 0, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
-                                        S.level.as<int>(), sig, sxyz, sq, Lp, M);
+    // at most 8 waves per CU, each striding over leaves: the P2M waves hold most of a SIMD's
+    // registers, and a capped grid leaves the rest of the GPU to the interaction lists that run
+    // beside it (measured: the lists' far part 1.59 -> 1.49 ms, the P2M 0.60 -> 0.73 ms)
+    const int grid = std::min(nleaves, 8 * num_cus());
+    k_p2m<T, P><<<grid, 64, 0, st>>>(leaves, nleaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
+                                     S.level.as<int>(), sig, sxyz, sq, Lp, M);
 }
 template <class T>
 static void p2m_dispatch(int p, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig,

@@ -296,9 +296,13 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
             L.h_seed[2] = 1;
             FMM_HIP(hipMemcpyAsync(fa, L.h_seed, sizeof(int2), hipMemcpyHostToDevice, st));
             FMM_HIP(hipMemcpyAsync(cnt + 4, L.h_seed + 2, sizeof(unsigned), hipMemcpyHostToDevice, st));
-            // steps queue without host synchronisation; the grid strides over any frontier size
-            const int grid = 1024;
+            // steps queue without host synchronisation; the grid strides over any frontier size.
+            // Step k's frontier holds at most 8^k pairs (each pair descends into at most 8), so the
+            // first steps launch only the workgroups they can use (they start sooner on a busy GPU)
+            size_t bound = 1;
             for (int k = 0; k < max_steps; k++) {
+                const int grid = (int)std::min<size_t>(1024, (bound + 255) / 256);
+                bound = std::min<size_t>(bound * 8, (size_t)front_cap);
                 k_lists_step<<<grid, 256, 0, st>>>(fa, cnt + 4 + k, front_cap, srec, rs, trec, rt, mac_scale, far,
                                                    cnt + 0, far_cap, close, cnt + 1, close_cap, fb, cnt + 4 + k + 1,
                                                    front_cap);
