@@ -294,6 +294,8 @@ template <> struct V4<double> {
 // denormal-scaling wrapper of rsqrtf is not needed
 __device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // P0[j] = (hi(y_j - o), q_x), P1[j] = (lo(y_j - o), q_y), P2[j] = q_z for every source slot j
 template <class R>
@@ -368,6 +370,7 @@ __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, con
                 }
                 __syncwarp();
                 R f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll 4
                 for (int jj = g; jj < nj; jj += G) {
                     const v4 yh = s0[wave][jj];
                     const v4 yl = s1[wave][jj];
@@ -378,12 +381,13 @@ __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, con
                         dy += uly - yl.y;
                         dz += ulz - yl.z;
                     }
-                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    // fused multiply-adds (explicit: this library is built without contraction)
+                    const R r2 = fma_t(dx, dx, fma_t(dy, dy, dz * dz));
                     const R ri = r2 > (R)0 ? rsqrt_t(r2) : (R)0;
                     const R ri3 = ri * ri * ri;
-                    f0 += (qy * dz - qz * dy) * ri3;
-                    f1 += (qz * dx - qx * dz) * ri3;
-                    f2 += (qx * dy - qy * dx) * ri3;
+                    f0 = fma_t(fma_t(qy, dz, -qz * dy), ri3, f0);
+                    f1 = fma_t(fma_t(qz, dx, -qx * dz), ri3, f1);
+                    f2 = fma_t(fma_t(qx, dy, -qy * dx), ri3, f2);
                 }
                 a0 += f0;
                 a1 += f1;
