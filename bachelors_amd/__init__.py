@@ -16,9 +16,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbachelors_fmm.so")
 
 FMM_OK = 0
-PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_F16X3 = 0, 1, 2, 3
 FRAME_REFERENCE, FRAME_SHARED = 0, 1
-PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "f16x3": PREC_F16X3}
 
 
 class M2MAlg(enum.IntEnum):          # multipole_solver_enums.hpp:3-10

@@ -132,6 +132,11 @@ struct fmm_ctx {
     Lists lists;
     ListScratch lscr;
     PairIndex m2l_ix;                            // far pairs by target node
+    // F16X3: per-coefficient scales dm, the M2L tables split in FP16 (their FP32 staging), the
+    // multipole column scales; down_l2l = L2L operators ahead of the M2L ones in the down tables
+    DevBuf dm, m2l_hi, m2l_lo, m2l_rinv, m2l_f32stage, cscale;
+    int down_l2l = 0;
+    const OpSpec* down_specs = nullptr;          // device specs of the down tables
     DevBuf m2l_slab;                             // M2L result per (pair, component)
     // evaluation
     DevBuf Bfar, Bnear, Bout;
@@ -150,6 +155,8 @@ struct fmm_ctx {
     bool lists_ready = false;
 
     bool f64() const { return prec == Precision::F64; }
+    // precision of the M2M / L2L passes: F16X3 splits only the M2L, the others run in FP32
+    Precision pass_prec() const { return prec == Precision::F16X3 ? Precision::F32 : prec; }
     size_t esize() const { return f64() ? sizeof(double) : sizeof(float); }
     void rec(int e) { if (ev_ok) FMM_HIP(hipEventRecord(ev[e], stream)); }
     void rec_on(int e, hipStream_t s) { if (ev_ok) FMM_HIP(hipEventRecord(ev[e], s)); }
@@ -180,6 +187,48 @@ void require(bool c, int code, const char* msg) {
 
 bool nonempty_host(const TreeDev& T, int n) { return T.h_first_child[n] >= 0 || T.h_npts[n] > 0; }
 
+// F16X3: the scale of real coefficient i = (l, m), a power of 2 with dm_i max |R_i(u)| in
+// (0.5, 1] over |u| <= sqrt(3) (the scaled position of any point in its box), so every scaled
+// multipole coefficient is bounded by the same sum of |charges|.  The maximum is taken over a
+// Fibonacci lattice of 4096 directions at |u| = sqrt(3) (R_i is homogeneous of degree l);
+// coefficients beyond L get 1.
+std::vector<float> coefficient_scales(int p, int kpad) {
+    const int L = (p + 1) * (p + 1);
+    std::vector<double> mx(L, 0.0);
+    const int N = 4096;
+    const double ga = M_PI * (3.0 - std::sqrt(5.0)), rr = std::sqrt(3.0);
+    std::vector<double> pr((p + 1) * (p + 1)), pi((p + 1) * (p + 1));
+    for (int n = 0; n < N; n++) {
+        const double z0 = 1.0 - (2.0 * n + 1.0) / N, s0 = std::sqrt(std::max(0.0, 1.0 - z0 * z0));
+        const double x = rr * s0 * std::cos(ga * n), y = rr * s0 * std::sin(ga * n), z = rr * z0, r2 = 3.0;
+        for (int m = 0; m <= p; m++) {
+            double dr = 1.0, di = 0.0;   // Phi_m^m = (x + iy)^m / (2^m m!)
+            for (int i = 1; i <= m; i++) {
+                const double nr = (dr * x - di * y) / (2.0 * i), ni = (dr * y + di * x) / (2.0 * i);
+                dr = nr;
+                di = ni;
+            }
+            double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
+            for (int l = m; l <= p; l++) {
+                double vr = dr, vi = di;
+                if (l > m) {
+                    vr = ((2.0 * l - 1) * z * p1r - r2 * p2r) / (double)(l * l - m * m);
+                    vi = ((2.0 * l - 1) * z * p1i - r2 * p2i) / (double)(l * l - m * m);
+                    p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
+                }
+                const double a = m == 0 ? std::fabs(vr) : std::sqrt(2.0) * std::hypot(vr, vi);
+                const int ip = l * l + l + m, im = l * l + l - m;
+                mx[ip] = std::max(mx[ip], a);
+                mx[im] = std::max(mx[im], a);
+            }
+        }
+    }
+    std::vector<float> dm(kpad, 1.0f);
+    for (int i = 0; i < L; i++)
+        if (mx[i] > 0) dm[i] = (float)std::ldexp(1.0, -(int)std::ceil(std::log2(mx[i])));
+    return dm;
+}
+
 // reference child order (quadrature_octree.cpp:88-96)
 const int SIGN[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                         {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
@@ -201,27 +250,31 @@ void resolve_pass(Pass& P, const char* dev) {
 }
 
 // O.nops operators from device specs, written in the precision the GEMMs apply
-void build_tables(fmm_ctx* c, OpTables& O, const OpSpec* d_specs, hipStream_t st) {
-    size_t n = (size_t)O.nops * c->Lp * c->Lp;
+// O.nops operators (from device specs) in the precision of the passes that apply them
+void build_tables(fmm_ctx* c, OpTables& O, const OpSpec* d_specs, hipStream_t st, int nops = -1) {
+    if (nops < 0) nops = O.nops;
+    size_t n = (size_t)nops * c->Lp * c->Lp;
     if (!n) return;
-    if (c->prec == Precision::F64)
-        make_operators(d_specs, O.nops, c->p, c->Lp, 0, O.f64.get<double>(n), nullptr, st);
-    else if (c->prec == Precision::F32)
-        make_operators(d_specs, O.nops, c->p, c->Lp, 1, O.f32.get<float>(n), nullptr, st);
+    const Precision pr = c->pass_prec();
+    if (pr == Precision::F64)
+        make_operators(d_specs, nops, c->p, c->Lp, 0, O.f64.get<double>(n), nullptr, st);
+    else if (pr == Precision::F32)
+        make_operators(d_specs, nops, c->p, c->Lp, 1, O.f32.get<float>(n), nullptr, st);
     else
-        make_operators(d_specs, O.nops, c->p, c->Lp, 2, O.hi.get<__bf16>(n), O.lo.get<__bf16>(n), st);
+        make_operators(d_specs, nops, c->p, c->Lp, 2, O.hi.get<__bf16>(n), O.lo.get<__bf16>(n), st);
 }
 
 const void* ops_ptr(const fmm_ctx* c, const OpTables& O) {
-    if (c->prec == Precision::F64) return O.f64.ptr;
-    if (c->prec == Precision::F32) return O.f32.ptr;
+    const Precision pr = c->pass_prec();
+    if (pr == Precision::F64) return O.f64.ptr;
+    if (pr == Precision::F32) return O.f32.ptr;
     return O.hi.ptr;
 }
 
 void run_pass(fmm_ctx* c, const OpTables& O, const Pass& P, const void* src_exp, void* dst_exp, int epilogue) {
     if (!P.ntiles) return;
-    class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, P.src, P.dst, src_exp, dst_exp, c->Lp, epilogue,
-               c->stream);
+    class_gemm(c->pass_prec(), ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, P.src, P.dst, src_exp, dst_exp, c->Lp,
+               epilogue, c->stream);
 }
 
 // Roots of both trees (tc, th: target root) and the start mark of an evaluation.
@@ -409,7 +462,7 @@ void stage_upward(fmm_ctx* c) {
         resolve_pass(P, dev);
         if (!P.ntiles) continue;
         if (parent_major[k])
-            class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, P.src, P.dst, c->M.ptr, c->M.ptr, c->Lp,
+            class_gemm(c->pass_prec(), ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, P.src, P.dst, c->M.ptr, c->M.ptr, c->Lp,
                        EPI_ADD, c->stream, false, 8);
         else
             run_pass(c, O, P, c->M.ptr, c->M.ptr, EPI_ATOMIC);
@@ -441,7 +494,7 @@ void run_per_pair(fmm_ctx* c, int kind, const int* d_src, const int* d_dst, int 
         for (int i = 0; i < n; i++) tiles[i] = GemmTile{i, 3 * (b + i), 3};
         GemmTile* dt = c->pp_tiles.get<GemmTile>(n);
         FMM_HIP(hipMemcpyAsync(dt, tiles.data(), n * sizeof(GemmTile), hipMemcpyHostToDevice, st));
-        class_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, dt, n, d_src, d_dst, src_exp, dst_exp, c->Lp,
+        class_gemm(c->pass_prec(), ops_ptr(c, O), O.lo.ptr, dt, n, d_src, d_dst, src_exp, dst_exp, c->Lp,
                    atomic ? EPI_ATOMIC : EPI_ADD, st);
         FMM_HIP(hipStreamSynchronize(st));   // the chunk buffers are reused
     }
@@ -531,7 +584,22 @@ void translate_tables(fmm_ctx* c, hipStream_t st) {
     make_m2l_specs(c->lists.class_begin.as<int>(), nc, c->lists.far_t.as<int>(), c->lists.far_s.as<int>(),
                    S.center.as<double>(), T.center.as<double>(), S.level.as<int>(), T.level.as<int>(), c->sigS, c->sigT,
                    c->cfg.m2l_full, d + n_l2l, st);
-    if (O.nops) {
+    c->down_l2l = n_l2l;
+    c->down_specs = d;
+    if (c->prec == Precision::F16X3) {
+        build_tables(c, O, d, st, n_l2l);   // L2L in FP32
+        if (nc) {
+            F16Split f;
+            f.R = m2l_f16_rows(c->L);
+            f.Kp = m2l_f16_kpad(c->L);
+            f.hi = c->m2l_hi.get<_Float16>((size_t)nc * f.R * f.Kp);
+            f.lo = c->m2l_lo.get<_Float16>((size_t)nc * f.R * f.Kp);
+            f.rinv = c->m2l_rinv.get<float>((size_t)nc * f.R);
+            f.dm = c->dm.as<float>();
+            make_operators(d + n_l2l, nc, c->p, c->Lp, 3, c->m2l_f32stage.get<float>((size_t)nc * c->Lp * c->Lp),
+                           nullptr, st, &f);
+        }
+    } else if (O.nops) {
         build_tables(c, O, d, st);
     }
     // the M2L tiles (planned while the tables are built)
@@ -562,12 +630,22 @@ void translate_gemm(fmm_ctx* c) {
     hipStream_t st = c->stream;
     const OpTables& O = c->down_ops;
     const Pass& P = c->m2l_pass;
+    const bool f16 = c->prec == Precision::F16X3;
+    if (f16 && P.ntiles) {   // multipole column scales (the B side of the FP16 split)
+        const int ncols = 3 * c->S.num_nodes;
+        column_scales(c->M.as<float>(), ncols, c->L, c->Lp, c->dm.as<float>(), c->cscale.get<float>(ncols), st);
+    }
     c->rec(E_GEMM0);
     if (P.ntiles) {
         const int R = m2l_slab_rows(c->prec, c->L, c->Lp);
         void* slab = c->m2l_slab.reserve((size_t)3 * P.npairs * R * c->esize());
-        m2l_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, c->lists.far_s.as<int>(), c->M.ptr, slab, c->L,
-                 c->Lp, st);
+        if (f16)
+            m2l_gemm_f16x3(c->m2l_hi.as<_Float16>(), c->m2l_lo.as<_Float16>(), c->m2l_rinv.as<float>(), c->down_l2l,
+                           P.tiles, P.ntiles, c->lists.far_s.as<int>(), c->M.as<float>(), c->dm.as<float>(),
+                           c->cscale.as<float>(), (float*)slab, c->L, c->Lp, st);
+        else
+            m2l_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, c->lists.far_s.as<int>(), c->M.ptr, slab,
+                     c->L, c->Lp, st);
     }
     c->rec(E_M2L_GEMM);
     FMM_HIP(hipEventRecord(c->j_gemm, st));
@@ -843,7 +921,7 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
     *out = nullptr;
     if (cfg->order < 1 || cfg->order > MAX_ORDER) return FMM_ERR_ARG;
     if (cfg->source_leaf_capacity < 2 || cfg->target_leaf_capacity < 1) return FMM_ERR_ARG;
-    if (cfg->precision < 0 || cfg->precision > 2 || cfg->frame < 0 || cfg->frame > 1) return FMM_ERR_ARG;
+    if (cfg->precision < 0 || cfg->precision > 3 || cfg->frame < 0 || cfg->frame > 1) return FMM_ERR_ARG;
     if (!(cfg->mac_scale >= 1.0)) return FMM_ERR_ARG;
     fmm_ctx* c = new fmm_ctx();
     c->cfg = *cfg;
@@ -861,6 +939,11 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
         for (int i = 0; i < E_COUNT; i++) FMM_HIP(hipEventCreate(&c->ev[i]));
         c->ev_ok = true;
         FMM_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));   // default priority
+        if (c->prec == Precision::F16X3) {
+            const int kp = m2l_f16_kpad(c->L);
+            std::vector<float> h = coefficient_scales(c->p, kp);
+            FMM_HIP(hipMemcpy(c->dm.get<float>(kp), h.data(), kp * sizeof(float), hipMemcpyHostToDevice));
+        }
         FMM_HIP(hipEventCreateWithFlags(&c->j_lists, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_pack, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_p2p, hipEventDisableTiming));
@@ -980,7 +1063,10 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
         require(c->cfg.frame == FMM_FRAME_SHARED, FMM_ERR_UNSUPPORTED, "the adjoint needs the shared frame");
         FMM_HIP(hipDeviceSynchronize());
         AdjointArgs a;
-        a.prec = c->prec;
+        a.prec = c->pass_prec();
+        if (c->prec == Precision::F16X3 && c->down_ops.nops > c->down_l2l)
+            // the transposed passes apply the down operators in FP32: the M2L ones too
+            build_tables(c, c->down_ops, c->down_specs, c->stream);
         a.p = c->p;
         a.Lp = c->Lp;
         a.S = &c->S;

@@ -409,6 +409,176 @@ __global__ __launch_bounds__(256) void k_gemm_m2l(const T* __restrict__ ops, con
     }
 }
 
+// ---------------------------------------------------------------- M2L, FP16 split
+// The M2L of the F16X3 precision: every product of the FP32 GEMM is replaced by three FP16
+// products on v_mfma_f32_16x16x32_f16 (FP32 accumulate), A = Ah + Al and B = Bh + Bl with
+// hi and lo in FP16 (22 significant bits together), C = Al Bh + Ah Bl + Ah Bh.  FP16 has a
+// narrow range, so both sides are equilibrated by powers of 2 (exact):
+//   A_eq = diag(rs) T diag(1/dm)   (operator tables: row scale rs per operator, dm per input
+//                                   coefficient; the tables hold A_eq split, rinv = 1 / rs)
+//   B_eq = diag(dm) M diag(s)      (multipoles: dm per coefficient, s per column, split here
+//                                   while staging B from the FP32 multipoles)
+// so T M = diag(rinv) (A_eq B_eq) diag(1/s), applied in the epilogue.  Rows R = 16 NB >= L,
+// K padded to Kp = 32 NKS >= L (zero operator columns and multipole rows beyond L).
+// Layout and schedule follow k_gemm_m2l: wave w owns the tile's columns [16w, 16w + 16) and all
+// NB row blocks; one LDS buffer refilled from registers whose loads overlap the MFMAs.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
+template <int NB, int NKS>
+__global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restrict__ ops_hi,
+                                                        const _Float16* __restrict__ ops_lo,
+                                                        const float* __restrict__ ops_rinv,
+                                                        const GemmTile* __restrict__ tiles,
+                                                        const int* __restrict__ pair_src,
+                                                        const float* __restrict__ src_exp, const float* __restrict__ dm,
+                                                        const float* __restrict__ cscale, float* __restrict__ slab,
+                                                        int Lp, int op_base) {
+    constexpr int R = 16 * NB, Kp = 32 * NKS, BK = 32, S = BK + 8;   // LDS row stride: 80 bytes
+    constexpr int NQ = (4 * R + 255) / 256;                          // A: 8-half vectors per thread and part
+    __shared__ __attribute__((aligned(16))) _Float16 Ah[R][S];
+    __shared__ __attribute__((aligned(16))) _Float16 Al[R][S];
+    __shared__ __attribute__((aligned(16))) _Float16 Bh[BN][S];
+    __shared__ __attribute__((aligned(16))) _Float16 Bl[BN][S];
+    // each XCD works on a contiguous eighth of the tiles (a bijection of the grid)
+    const int nt = gridDim.x, xcd = blockIdx.x & 7, q = nt >> 3, rem = nt & 7;
+    const int t = xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
+    const GemmTile tile = tiles[t];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, li = lane & 15;
+    const int op = tile.op - op_base;   // tile.op numbers the operators of all down passes
+    const _Float16* Ahg = ops_hi + (size_t)op * R * Kp;
+    const _Float16* Alg = ops_lo + (size_t)op * R * Kp;
+    // B staging: column bc, k = bq .. bq + 7 of each K step
+    const int bc = tid >> 2, bq = (tid & 3) * 8;
+    const float* bptr = nullptr;
+    float bs = 0.0f;
+    if (bc < tile.ncol) {
+        const int col = tile.col0 + bc, pair = col / 3, comp = col - 3 * pair;
+        const int c = pair_src[pair] * 3 + comp;
+        bptr = src_exp + (size_t)c * Lp;
+        bs = cscale[c];
+    }
+    half8 rah[NQ], ral[NQ];
+    float4 rb0, rb1;
+    float4 d0, d1;
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const int qi = tid + 256 * j;
+            if (qi < 4 * R) {
+                const size_t o = (size_t)(qi >> 2) * Kp + k0 + (qi & 3) * 8;
+                rah[j] = *(const half8*)(Ahg + o);
+                ral[j] = *(const half8*)(Alg + o);
+            }
+        }
+        d0 = *(const float4*)(dm + k0 + bq);
+        d1 = *(const float4*)(dm + k0 + bq + 4);
+        if (bptr) {
+            rb0 = *(const float4*)(bptr + k0 + bq);
+            rb1 = *(const float4*)(bptr + k0 + bq + 4);
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const int qi = tid + 256 * j;
+            if (qi < 4 * R) {
+                *(half8*)&Ah[qi >> 2][(qi & 3) * 8] = rah[j];
+                *(half8*)&Al[qi >> 2][(qi & 3) * 8] = ral[j];
+            }
+        }
+        const float v[8] = {rb0.x * d0.x, rb0.y * d0.y, rb0.z * d0.z, rb0.w * d0.w,
+                            rb1.x * d1.x, rb1.y * d1.y, rb1.z * d1.z, rb1.w * d1.w};
+        half8 h, l;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float x = bptr ? v[i] * bs : 0.0f;
+            const _Float16 hh = (_Float16)x;
+            h[i] = hh;
+            l[i] = (_Float16)(x - (float)hh);
+        }
+        *(half8*)&Bh[bc][bq] = h;
+        *(half8*)&Bl[bc][bq] = l;
+    };
+    f32x4 acc[NB];
+#pragma unroll
+    for (int a = 0; a < NB; a++) acc[a] = f32x4{0, 0, 0, 0};
+    gload(0);
+    for (int kt = 0; kt < NKS; kt++) {
+        lstore();
+        __syncthreads();
+        if (kt + 1 < NKS) gload((kt + 1) * BK);
+        const int kk = 8 * g;
+        const half8 bh = *(const half8*)&Bh[wave * 16 + li][kk];
+        const half8 bl = *(const half8*)&Bl[wave * 16 + li][kk];
+#pragma unroll
+        for (int a = 0; a < NB; a++) {
+            const half8 ah = *(const half8*)&Ah[a * 16 + li][kk];
+            const half8 al = *(const half8*)&Al[a * 16 + li][kk];
+            acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[a], 0, 0, 0);
+            acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[a], 0, 0, 0);
+            acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[a], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // epilogue: undo the scales; the slab is written once and read once (nontemporal)
+    const int cl = wave * 16 + li;
+    if (cl >= tile.ncol) return;
+    const int col = tile.col0 + cl, pair = col / 3, comp = col - 3 * pair;
+    const float sinv = 1.0f / cscale[pair_src[pair] * 3 + comp];
+    const float* rv = ops_rinv + (size_t)op * R;
+    float* out = slab + (size_t)col * R;
+#pragma unroll
+    for (int a = 0; a < NB; a++) {
+        const float4 r = *(const float4*)(rv + a * 16 + 4 * g);
+        const f32x4 v = f32x4{acc[a][0] * r.x * sinv, acc[a][1] * r.y * sinv, acc[a][2] * r.z * sinv,
+                              acc[a][3] * r.w * sinv};
+        __builtin_nontemporal_store(v, (f32x4*)(out + a * 16 + 4 * g));
+    }
+}
+
+// per multipole column c (node * 3 + component): the power of 2 s_c that brings
+// max_i |M[c][i] dm[i]| into [0.5, 1) (1 for a zero column); one wave per column
+__global__ __launch_bounds__(256) void k_column_scales(const float* __restrict__ M, int ncols, int L, int Lp,
+                                                       const float* __restrict__ dm, float* __restrict__ scale) {
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (c >= ncols) return;
+    const float* col = M + (size_t)c * Lp;
+    float mx = 0.0f;
+    for (int i = lane; i < L; i += 64) mx = fmaxf(mx, fabsf(col[i] * dm[i]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if (lane == 0) scale[c] = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
+}
+
+void column_scales(const float* M, int ncols, int L, int Lp, const float* dm, float* scale, hipStream_t st) {
+    if (ncols <= 0) return;
+    k_column_scales<<<(ncols + 3) / 4, 256, 0, st>>>(M, ncols, L, Lp, dm, scale);
+    FMM_HIP(hipGetLastError());
+}
+
+int m2l_f16_rows(int L) { return 16 * ((L + 15) / 16); }
+int m2l_f16_kpad(int L) { return 32 * ((L + 31) / 32); }
+
+void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv, int op_base,
+                    const GemmTile* d_tiles, int ntiles, const int* pair_src, const float* src_exp, const float* dm,
+                    const float* cscale, float* slab, int L, int Lp, hipStream_t st) {
+    if (ntiles == 0) return;
+    const int nb = m2l_f16_rows(L) / 16, nks = m2l_f16_kpad(L) / 32;
+    if (m2l_f16_kpad(L) > Lp) throw Error(4, "FP16 M2L: K padding beyond the expansion length");
+#define FMM_F16(NBV, NKV)                                                                                         \
+    if (nb == NBV && nks == NKV) {                                                                                \
+        k_gemm_m2l_f16x3<NBV, NKV><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_exp, dm, \
+                                                           cscale, slab, Lp, op_base);                            \
+        FMM_HIP(hipGetLastError());                                                                               \
+        return;                                                                                                   \
+    }
+    FMM_F16(1, 1) FMM_F16(2, 1) FMM_F16(3, 2) FMM_F16(4, 2) FMM_F16(4, 3) FMM_F16(5, 3) FMM_F16(6, 3) FMM_F16(7, 4)
+    FMM_F16(8, 4) FMM_F16(9, 5) FMM_F16(10, 5) FMM_F16(11, 6) FMM_F16(12, 6) FMM_F16(13, 7) FMM_F16(14, 7)
+    FMM_F16(15, 8) FMM_F16(16, 8) FMM_F16(17, 9) FMM_F16(18, 9) FMM_F16(19, 10) FMM_F16(20, 10) FMM_F16(21, 11)
+    FMM_F16(22, 11) FMM_F16(23, 12) FMM_F16(24, 12) FMM_F16(25, 13) FMM_F16(26, 13) FMM_F16(27, 14) FMM_F16(28, 14)
+#undef FMM_F16
+    throw Error(4, "FP16 M2L: no kernel instance for this order");
+}
+
 template <int NB>
 static void launch_m2l_nb(Precision prec, const void* ops, const GemmTile* d_tiles, int ntiles, const int* pair_src,
                           const void* src_exp, void* slab, int Lp, hipStream_t st) {
@@ -423,6 +593,7 @@ static void launch_m2l_nb(Precision prec, const void* ops, const GemmTile* d_til
 
 int m2l_slab_rows(Precision prec, int L, int Lp) {
     if (prec == Precision::BF16X3) return Lp;
+    if (prec == Precision::F16X3) return m2l_f16_rows(L);
     int nb = (L + 15) / 16;
     switch (nb) {
         case 1: case 2: case 3: case 4: case 6: case 7: case 8: case 9: case 11: case 13: case 15: case 16: case 19:

@@ -3,7 +3,7 @@
 
 namespace fmm {
 
-enum class Precision { F64 = 0, F32 = 1, BF16X3 = 2 };
+enum class Precision { F64 = 0, F32 = 1, BF16X3 = 2, F16X3 = 3 };
 
 struct GemmTile {
     int op;      // operator index
@@ -44,11 +44,23 @@ void build_pair_index(const int* key, int npairs, int nnodes, PairIndex& ix, hip
 void reduce_pairs(bool f64, PairIndex& ix, const void* slab, int R, int Lp, void* out, hipStream_t st);
 
 // Row length of the M2L slab: L padded to 16 (MFMA block) where a kernel instance exists for it
-// (FP32/FP64), else Lp (BF16X3, whose K step is 32, and orders without an instance).
+// (FP32/FP64, F16X3), else Lp (BF16X3, whose K step is 32, and orders without an instance).
 int m2l_slab_rows(Precision prec, int L, int Lp);
 // M2L of all classes into the slab [3 * pairs][m2l_slab_rows] (column 3 pair + component).
 void m2l_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
               const int* pair_src, const void* src_exp, void* slab, int L, int Lp, hipStream_t st);
+
+// F16X3 M2L (FP16 split products on MFMA, equilibrated by powers of 2): operator rows
+// m2l_f16_rows(L), K padded to m2l_f16_kpad(L); src_exp the FP32 multipoles [node][3][Lp],
+// dm[>= kpad] the per-coefficient scales the tables were built with, cscale[node * 3 + comp]
+// from column_scales; the slab rows are m2l_f16_rows(L) long.  Tile operator k is table entry
+// k - op_base.
+int m2l_f16_rows(int L);
+int m2l_f16_kpad(int L);
+void column_scales(const float* M, int ncols, int L, int Lp, const float* dm, float* scale, hipStream_t st);
+void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv, int op_base,
+                    const GemmTile* d_tiles, int ntiles, const int* pair_src, const float* src_exp, const float* dm,
+                    const float* cscale, float* slab, int L, int Lp, hipStream_t st);
 
 void plan_tiles(const std::vector<int>& class_begin, const std::vector<int>& class_op, std::vector<GemmTile>& tiles);
 

@@ -59,9 +59,11 @@ __device__ __forceinline__ void lm_of(int i, int& l, int& m) {
 
 // One workgroup per operator.  LDS: the harmonic table X(n, k>=0), the degree/order of every
 // coefficient index and the per-degree scale factor of the operator kind.
-template <int MODE>   // output: 0 = double, 1 = float, 2 = bf16 hi + bf16 lo (of the float value)
+// output: 0 = double, 1 = float, 2 = bf16 hi + bf16 lo (of the float value), 3 = the FP16 split
+// of the equilibrated operator (see F16Split; `out` is then the FP32 staging table)
+template <int MODE>
 __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ specs, int p, int pI_full, int Lp,
-                                                  void* __restrict__ out, __bf16* __restrict__ out_lo) {
+                                                  void* __restrict__ out, __bf16* __restrict__ out_lo, F16Split f16) {
     extern __shared__ cplx X[];   // packed (n, k>=0), n <= pH
     const OpSpec sp = specs[blockIdx.x];
     const int kind = sp.kind;
@@ -113,6 +115,11 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
     __syncthreads();
     const double c = 0.70710678118654752440;
     const size_t base = (size_t)blockIdx.x * Lp * Lp;
+    __shared__ unsigned rmax[MODE == 3 ? 512 : 1];   // FP16 split: max |entry| of each row (float bits)
+    if (MODE == 3) {
+        for (int r = threadIdx.x; r < f16.R; r += blockDim.x) rmax[r] = 0u;
+        __syncthreads();
+    }
     // element e = i Lp + j; (i, j) advanced by the stride instead of divided out (blockDim < Lp * Lp)
     int i = threadIdx.x / Lp, j = threadIdx.x - (threadIdx.x / Lp) * Lp;
     const int di = blockDim.x / Lp, dj = blockDim.x - di * Lp;
@@ -141,24 +148,61 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
             ((double*)out)[base + e] = val;
         } else if (MODE == 1) {
             ((float*)out)[base + e] = (float)val;
-        } else {
+        } else if (MODE == 2) {
             float v = (float)val;
             __bf16 h = (__bf16)v;
             ((__bf16*)out)[base + e] = h;
             out_lo[base + e] = (__bf16)(v - (float)h);
+        } else {
+            // column j of the operator divided by the input coefficient's scale (a power of 2)
+            const float v = j < L ? (float)(val * (double)(1.0f / f16.dm[j])) : 0.0f;   // exact: dm is 2^k
+            ((float*)out)[base + e] = v;
+            // the 64 lanes of a wave hold 64 consecutive entries of one row (Lp is a multiple
+            // of 64 and the stride 256): one LDS atomic per wave and row
+            unsigned b = __float_as_uint(fabsf(v));
+            for (int off = 32; off > 0; off >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, off));
+            if ((threadIdx.x & 63) == 0 && i < f16.R && b) atomicMax(&rmax[i], b);
+        }
+    }
+    if (MODE == 3) {
+        // rows scaled by powers of 2 to a maximum in [0.5, 1), then split hi + lo in FP16; the
+        // FP32 staging entries written above by other threads of this workgroup are visible
+        // after the barrier (global writes and reads of one workgroup)
+        __threadfence_block();
+        __syncthreads();
+        const int R = f16.R, Kp = f16.Kp;
+        const size_t ob = (size_t)blockIdx.x * R * Kp;
+        int i2 = threadIdx.x / Kp, j2 = threadIdx.x - (threadIdx.x / Kp) * Kp;
+        const int di2 = blockDim.x / Kp, dj2 = blockDim.x - di2 * Kp;
+        for (int e = threadIdx.x; e < R * Kp; e += blockDim.x, i2 += di2, j2 += dj2) {
+            if (j2 >= Kp) { j2 -= Kp; i2++; }
+            const int i = i2, j = j2;
+            const float mx = __uint_as_float(rmax[i]);
+            const float rs = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
+            const float v = (i < Lp && j < Lp) ? ((const float*)out)[base + (size_t)i * Lp + j] * rs : 0.0f;
+            const _Float16 h = (_Float16)v;
+            f16.hi[ob + e] = h;
+            f16.lo[ob + e] = (_Float16)(v - (float)h);
+            if (j == 0) f16.rinv[(size_t)blockIdx.x * R + i] = 1.0f / rs;
         }
     }
 }
 
 void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, int out_mode, void* d_out, __bf16* d_out_lo,
-                    hipStream_t st) {
+                    hipStream_t st, const F16Split* f16) {
     if (nops == 0) return;
     int pI = 2 * p, L = (p + 1) * (p + 1);
     size_t lds = (size_t)(pI + 1) * (pI + 2) / 2 * sizeof(cplx) + (size_t)((L + 1) & ~1) * sizeof(short2) +
                  (size_t)(p + 1) * sizeof(double);
-    if (out_mode == 0) k_make_ops<0><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr);
-    else if (out_mode == 1) k_make_ops<1><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr);
-    else k_make_ops<2><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, d_out_lo);
+    const F16Split none{};
+    if (out_mode == 0) k_make_ops<0><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, none);
+    else if (out_mode == 1) k_make_ops<1><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, none);
+    else if (out_mode == 2) k_make_ops<2><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, d_out_lo, none);
+    else {
+        if (!f16 || f16->R > 512 || f16->R > Lp || f16->Kp > Lp || f16->R < L || f16->Kp < L)
+            throw Error(4, "FP16 split tables: bad row/column padding");
+        k_make_ops<3><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, *f16);
+    }
     FMM_HIP(hipGetLastError());
 }
 

@@ -29,7 +29,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 METRIC = "FMM B-field points/s at rel-L2 ≤1e-6 vs FP64 oracle, mixed vs plain FP64"
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}   # dense MFMA; bf16x3 = 3 bf16 GEMMs
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0,   # dense MFMA; bf16x3 = 3 bf16 GEMMs,
+               "f16x3": 2500.0 / 3.0}                                 # f16x3 = 3 fp16 GEMMs
 
 
 def shell_sources(rng, n, r0=0.8, r1=1.0, zh=0.5):
@@ -154,7 +155,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "f16x3"])
     ap.add_argument("--sources", type=int, default=1 << 20)
     ap.add_argument("--targets", type=int, default=1 << 20)
     ap.add_argument("--order", type=int, default=12)

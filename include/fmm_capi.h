@@ -47,6 +47,8 @@ extern "C" {
 #define FMM_PREC_FP64 0        /* translations on FP64 MFMA, FP64 expansions */
 #define FMM_PREC_FP32 1        /* FP32 MFMA, FP32 accumulate */
 #define FMM_PREC_BF16X3 2      /* BF16 split products hi*hi + hi*lo + lo*hi, FP32 accumulate */
+#define FMM_PREC_F16X3 3       /* M2L on FP16 split products (operands equilibrated by powers of 2),
+                                  FP32 accumulate; M2M and L2L in FP32 */
 
 #define FMM_FRAME_REFERENCE 0  /* target root = reference bounding box (math.cpp:39-57) */
 #define FMM_FRAME_SHARED 1     /* target root = source root: M2L operators shared per offset */

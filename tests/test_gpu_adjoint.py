@@ -16,7 +16,7 @@ def _case(seed=21, ns=6000, nt=1500):
     return rng, src, w, tgt
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32", 2e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32", 2e-6), ("f16x3", 2e-6)])
 def test_adjoint_is_the_transpose(precision, tol):
     """B is linear in w, so g.B(w) == w.(adjoint g) for every g and w (exact up to rounding,
     independent of the FMM truncation error)."""

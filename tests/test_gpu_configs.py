@@ -47,8 +47,10 @@ def test_config3_coaxial_shells_p16(orc):
     assert rel_l2(B64, orc.direct_B(src, w, tgt)) < 1e-8        # oracle's own truncation: 2.98e-9
     _, B32 = _gpu(fm, src, w, tgt, precision="fp32", **cfg)
     _, Bbf = _gpu(fm, src, w, tgt, precision="bf16x3", **cfg)
+    _, B16 = _gpu(fm, src, w, tgt, precision="f16x3", **cfg)
     assert rel_l2(B32, B64) < 3e-6
     assert rel_l2(Bbf, B64) < 2e-5
+    assert rel_l2(B16, B64) < 3e-6
 
 
 def test_config5_gaussian_clusters_p14(orc):
@@ -67,5 +69,7 @@ def test_config5_gaussian_clusters_p14(orc):
     assert rel_l2(B64, orc.direct_B(src, w, tgt, J=J)) < 3e-8   # oracle's own truncation: 6.6e-9
     _, B32 = _gpu(fm, src, w, tgt, J=J, precision="fp32", **cfg)
     _, Bbf = _gpu(fm, src, w, tgt, J=J, precision="bf16x3", **cfg)
+    _, B16 = _gpu(fm, src, w, tgt, J=J, precision="f16x3", **cfg)
     assert rel_l2(B32, B64) < 3e-6
     assert rel_l2(Bbf, B64) < 2e-5
+    assert rel_l2(B16, B64) < 3e-6

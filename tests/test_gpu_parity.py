@@ -197,7 +197,7 @@ def test_expansions_fp64(fm, orc, small_case):
     assert checked > 20
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 3e-6), ("bf16x3", 1e-4)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 3e-6), ("bf16x3", 1e-4), ("f16x3", 3e-6)])
 def test_B_matches_oracle(fm, orc, small_case, precision, tol):
     src, w, tgt, R = small_case
     F, B = _gpu_run(fm, src, w, tgt, 8, 32, 32, precision)
@@ -242,7 +242,7 @@ def test_reference_frame_fmm_matches_oracle(fm, orc, precision, tol):
     assert rel_l2(B[ok], Bo[ok]) < tol
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6), ("f16x3", 1e-6)])
 def test_fused_run_matches_staged_run(fm, precision, tol):
     """fmm_run_B_dev (lists and near field on a side stream) equals the stage-by-stage calls; the
     only difference allowed is the order of the atomic M2M additions on narrow levels."""
