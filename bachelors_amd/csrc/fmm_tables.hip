@@ -170,20 +170,31 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
         // after the barrier (global writes and reads of one workgroup)
         __threadfence_block();
         __syncthreads();
-        const int R = f16.R, Kp = f16.Kp;
+        const int R = f16.R, Kp = f16.Kp, G8 = Kp / 8;   // groups of 8 consecutive columns
         const size_t ob = (size_t)blockIdx.x * R * Kp;
-        int i2 = threadIdx.x / Kp, j2 = threadIdx.x - (threadIdx.x / Kp) * Kp;
-        const int di2 = blockDim.x / Kp, dj2 = blockDim.x - di2 * Kp;
-        for (int e = threadIdx.x; e < R * Kp; e += blockDim.x, i2 += di2, j2 += dj2) {
-            if (j2 >= Kp) { j2 -= Kp; i2++; }
-            const int i = i2, j = j2;
+        const float* stg = (const float*)out + base;
+        int i = threadIdx.x / G8, q = threadIdx.x - (threadIdx.x / G8) * G8;
+        const int di = blockDim.x / G8, dq = blockDim.x - di * G8;
+#pragma unroll 4
+        for (int e = threadIdx.x; e < R * G8; e += blockDim.x, i += di, q += dq) {
+            if (q >= G8) { q -= G8; i++; }
             const float mx = __uint_as_float(rmax[i]);
             const float rs = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
-            const float v = (i < Lp && j < Lp) ? ((const float*)out)[base + (size_t)i * Lp + j] * rs : 0.0f;
-            const _Float16 h = (_Float16)v;
-            f16.hi[ob + e] = h;
-            f16.lo[ob + e] = (_Float16)(v - (float)h);
-            if (j == 0) f16.rinv[(size_t)blockIdx.x * R + i] = 1.0f / rs;
+            const float4 a = *(const float4*)(stg + (size_t)i * Lp + 8 * q);
+            const float4 b = *(const float4*)(stg + (size_t)i * Lp + 8 * q + 4);
+            const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+            h8 hi, lo;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float x = v[k] * rs;
+                const _Float16 h = (_Float16)x;
+                hi[k] = h;
+                lo[k] = (_Float16)(x - (float)h);
+            }
+            *(h8*)(f16.hi + ob + (size_t)i * Kp + 8 * q) = hi;
+            *(h8*)(f16.lo + ob + (size_t)i * Kp + 8 * q) = lo;
+            if (q == 0) f16.rinv[(size_t)blockIdx.x * R + i] = 1.0f / rs;
         }
     }
 }
@@ -199,7 +210,7 @@ void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, int out_mode
     else if (out_mode == 1) k_make_ops<1><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, none);
     else if (out_mode == 2) k_make_ops<2><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, d_out_lo, none);
     else {
-        if (!f16 || f16->R > 512 || f16->R > Lp || f16->Kp > Lp || f16->R < L || f16->Kp < L)
+        if (!f16 || f16->R > 512 || f16->R > Lp || f16->Kp > Lp || f16->R < L || f16->Kp < L || f16->Kp % 8)
             throw Error(4, "FP16 split tables: bad row/column padding");
         k_make_ops<3><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, *f16);
     }

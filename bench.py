@@ -8,13 +8,14 @@ shared-frame trees, reference MAC.  One step = one full evaluation from device-r
 inputs to device-resident B in input order: both octrees, interaction lists, FP64
 operator tables, P2M, M2M, M2L, L2L, L2P and near-field P2P.
 
-The mixed-precision run (FP32 MFMA translations by default) gives `value`; the plain
+The mixed-precision run gives `value`: by default F16X3 (the M2L on FP16 split products, hi + lo
+operands equilibrated by powers of 2, FP32 accumulate; M2M and L2L on FP32 MFMA); the plain
 FP64 FMM (FP64 MFMA translations, FP64 P2P) is timed on the same inputs.  rel-L2 of
 each is measured against an FP64 direct sum on the first 16,384 targets.  The CPU
 baseline is the oracle's restatement of the reference CPU FMM path, single thread, on
 a bounded sample of the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|bf16x3]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f16x3|fp32|bf16x3]
 """
 import argparse
 import ctypes
@@ -31,6 +32,10 @@ sys.path.insert(0, HERE)
 METRIC = "FMM B-field points/s at rel-L2 ≤1e-6 vs FP64 oracle, mixed vs plain FP64"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0,   # dense MFMA; bf16x3 = 3 bf16 GEMMs,
                "f16x3": 2500.0 / 3.0}                                 # f16x3 = 3 fp16 GEMMs
+HBM_GBS = 8000.0
+TRANSLATIONS = {"f16x3": "M2L on FP16 split MFMA products (F16X3), M2M/L2L on FP32 MFMA",
+                "fp32": "FP32 MFMA translations", "bf16x3": "BF16 split MFMA translations (BF16X3)",
+                "fp64": "FP64 MFMA translations"}
 
 
 def shell_sources(rng, n, r0=0.8, r1=1.0, zh=0.5):
@@ -126,10 +131,11 @@ def other_stage_rooflines(st, L, args):
 
 def measured_traffic(args):
     """HBM bytes per M2L GEMM launch from the committed rocprofv3 counter summary of the default
-    workload (FETCH_SIZE + WRITE_SIZE, calibrated by a 1 GiB copy); None for other workloads."""
-    default = (args.sources, args.targets, args.order, args.cap, args.precision) == (1 << 20, 1 << 20, 12, 64, "fp32")
+    workload and this precision (FETCH_SIZE + WRITE_SIZE, calibrated by a 1 GiB copy); None for
+    other workloads."""
+    default = (args.sources, args.targets, args.order, args.cap) == (1 << 20, 1 << 20, 12, 64)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "m2l_gemm_hbm_traffic_1M_shell_p12_fp32.md")
+                        "m2l_gemm_hbm_traffic_1M_shell_p12_%s.md" % args.precision)
     if not default or not os.path.exists(path):
         return None
     for line in open(path):
@@ -155,7 +161,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "f16x3"])
+    ap.add_argument("--precision", default="f16x3", choices=["f16x3", "fp32", "bf16x3"])
     ap.add_argument("--sources", type=int, default=1 << 20)
     ap.add_argument("--targets", type=int, default=1 << 20)
     ap.add_argument("--order", type=int, default=12)
@@ -204,6 +210,19 @@ def main():
     m2l_s = st["t_m2l_gemm"] * 1e-3      # the translation GEMM alone (HIP events around its launch)
     achieved = m2l_flops / m2l_s / 1e12 if m2l_s > 0 else 0.0
     peak = PEAK_TFLOPS[args.precision]
+    # algorithmic bytes of the launch: each far pair reads its source multipole and writes its
+    # translated local once (3 x L FP32 values each way), plus one operator per class (L x L
+    # entries of 4 bytes: FP32, or FP16 hi + lo)
+    m2l_bytes = st["far_pairs"] * 2 * 3 * L * 4 + st["m2l_classes"] * L * L * 4
+    gbs = m2l_bytes / m2l_s / 1e9 if m2l_s > 0 else 0.0
+    if gbs / HBM_GBS > achieved / peak:   # the binding resource: HBM bytes or MFMA flops
+        roof = {"bound": "hbm", "kernel": "M2L class-batched GEMM", "achieved": gbs, "peak": HBM_GBS, "unit": "GB/s",
+                "frac": gbs / HBM_GBS, "traffic": measured_traffic(args),
+                "mfma": {"achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak}}
+    else:
+        roof = {"bound": "mfma", "kernel": "M2L class-batched GEMM", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(args),
+                "hbm": {"achieved": gbs, "peak": HBM_GBS, "unit": "GB/s", "frac": gbs / HBM_GBS}}
     out = {
         "metric": METRIC,
         "value": args.targets / (ms * 1e-3),
@@ -218,17 +237,16 @@ def main():
         "dtype": args.precision,
         "data": "synthetic (seeded cylindrical-shell sources, uniform targets)",
         "config": {"workload": "Biot-Savart FMM: %d shell sources (r 0.8-1.0, |z|<=0.5, J=phi-hat, w=1/N), "
-                               "%d uniform targets in [-2,2]^3, p=%d, leaf capacity %d, shared frame, "
-                               "%s MFMA translations, FP64 tables/harmonics/accumulation"
-                               % (args.sources, args.targets, args.order, args.cap, args.precision.upper())},
+                               "%d uniform targets in [-2,2]^3, p=%d, leaf capacity %d, shared frame, %s, "
+                               "FP64 tables/harmonics/accumulation"
+                               % (args.sources, args.targets, args.order, args.cap, TRANSLATIONS[args.precision])},
         "rel_l2_vs_fp64_direct": err,
         "error_subset": sub,
         "finite": finite,
         "stage_ms": {k[2:]: round(v, 4) for k, v in st.items() if k.startswith("t_")},
         "stage_ms_staged": {k[2:]: round(v, 4) for k, v in staged.items() if k.startswith("t_")},
         "counts": {k: st[k] for k in ("source_nodes", "target_nodes", "far_pairs", "close_pairs", "m2l_classes")},
-        "roofline": {"bound": "mfma", "kernel": "M2L class-batched GEMM", "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(args)},
+        "roofline": roof,
         "other_stages": other_stage_rooflines(staged, L, args),
     }
     if not args.no_fp64:
