@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void k_gemm_m2l(const T* __restrict__ ops, con
 // NB row blocks; one LDS buffer refilled from registers whose loads overlap the MFMAs.
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 
-template <int NB, int NKS>
+template <int NB, int NKS, int NC>
 __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restrict__ ops_hi,
                                                         const _Float16* __restrict__ ops_lo,
                                                         const float* __restrict__ ops_rinv,
@@ -435,10 +435,11 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
                                                         int Lp, int op_base) {
     constexpr int R = 16 * NB, Kp = 32 * NKS, BK = 32, S = BK + 8;   // LDS row stride: 80 bytes
     constexpr int NQ = (4 * R + 255) / 256;                          // A: 8-half vectors per thread and part
+    constexpr int W = 64 * NC;                                       // tile columns (NC blocks of 16 per wave)
     __shared__ __attribute__((aligned(16))) _Float16 Ah[R][S];
     __shared__ __attribute__((aligned(16))) _Float16 Al[R][S];
-    __shared__ __attribute__((aligned(16))) _Float16 Bh[BN][S];
-    __shared__ __attribute__((aligned(16))) _Float16 Bl[BN][S];
+    __shared__ __attribute__((aligned(16))) _Float16 Bh[W][S];
+    __shared__ __attribute__((aligned(16))) _Float16 Bl[W][S];
     // each XCD works on a contiguous eighth of the tiles (a bijection of the grid)
     const int nt = gridDim.x, xcd = blockIdx.x & 7, q = nt >> 3, rem = nt & 7;
     const int t = xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
@@ -447,18 +448,23 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
     const int op = tile.op - op_base;   // tile.op numbers the operators of all down passes
     const _Float16* Ahg = ops_hi + (size_t)op * R * Kp;
     const _Float16* Alg = ops_lo + (size_t)op * R * Kp;
-    // B staging: column bc, k = bq .. bq + 7 of each K step
+    // B staging: columns bc + 64 c, k = bq .. bq + 7 of each K step
     const int bc = tid >> 2, bq = (tid & 3) * 8;
-    const float* bptr = nullptr;
-    float bs = 0.0f;
-    if (bc < tile.ncol) {
-        const int col = tile.col0 + bc, pair = col / 3, comp = col - 3 * pair;
-        const int c = pair_src[pair] * 3 + comp;
-        bptr = src_exp + (size_t)c * Lp;
-        bs = cscale[c];
+    const float* bptr[NC];
+    float bs[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        bptr[c] = nullptr;
+        bs[c] = 0.0f;
+        if (bc + 64 * c < tile.ncol) {
+            const int col = tile.col0 + bc + 64 * c, pair = col / 3, comp = col - 3 * pair;
+            const int sc = pair_src[pair] * 3 + comp;
+            bptr[c] = src_exp + (size_t)sc * Lp;
+            bs[c] = cscale[sc];
+        }
     }
     half8 rah[NQ], ral[NQ];
-    float4 rb0, rb1;
+    float4 rb0[NC], rb1[NC];
     float4 d0, d1;
     auto gload = [&](int k0) {
 #pragma unroll
@@ -472,10 +478,12 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
         }
         d0 = *(const float4*)(dm + k0 + bq);
         d1 = *(const float4*)(dm + k0 + bq + 4);
-        if (bptr) {
-            rb0 = *(const float4*)(bptr + k0 + bq);
-            rb1 = *(const float4*)(bptr + k0 + bq + 4);
-        }
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+            if (bptr[c]) {
+                rb0[c] = *(const float4*)(bptr[c] + k0 + bq);
+                rb1[c] = *(const float4*)(bptr[c] + k0 + bq + 4);
+            }
     };
     auto lstore = [&]() {
 #pragma unroll
@@ -486,53 +494,68 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
                 *(half8*)&Al[qi >> 2][(qi & 3) * 8] = ral[j];
             }
         }
-        const float v[8] = {rb0.x * d0.x, rb0.y * d0.y, rb0.z * d0.z, rb0.w * d0.w,
-                            rb1.x * d1.x, rb1.y * d1.y, rb1.z * d1.z, rb1.w * d1.w};
-        half8 h, l;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const float x = bptr ? v[i] * bs : 0.0f;
-            const _Float16 hh = (_Float16)x;
-            h[i] = hh;
-            l[i] = (_Float16)(x - (float)hh);
+        for (int c = 0; c < NC; c++) {
+            const float v[8] = {rb0[c].x * d0.x, rb0[c].y * d0.y, rb0[c].z * d0.z, rb0[c].w * d0.w,
+                                rb1[c].x * d1.x, rb1[c].y * d1.y, rb1[c].z * d1.z, rb1[c].w * d1.w};
+            half8 h, l;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const float x = bptr[c] ? v[i] * bs[c] : 0.0f;
+                const _Float16 hh = (_Float16)x;
+                h[i] = hh;
+                l[i] = (_Float16)(x - (float)hh);
+            }
+            *(half8*)&Bh[bc + 64 * c][bq] = h;
+            *(half8*)&Bl[bc + 64 * c][bq] = l;
         }
-        *(half8*)&Bh[bc][bq] = h;
-        *(half8*)&Bl[bc][bq] = l;
     };
-    f32x4 acc[NB];
+    f32x4 acc[NB][NC];
 #pragma unroll
-    for (int a = 0; a < NB; a++) acc[a] = f32x4{0, 0, 0, 0};
+    for (int a = 0; a < NB; a++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[a][c] = f32x4{0, 0, 0, 0};
     gload(0);
     for (int kt = 0; kt < NKS; kt++) {
         lstore();
         __syncthreads();
         if (kt + 1 < NKS) gload((kt + 1) * BK);
         const int kk = 8 * g;
-        const half8 bh = *(const half8*)&Bh[wave * 16 + li][kk];
-        const half8 bl = *(const half8*)&Bl[wave * 16 + li][kk];
+        half8 bh[NC], bl[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            bh[c] = *(const half8*)&Bh[wave * 16 * NC + 16 * c + li][kk];
+            bl[c] = *(const half8*)&Bl[wave * 16 * NC + 16 * c + li][kk];
+        }
 #pragma unroll
         for (int a = 0; a < NB; a++) {
             const half8 ah = *(const half8*)&Ah[a * 16 + li][kk];
             const half8 al = *(const half8*)&Al[a * 16 + li][kk];
-            acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[a], 0, 0, 0);
-            acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[a], 0, 0, 0);
-            acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[a], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[c], acc[a][c], 0, 0, 0);
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[c], acc[a][c], 0, 0, 0);
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[c], acc[a][c], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
     // epilogue: undo the scales; the slab is written once and read once (nontemporal)
-    const int cl = wave * 16 + li;
-    if (cl >= tile.ncol) return;
-    const int col = tile.col0 + cl, pair = col / 3, comp = col - 3 * pair;
-    const float sinv = 1.0f / cscale[pair_src[pair] * 3 + comp];
     const float* rv = ops_rinv + (size_t)op * R;
-    float* out = slab + (size_t)col * R;
 #pragma unroll
-    for (int a = 0; a < NB; a++) {
-        const float4 r = *(const float4*)(rv + a * 16 + 4 * g);
-        const f32x4 v = f32x4{acc[a][0] * r.x * sinv, acc[a][1] * r.y * sinv, acc[a][2] * r.z * sinv,
-                              acc[a][3] * r.w * sinv};
-        __builtin_nontemporal_store(v, (f32x4*)(out + a * 16 + 4 * g));
+    for (int c = 0; c < NC; c++) {
+        const int cl = wave * 16 * NC + 16 * c + li;
+        if (cl >= tile.ncol) continue;
+        const int col = tile.col0 + cl, pair = col / 3, comp = col - 3 * pair;
+        const float sinv = 1.0f / cscale[pair_src[pair] * 3 + comp];
+        float* out = slab + (size_t)col * R;
+#pragma unroll
+        for (int a = 0; a < NB; a++) {
+            const float4 r = *(const float4*)(rv + a * 16 + 4 * g);
+            const f32x4 v = f32x4{acc[a][c][0] * r.x * sinv, acc[a][c][1] * r.y * sinv, acc[a][c][2] * r.z * sinv,
+                                  acc[a][c][3] * r.w * sinv};
+            __builtin_nontemporal_store(v, (f32x4*)(out + a * 16 + 4 * g));
+        }
     }
 }
 
@@ -559,17 +582,22 @@ int m2l_f16_rows(int L) { return 16 * ((L + 15) / 16); }
 int m2l_f16_kpad(int L) { return 32 * ((L + 31) / 32); }
 
 void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv, int op_base,
-                    const GemmTile* d_tiles, int ntiles, const int* pair_src, const float* src_exp, const float* dm,
-                    const float* cscale, float* slab, int L, int Lp, hipStream_t st) {
+                    const GemmTile* d_tiles, int ntiles, int tile_cols, const int* pair_src, const float* src_exp,
+                    const float* dm, const float* cscale, float* slab, int L, int Lp, hipStream_t st) {
     if (ntiles == 0) return;
     const int nb = m2l_f16_rows(L) / 16, nks = m2l_f16_kpad(L) / 32;
     if (m2l_f16_kpad(L) > Lp) throw Error(4, "FP16 M2L: K padding beyond the expansion length");
-#define FMM_F16(NBV, NKV)                                                                                         \
-    if (nb == NBV && nks == NKV) {                                                                                \
-        k_gemm_m2l_f16x3<NBV, NKV><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_exp, dm, \
-                                                           cscale, slab, Lp, op_base);                            \
-        FMM_HIP(hipGetLastError());                                                                               \
-        return;                                                                                                   \
+    if (tile_cols != 64 && tile_cols != 128) throw Error(4, "FP16 M2L: tiles of 64 or 128 columns");
+#define FMM_F16(NBV, NKV)                                                                                            \
+    if (nb == NBV && nks == NKV) {                                                                                   \
+        if (tile_cols == 128)                                                                                        \
+            k_gemm_m2l_f16x3<NBV, NKV, 2><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src,       \
+                                                                  src_exp, dm, cscale, slab, Lp, op_base);           \
+        else                                                                                                         \
+            k_gemm_m2l_f16x3<NBV, NKV, 1><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src,       \
+                                                                  src_exp, dm, cscale, slab, Lp, op_base);           \
+        FMM_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                      \
     }
     FMM_F16(1, 1) FMM_F16(2, 1) FMM_F16(3, 2) FMM_F16(4, 2) FMM_F16(4, 3) FMM_F16(5, 3) FMM_F16(6, 3) FMM_F16(7, 4)
     FMM_F16(8, 4) FMM_F16(9, 5) FMM_F16(10, 5) FMM_F16(11, 6) FMM_F16(12, 6) FMM_F16(13, 7) FMM_F16(14, 7)
@@ -673,12 +701,13 @@ void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmT
 }
 
 // host-side tiling of class column ranges: class c owns pairs [begin[c], begin[c+1])
-void plan_tiles(const std::vector<int>& class_begin, const std::vector<int>& class_op, std::vector<GemmTile>& tiles) {
+void plan_tiles(const std::vector<int>& class_begin, const std::vector<int>& class_op, std::vector<GemmTile>& tiles,
+                int width) {
     tiles.clear();
     int nc = (int)class_begin.size() - 1;
     for (int c = 0; c < nc; c++) {
         int c0 = 3 * class_begin[c], c1 = 3 * class_begin[c + 1];
-        for (int j = c0; j < c1; j += BN) tiles.push_back(GemmTile{class_op[c], j, std::min(BN, c1 - j)});
+        for (int j = c0; j < c1; j += width) tiles.push_back(GemmTile{class_op[c], j, std::min(width, c1 - j)});
     }
 }
 
