@@ -120,13 +120,19 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
         for (int r = threadIdx.x; r < f16.R; r += blockDim.x) rmax[r] = 0u;
         __syncthreads();
     }
-    // element e = i Lp + j; (i, j) advanced by the stride instead of divided out (blockDim < Lp * Lp)
-    int i = threadIdx.x / Lp, j = threadIdx.x - (threadIdx.x / Lp) * Lp;
-    const int di = blockDim.x / Lp, dj = blockDim.x - di * Lp;
-    for (int e = threadIdx.x; e < Lp * Lp; e += blockDim.x, i += di, j += dj) {
-        if (j >= Lp) { j -= Lp; i++; }
+    // the L x L entries: e = i L + j, (i, j) advanced by the stride instead of divided out
+    // (blockDim < L * L for p >= 15 is not required: the row index may advance by several).  The
+    // trip count is uniform over the workgroup (lanes past the end compute nothing), so a wave
+    // can reduce its row maxima with shuffles.
+    const int LL = L * L;
+    int i = threadIdx.x / L, j = threadIdx.x - (threadIdx.x / L) * L;
+    const int di = blockDim.x / L, dj = blockDim.x - di * L;
+    for (int e0 = 0; e0 < LL; e0 += blockDim.x, i += di, j += dj) {
+        if (j >= L) { j -= L; i++; }
+        const int e = e0 + threadIdx.x;
+        const bool in = e < LL;
         double val = 0;
-        if (i < L && j < L) {
+        if (in) {
             const short2 oi = lmt[i], oj = lmt[j];
             const int l = oi.x, mr = oi.y, a = oj.x, br = oj.y;
             // complex input v(b) for the real unit vector j: one or two complex orders b0, b1
@@ -144,24 +150,55 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
             else { cplx o1 = out_c(-mr), o2 = out_c(mr); val = c * (o1.im - o2.im); }
             val *= fac[kind == OPK_M2M ? l : a];
         }
+        const size_t o = base + (size_t)i * Lp + j;
         if (MODE == 0) {
-            ((double*)out)[base + e] = val;
+            if (in) ((double*)out)[o] = val;
         } else if (MODE == 1) {
-            ((float*)out)[base + e] = (float)val;
+            if (in) ((float*)out)[o] = (float)val;
         } else if (MODE == 2) {
             float v = (float)val;
             __bf16 h = (__bf16)v;
-            ((__bf16*)out)[base + e] = h;
-            out_lo[base + e] = (__bf16)(v - (float)h);
+            if (in) {
+                ((__bf16*)out)[o] = h;
+                out_lo[o] = (__bf16)(v - (float)h);
+            }
         } else {
             // column j of the operator divided by the input coefficient's scale (a power of 2)
-            const float v = j < L ? (float)(val * (double)(1.0f / f16.dm[j])) : 0.0f;   // exact: dm is 2^k
-            ((float*)out)[base + e] = v;
-            // the 64 lanes of a wave hold 64 consecutive entries of one row (Lp is a multiple
-            // of 64 and the stride 256): one LDS atomic per wave and row
-            unsigned b = __float_as_uint(fabsf(v));
-            for (int off = 32; off > 0; off >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, off));
-            if ((threadIdx.x & 63) == 0 && i < f16.R && b) atomicMax(&rmax[i], b);
+            const float v = in ? (float)(val * (double)(1.0f / f16.dm[j])) : 0.0f;   // exact: dm is 2^k
+            if (in) ((float*)out)[o] = v;
+            const unsigned b = __float_as_uint(fabsf(v));
+            if (L >= 64) {
+                // a wave's 64 consecutive entries span at most two rows: one shuffle reduction and
+                // one LDS atomic per row
+                // (lanes past the end carry 0, whatever their row index)
+                const int lane = threadIdx.x & 63;
+                const int ia = __shfl(i, 0), ib = ia + 1;
+                unsigned ba = i == ia ? b : 0u, bb = i == ib ? b : 0u;
+                for (int off = 32; off > 0; off >>= 1) {
+                    ba = max(ba, (unsigned)__shfl_xor((int)ba, off));
+                    bb = max(bb, (unsigned)__shfl_xor((int)bb, off));
+                }
+                if (lane == 0 && ba && ia < L) atomicMax(&rmax[ia], ba);
+                if (lane == 0 && bb && ib < L) atomicMax(&rmax[ib], bb);
+            } else if (in && b) {
+                atomicMax(&rmax[i], b);
+            }
+        }
+    }
+    if (MODE != 3) {
+        // zero padding: rows L .. Lp-1, then columns L .. Lp-1 of rows < L
+        const int np = Lp - L;
+        for (int e = L * Lp + threadIdx.x; e < Lp * Lp; e += blockDim.x) {
+            if (MODE == 0) ((double*)out)[base + e] = 0.0;
+            else if (MODE == 1) ((float*)out)[base + e] = 0.0f;
+            else { ((__bf16*)out)[base + e] = (__bf16)0.0f; out_lo[base + e] = (__bf16)0.0f; }
+        }
+        for (int e = threadIdx.x; e < L * np; e += blockDim.x) {
+            const int r = e / np;
+            const size_t o = base + (size_t)r * Lp + L + (e - r * np);
+            if (MODE == 0) ((double*)out)[o] = 0.0;
+            else if (MODE == 1) ((float*)out)[o] = 0.0f;
+            else { ((__bf16*)out)[o] = (__bf16)0.0f; out_lo[o] = (__bf16)0.0f; }
         }
     }
     if (MODE == 3) {
@@ -182,7 +219,10 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
             const float rs = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
             const float4 a = *(const float4*)(stg + (size_t)i * Lp + 8 * q);
             const float4 b = *(const float4*)(stg + (size_t)i * Lp + 8 * q + 4);
-            const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int k = 0; k < 8; k++)   // the staging holds the L x L block only
+                if (i >= L || 8 * q + k >= L) v[k] = 0.0f;
             typedef _Float16 h8 __attribute__((ext_vector_type(8)));
             h8 hi, lo;
 #pragma unroll
