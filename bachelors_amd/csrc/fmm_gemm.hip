@@ -560,21 +560,24 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
 }
 
 // per multipole column c (node * 3 + component): the power of 2 s_c that brings
-// max_i |M[c][i] dm[i]| into [0.5, 1) (1 for a zero column); one wave per column
+// max_i |M[c][i] dm[i]| into [0.5, 1) (1 for a zero column); a wave per column, the waves of a
+// capped grid striding over the columns
 __global__ __launch_bounds__(256) void k_column_scales(const float* __restrict__ M, int ncols, int L, int Lp,
                                                        const float* __restrict__ dm, float* __restrict__ scale) {
-    const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (c >= ncols) return;
-    const float* col = M + (size_t)c * Lp;
-    float mx = 0.0f;
-    for (int i = lane; i < L; i += 64) mx = fmaxf(mx, fabsf(col[i] * dm[i]));
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-    if (lane == 0) scale[c] = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
+    const int lane = threadIdx.x & 63;
+    for (int c = blockIdx.x * 4 + (threadIdx.x >> 6); c < ncols; c += gridDim.x * 4) {
+        const float* col = M + (size_t)c * Lp;
+        float mx = 0.0f;
+        for (int i = lane; i < L; i += 64) mx = fmaxf(mx, fabsf(col[i] * dm[i]));
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+        if (lane == 0) scale[c] = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
+    }
 }
 
 void column_scales(const float* M, int ncols, int L, int Lp, const float* dm, float* scale, hipStream_t st) {
     if (ncols <= 0) return;
-    k_column_scales<<<(ncols + 3) / 4, 256, 0, st>>>(M, ncols, L, Lp, dm, scale);
+    const int grid = std::min((ncols + 3) / 4, 8 * num_cus());
+    k_column_scales<<<grid, 256, 0, st>>>(M, ncols, L, Lp, dm, scale);
     FMM_HIP(hipGetLastError());
 }
 

@@ -8,6 +8,8 @@
 // and written for scaled expansions (multipoles divided by sigma^degree, locals
 // multiplied by sigma^(degree+1)), which makes every operator of the shared frame
 // independent of the level and keeps FP32/BF16 magnitudes uniform across degrees.
+#include <algorithm>
+
 #include "fmm_tables.hpp"
 #include "fmm_recip.hpp"
 
@@ -57,15 +59,18 @@ __device__ __forceinline__ void lm_of(int i, int& l, int& m) {
     m = i - l * l - l;
 }
 
-// One workgroup per operator.  LDS: the harmonic table X(n, k>=0), the degree/order of every
-// coefficient index and the per-degree scale factor of the operator kind.
+// nsplit workgroups per operator, each owning a contiguous range of output rows (so a launch
+// of few operators still fills the GPU).  LDS: the harmonic table X(n, k>=0), the degree/order
+// of every coefficient index and the per-degree scale factor of the operator kind.
 // output: 0 = double, 1 = float, 2 = bf16 hi + bf16 lo (of the float value), 3 = the FP16 split
 // of the equilibrated operator (see F16Split; `out` is then the FP32 staging table)
 template <int MODE>
 __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ specs, int p, int pI_full, int Lp,
-                                                  void* __restrict__ out, __bf16* __restrict__ out_lo, F16Split f16) {
+                                                  void* __restrict__ out, __bf16* __restrict__ out_lo, F16Split f16,
+                                                  int nsplit) {
     extern __shared__ cplx X[];   // packed (n, k>=0), n <= pH
-    const OpSpec sp = specs[blockIdx.x];
+    const int opi = blockIdx.x / nsplit, slice = blockIdx.x - opi * nsplit;
+    const OpSpec sp = specs[opi];
     const int kind = sp.kind;
     const int pH = (kind == OPK_M2L) ? (sp.m2l_full ? pI_full : p) : p;
     const int pI = pH;
@@ -114,18 +119,22 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
         fac[n] = kind == OPK_M2M ? pow(ratio, n) : kind == OPK_L2L ? pow(1.0 / ratio, n + 1) : pow(ratio, n);
     __syncthreads();
     const double c = 0.70710678118654752440;
-    const size_t base = (size_t)blockIdx.x * Lp * Lp;
+    const size_t base = (size_t)opi * Lp * Lp;
+    // output rows of this workgroup: [r0, r1) of Rout (Lp, or the FP16 split's R)
+    const int Rout = MODE == 3 ? f16.R : Lp, rsl = (Rout + nsplit - 1) / nsplit;
+    const int r0 = min(Rout, slice * rsl), r1 = min(Rout, r0 + rsl);
+    const int c0 = min(r0, L), c1 = min(r1, L);   // computed rows (< L)
     __shared__ unsigned rmax[MODE == 3 ? 512 : 1];   // FP16 split: max |entry| of each row (float bits)
     if (MODE == 3) {
         for (int r = threadIdx.x; r < f16.R; r += blockDim.x) rmax[r] = 0u;
         __syncthreads();
     }
-    // the L x L entries: e = i L + j, (i, j) advanced by the stride instead of divided out
-    // (blockDim < L * L for p >= 15 is not required: the row index may advance by several).  The
-    // trip count is uniform over the workgroup (lanes past the end compute nothing), so a wave
-    // can reduce its row maxima with shuffles.
-    const int LL = L * L;
-    int i = threadIdx.x / L, j = threadIdx.x - (threadIdx.x / L) * L;
+    // the entries (i < L, j < L) of rows [c0, c1): e = (i - c0) L + j, (i, j) advanced by the
+    // stride instead of divided out (the row index may advance by several).  The trip count is
+    // uniform over the workgroup (lanes past the end compute nothing), so a wave can reduce its
+    // row maxima with shuffles.
+    const int LL = (c1 - c0) * L;
+    int i = c0 + threadIdx.x / L, j = threadIdx.x - (threadIdx.x / L) * L;
     const int di = blockDim.x / L, dj = blockDim.x - di * L;
     for (int e0 = 0; e0 < LL; e0 += blockDim.x, i += di, j += dj) {
         if (j >= L) { j -= L; i++; }
@@ -186,16 +195,16 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
         }
     }
     if (MODE != 3) {
-        // zero padding: rows L .. Lp-1, then columns L .. Lp-1 of rows < L
-        const int np = Lp - L;
-        for (int e = L * Lp + threadIdx.x; e < Lp * Lp; e += blockDim.x) {
+        // zero padding of rows [r0, r1): rows >= L whole, then columns L .. Lp-1 of rows < L
+        const int np = Lp - L, z0 = max(r0, L);
+        for (int e = z0 * Lp + threadIdx.x; e < r1 * Lp; e += blockDim.x) {
             if (MODE == 0) ((double*)out)[base + e] = 0.0;
             else if (MODE == 1) ((float*)out)[base + e] = 0.0f;
             else { ((__bf16*)out)[base + e] = (__bf16)0.0f; out_lo[base + e] = (__bf16)0.0f; }
         }
-        for (int e = threadIdx.x; e < L * np; e += blockDim.x) {
-            const int r = e / np;
-            const size_t o = base + (size_t)r * Lp + L + (e - r * np);
+        for (int e = threadIdx.x; e < (c1 - c0) * np; e += blockDim.x) {
+            const int r = c0 + e / np;
+            const size_t o = base + (size_t)r * Lp + L + (e - (r - c0) * np);
             if (MODE == 0) ((double*)out)[o] = 0.0;
             else if (MODE == 1) ((float*)out)[o] = 0.0f;
             else { ((__bf16*)out)[o] = (__bf16)0.0f; out_lo[o] = (__bf16)0.0f; }
@@ -208,12 +217,12 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
         __threadfence_block();
         __syncthreads();
         const int R = f16.R, Kp = f16.Kp, G8 = Kp / 8;   // groups of 8 consecutive columns
-        const size_t ob = (size_t)blockIdx.x * R * Kp;
+        const size_t ob = (size_t)opi * R * Kp;
         const float* stg = (const float*)out + base;
-        int i = threadIdx.x / G8, q = threadIdx.x - (threadIdx.x / G8) * G8;
+        int i = r0 + threadIdx.x / G8, q = threadIdx.x - (threadIdx.x / G8) * G8;
         const int di = blockDim.x / G8, dq = blockDim.x - di * G8;
 #pragma unroll 4
-        for (int e = threadIdx.x; e < R * G8; e += blockDim.x, i += di, q += dq) {
+        for (int e = threadIdx.x; e < (r1 - r0) * G8; e += blockDim.x, i += di, q += dq) {
             if (q >= G8) { q -= G8; i++; }
             const float mx = __uint_as_float(rmax[i]);
             const float rs = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
@@ -234,7 +243,7 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
             }
             *(h8*)(f16.hi + ob + (size_t)i * Kp + 8 * q) = hi;
             *(h8*)(f16.lo + ob + (size_t)i * Kp + 8 * q) = lo;
-            if (q == 0) f16.rinv[(size_t)blockIdx.x * R + i] = 1.0f / rs;
+            if (q == 0) f16.rinv[(size_t)opi * R + i] = 1.0f / rs;
         }
     }
 }
@@ -246,13 +255,16 @@ void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, int out_mode
     size_t lds = (size_t)(pI + 1) * (pI + 2) / 2 * sizeof(cplx) + (size_t)((L + 1) & ~1) * sizeof(short2) +
                  (size_t)(p + 1) * sizeof(double);
     const F16Split none{};
-    if (out_mode == 0) k_make_ops<0><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, none);
-    else if (out_mode == 1) k_make_ops<1><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, none);
-    else if (out_mode == 2) k_make_ops<2><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, d_out_lo, none);
+    // about 2048 workgroups in all, at least 8 rows each
+    const int nsplit = std::max(1, std::min((Lp + 7) / 8, 2048 / nops));
+    const int grid = nops * nsplit;
+    if (out_mode == 0) k_make_ops<0><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, none, nsplit);
+    else if (out_mode == 1) k_make_ops<1><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, none, nsplit);
+    else if (out_mode == 2) k_make_ops<2><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, d_out_lo, none, nsplit);
     else {
         if (!f16 || f16->R > 512 || f16->R > Lp || f16->Kp > Lp || f16->R < L || f16->Kp < L || f16->Kp % 8)
             throw Error(4, "FP16 split tables: bad row/column padding");
-        k_make_ops<3><<<nops, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, *f16);
+        k_make_ops<3><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, *f16, nsplit);
     }
     FMM_HIP(hipGetLastError());
 }
