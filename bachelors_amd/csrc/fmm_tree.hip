@@ -547,6 +547,29 @@ __global__ void k_leaf_keys(const int* __restrict__ node_of, const int* __restri
     if (irregular[q] <= lvl[nd]) *bad = 1;
 }
 
+// In-leaf input order without a second sort: point j (path-sorted position) of leaf nd, whose
+// points are the positions [sb, se), goes to slot sb + (number of the leaf's points with a
+// smaller input index).  flags[0]: a point's ambiguous level is at or above its leaf (the tree
+// must be rebuilt level by level); flags[1]: a leaf holds more than LEAF_RANK_MAX points (the
+// quadratic count is skipped and the stable sort by leaf start orders the slots instead).
+constexpr int LEAF_RANK_MAX = 512;
+__global__ void k_leaf_slots(const int* __restrict__ node_of, const int* __restrict__ order, int n0,
+                             const int* __restrict__ sb, const int* __restrict__ se, const int* __restrict__ lvl,
+                             const int* __restrict__ irregular, int* __restrict__ slots, int* __restrict__ flags) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n0) return;
+    const int nd = node_of[j], q = order[j];
+    if (irregular[q] <= lvl[nd]) flags[0] = 1;
+    const int b = sb[nd], e = se[nd];
+    if (e - b > LEAF_RANK_MAX) {
+        flags[1] = 1;
+        return;
+    }
+    int r = 0;
+    for (int u = b; u < e; u++) r += order[u] < q;
+    slots[b + r] = q;
+}
+
 __global__ void k_count_root(const uint32_t* __restrict__ inroot, const uint32_t* __restrict__ rank, int n,
                              int* __restrict__ out) {
     out[0] = (int)(rank[n - 1] + inroot[n - 1]);
@@ -874,20 +897,26 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
             T.end.reserve((size_t)nn * sizeof(int));
             k_node_ranges<<<blocks(nn), 256, 0, st>>>(T.first_child.as<int>(), S.sb.as<int>(), S.se.as<int>(), nn,
                                                       T.npts.as<int>(), T.begin.as<int>(), T.end.as<int>());
-            // in-leaf input order: stable sort of the (input-ordered) root points by leaf start
-            uint32_t* lkey = S.key.get<uint32_t>(std::max(n0, 1));
-            uint32_t* lkeys = S.key2.get<uint32_t>(std::max(n0, 1));
-            int* bad = S.need.get<int>(4) + 3;
-            FMM_HIP(hipMemsetAsync(bad, 0, sizeof(int), st));
-            if (n0) k_leaf_keys<<<blocks(n0), 256, 0, st>>>(S.node_of.as<int>(), order, n0, rank, S.sb.as<int>(),
-                                                            T.level.as<int>(), irr, lkey, bad);
-            FMM_HIP(hipMemcpyAsync(S.h_last, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+            // in-leaf input order: each leaf's points ranked by input index (a stable sort of
+            // the input-ordered root points by leaf start if a leaf is too large for that)
+            int* slots = T.slots.get<int>(std::max(n0, 1));
+            int* flags = S.need.get<int>(4) + 2;
+            FMM_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(int), st));
+            if (n0) k_leaf_slots<<<blocks(n0), 256, 0, st>>>(S.node_of.as<int>(), order, n0, S.sb.as<int>(),
+                                                             S.se.as<int>(), T.level.as<int>(), irr, slots, flags);
+            FMM_HIP(hipMemcpyAsync(S.h_last, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
             FMM_HIP(hipStreamSynchronize(st));
             if (S.h_last[0] == 0) {
-                int bits = 1;
-                while (bits < 32 && (1u << bits) <= (unsigned)n0) bits++;
-                int* slots = T.slots.get<int>(std::max(n0, 1));
-                if (n0) sort_pairs_u32(lkey, lkeys, idx, slots, n0, bits, S.tmp, st);
+                if (S.h_last[1] && n0) {
+                    uint32_t* lkey = S.key.get<uint32_t>(n0);
+                    uint32_t* lkeys = S.key2.get<uint32_t>(n0);
+                    int* bad = S.need.get<int>(4) + 3;
+                    k_leaf_keys<<<blocks(n0), 256, 0, st>>>(S.node_of.as<int>(), order, n0, rank, S.sb.as<int>(),
+                                                            T.level.as<int>(), irr, lkey, bad);
+                    int bits = 1;
+                    while (bits < 32 && (1u << bits) <= (unsigned)n0) bits++;
+                    sort_pairs_u32(lkey, lkeys, idx, slots, n0, bits, S.tmp, st);
+                }
                 T.num_slots = n0;
                 T.dropped = n - n0;
                 T.sorted_build = true;
