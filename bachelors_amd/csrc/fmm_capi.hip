@@ -1167,6 +1167,8 @@ int fmm_get_stats(fmm_ctx* c, fmm_stats* s) {
             std::vector<int> ct(c->lists.num_close), cs(c->lists.num_close);
             FMM_HIP(hipMemcpy(ct.data(), c->lists.close_t.ptr, ct.size() * sizeof(int), hipMemcpyDeviceToHost));
             FMM_HIP(hipMemcpy(cs.data(), c->lists.close_s.ptr, cs.size() * sizeof(int), hipMemcpyDeviceToHost));
+            fetch_tree_details(c->T);
+            fetch_tree_details(c->S);
             long long tot = 0;
             for (size_t i = 0; i < ct.size(); i++)
                 tot += (long long)(c->T.h_end[ct[i]] - c->T.h_begin[ct[i]]) * (c->S.h_end[cs[i]] - c->S.h_begin[cs[i]]);
@@ -1216,7 +1218,8 @@ int fmm_tree_export(fmm_ctx* c, int which, int* level, int* parent, int* first_c
                     double* center, int* slots) {
     return guard(c, [&] {
         require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built");
-        const TreeDev& T = which ? c->T : c->S;
+        TreeDev& T = which ? c->T : c->S;
+        fetch_tree_details(T);
         int n = T.num_nodes;
         std::memcpy(level, T.h_level.data(), n * sizeof(int));
         std::memcpy(parent, T.h_parent.data(), n * sizeof(int));
@@ -1272,9 +1275,10 @@ int fmm_export_B_parts(fmm_ctx* c, double* Bfar, double* Bnear) {
 int fmm_export_expansion(fmm_ctx* c, int which, int node, double* out) {
     return guard(c, [&] {
         require(which ? c->stage >= ST_DOWN : c->stage >= ST_LEAVES, FMM_ERR_STATE, "expansion not computed");
-        const TreeDev& T = which ? c->T : c->S;
+        TreeDev& T = which ? c->T : c->S;
         require(node >= 0 && node < T.num_nodes, FMM_ERR_ARG, "node out of range");
         FMM_HIP(hipStreamSynchronize(c->stream));
+        fetch_tree_details(T);
         const DevBuf& E = which ? c->Lx : c->M;
         std::vector<double> v(3 * (size_t)c->Lp);
         if (c->f64()) {

@@ -127,8 +127,10 @@ struct TreeDev {
     DevBuf center;                     // double3 per node
     DevBuf parent, first_child, arrivals, npts, begin, end, level;  // int per node
     DevBuf slots;                      // int per slot: input point index
-    PinnedVec<int> h_first_child, h_npts, h_begin, h_end, h_level, h_parent;
+    PinnedVec<int> h_first_child, h_npts;         // host mirrors the evaluation reads
+    PinnedVec<int> h_begin, h_end, h_level, h_parent;   // fetched on demand (fetch_tree_details)
     PinnedVec<double> h_center;
+    bool details_on_host = false;
     int dropped = 0;                   // input points outside the root
     bool sorted_build = false;         // built by the one-sort path (else level by level)
 };

@@ -9,6 +9,12 @@
 
 namespace fmm {
 
+// The octree point sorts take rocPRIM's onesweep path (a histogram pass plus one launch per
+// 8-bit digit): the default switches to a block-sort + merge path, about 20 launches, for up
+// to 2^20 items, which is slower for 32-bit keys at that size.  The pair sorts keep the default.
+typedef rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config, rocprim::default_config, 0>
+    OnesweepSort;
+
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, DevBuf& tmp, hipStream_t st) {
     if (n == 0) return;
     size_t bytes = 0;
@@ -24,6 +30,15 @@ void sort_pairs_u32(const uint32_t* kin, uint32_t* kout, const int* vin, int* vo
     FMM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, (unsigned)end_bit, st));
     void* t = tmp.reserve(bytes);
     FMM_HIP(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, (unsigned)end_bit, st));
+}
+
+void sort_points_u32(const uint32_t* kin, uint32_t* kout, const int* vin, int* vout, size_t n, int end_bit,
+                     DevBuf& tmp, hipStream_t st) {
+    if (n == 0) return;
+    size_t bytes = 0;
+    FMM_HIP(rocprim::radix_sort_pairs<OnesweepSort>(nullptr, bytes, kin, kout, vin, vout, n, 0u, (unsigned)end_bit, st));
+    void* t = tmp.reserve(bytes);
+    FMM_HIP(rocprim::radix_sort_pairs<OnesweepSort>(t, bytes, kin, kout, vin, vout, n, 0u, (unsigned)end_bit, st));
 }
 
 void sort_pairs_u64(const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, size_t n, int end_bit,

@@ -207,24 +207,36 @@ static void set_level_sizes(TreeDev& T, const double root_h[3], bool cubic_scale
     }
 }
 
+// the host mirrors an evaluation needs (node emptiness and children); the rest of the node
+// arrays stay on the device until fetch_tree_details asks for them
 static void copy_host_mirrors(TreeDev& T, hipStream_t st) {
     int num_nodes = T.num_nodes;
     T.h_first_child.resize(num_nodes);
     T.h_npts.resize(num_nodes);
+    T.details_on_host = false;
+    FMM_HIP(hipMemcpyAsync(T.h_first_child.data(), T.first_child.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(T.h_npts.data(), T.npts.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+}
+
+void fetch_tree_details(TreeDev& T) {
+    if (T.details_on_host) return;
+    const int num_nodes = T.num_nodes;
     T.h_begin.resize(num_nodes);
     T.h_end.resize(num_nodes);
     T.h_level.resize(num_nodes);
     T.h_parent.resize(num_nodes);
     T.h_center.resize(3 * (size_t)num_nodes);
-    FMM_HIP(hipMemcpyAsync(T.h_first_child.data(), T.first_child.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_npts.data(), T.npts.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_begin.data(), T.begin.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_end.data(), T.end.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_level.data(), T.level.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_parent.data(), T.parent.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(T.h_center.data(), T.center.ptr, 3 * (size_t)num_nodes * sizeof(double),
-                           hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipStreamSynchronize(st));
+    FMM_HIP(hipDeviceSynchronize());   // the tree may have been built on either stream
+    if (num_nodes) {
+        FMM_HIP(hipMemcpy(T.h_begin.data(), T.begin.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost));
+        FMM_HIP(hipMemcpy(T.h_end.data(), T.end.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost));
+        FMM_HIP(hipMemcpy(T.h_level.data(), T.level.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost));
+        FMM_HIP(hipMemcpy(T.h_parent.data(), T.parent.ptr, num_nodes * sizeof(int), hipMemcpyDeviceToHost));
+        FMM_HIP(hipMemcpy(T.h_center.data(), T.center.ptr, 3 * (size_t)num_nodes * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    }
+    T.details_on_host = true;
 }
 
 // Level-by-level build: every level partitions the points still being refined into the
@@ -593,11 +605,14 @@ __global__ void k_zero_i32(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// A level with its boundaries kept on the device (no host sync), in two grid-wide kernels over
-// LEVEL_WGS contiguous chunks of the nodes [level_begin[k], level_begin[k+1]):
-// k_level_count counts the subdividing nodes of each chunk, k_level_children numbers them
-// (chunk prefix + block scan, i.e. parent order) and makes their 8 children (float centres as
-// the reference, empty sorted ranges).  Children beyond node_cap set flags[1] and none are made.
+// A level with its boundaries kept on the device (no host sync): k_level_children numbers the
+// subdividing nodes of level k (their per-chunk counts come from the level before, chunk prefix
+// + block scan, i.e. parent order) and makes their 8 children (float centres as the
+// reference, empty sorted ranges); k_assign_count moves the points to the new children, gives
+// the children's ranges and counts the subdividing children per chunk of their level.  The
+// chunks are LEVEL_WGS contiguous pieces of the nodes [level_begin[k], level_begin[k+1]).
+// Children beyond node_cap set flags[1] and none are made; a node below the key's depth that
+// would subdivide sets flags[0].
 constexpr int LEVEL_WGS = 256;
 
 __device__ __forceinline__ int block_excl_sum(int v, int& total) {   // 256 threads
@@ -626,42 +641,32 @@ __device__ __forceinline__ void level_chunk(const int* level_begin, int k, int& 
     b = lb + min(c1, c0 + ((int)threadIdx.x + 1) * per_t);
 }
 
-__global__ __launch_bounds__(256) void k_level_count(int k, int depth, int max_depth, int thr,
-                                                     const int* __restrict__ level_begin, int* __restrict__ flags,
-                                                     const int* __restrict__ sb, const int* __restrict__ se,
-                                                     int* __restrict__ partial) {
-    const bool allow = k < max_depth && k < depth, deeper = k < max_depth && k >= depth;
-    int a, b;
-    level_chunk(level_begin, k, a, b);
-    int cnt = 0;
-    for (int i = a; i < b; i++) {
-        bool sub = se[i] - sb[i] >= thr;
-        if (sub && deeper) flags[0] = 1;
-        cnt += (allow && sub) ? 1 : 0;
-    }
-    int total;
-    block_excl_sum(cnt, total);
-    if (threadIdx.x == 0) partial[blockIdx.x] = total;
-}
+// level k may subdivide its nodes / a level-k node that would subdivide needs a longer key
+__device__ __forceinline__ bool level_allows(int k, int depth, int max_depth) { return k < max_depth && k < depth; }
+__device__ __forceinline__ bool level_deeper(int k, int depth, int max_depth) { return k < max_depth && k >= depth; }
 
+// part: subdividing nodes per chunk of level k (filled by the level before); next: the same
+// for level k + 1, cleared here for k_assign_count
 __global__ __launch_bounds__(256) void k_level_children(int k, int depth, int max_depth, int thr, double3 ph,
                                                         int node_cap, int* __restrict__ level_begin,
-                                                        int* __restrict__ flags, const int* __restrict__ partial,
-                                                        double* __restrict__ center, int* __restrict__ first_child,
-                                                        int* __restrict__ parent, int* __restrict__ level,
-                                                        int* __restrict__ sb, int* __restrict__ se) {
-    const bool allow = k < max_depth && k < depth;
+                                                        int* __restrict__ flags, const int* __restrict__ part,
+                                                        int* __restrict__ next, double* __restrict__ center,
+                                                        int* __restrict__ first_child, int* __restrict__ parent,
+                                                        int* __restrict__ level, int* __restrict__ sb,
+                                                        int* __restrict__ se) {
+    const bool allow = level_allows(k, depth, max_depth);
     const int le = level_begin[k + 1];
     // subdividing nodes before this chunk, and in the whole level
     int before = 0, all = 0;
     for (int q = threadIdx.x; q < LEVEL_WGS; q += 256) {
-        int v = partial[q];
+        int v = part[q];
         all += v;
         if (q < (int)blockIdx.x) before += v;
     }
     int tb, ta;
     block_excl_sum(before, tb);
     block_excl_sum(all, ta);
+    if (threadIdx.x == 0) next[blockIdx.x] = 0;
     const bool fits = (long long)le + 8LL * ta <= (long long)node_cap;
     int a, b;
     level_chunk(level_begin, k, a, b);
@@ -694,11 +699,44 @@ __global__ __launch_bounds__(256) void k_level_children(int k, int depth, int ma
     }
 }
 
-// empty children of the subdivided nodes of level k (grid-strided, bounds from the device)
-__global__ void k_fix_empty_dev(const int* __restrict__ first_child, const int* __restrict__ level_begin, int k,
+// every sorted point of a subdivided level-k node moves to its child; runs of equal path
+// prefix give the children's ranges.  The first point of a child's run tests whether the run
+// reaches thr points (the keys are sorted, so it does iff point j + thr - 1 has the same
+// prefix) and counts the child into its chunk of level k + 1.
+template <class K>
+__global__ void k_assign_count(const K* __restrict__ sk, int n0, int k, int depth, int max_depth, int thr,
+                               const int* __restrict__ level_begin, const int* __restrict__ first_child,
+                               int* __restrict__ node_of, int* __restrict__ sb, int* __restrict__ se,
+                               int* __restrict__ next, int* __restrict__ flags) {
+    const int lb1 = level_begin[k + 1], le1 = level_begin[k + 2];
+    if (le1 == lb1) return;   // no node of level k subdivided
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n0) return;
+    const int fc = first_child[node_of[j]];
+    if (fc < 0) return;
+    const int child_level = k + 1, sh = 3 * (depth - child_level);
+    const K pre = sk[j] >> sh;
+    const int c = fc + (int)(pre & (K)7);
+    node_of[j] = c;
+    if (j == 0 || (sk[j - 1] >> sh) != pre) {
+        sb[c] = j;
+        const bool sub = j + thr - 1 < n0 && (sk[j + thr - 1] >> sh) == pre;
+        if (sub && level_deeper(child_level, depth, max_depth)) flags[0] = 1;
+        if (sub && level_allows(child_level, depth, max_depth)) {
+            const int per_wg = (le1 - lb1 + LEVEL_WGS - 1) / LEVEL_WGS;
+            atomicAdd(&next[(c - lb1) / per_wg], 1);
+        }
+    }
+    if (j == n0 - 1 || (sk[j + 1] >> sh) != pre) se[c] = j + 1;
+}
+
+// empty children of every subdivided node, all levels at once (a node's own range is final
+// once the points are assigned; empty nodes never subdivide): begin = end = where they sit
+// between their siblings (DFS position)
+__global__ void k_fix_empty_all(const int* __restrict__ first_child, const int* __restrict__ level_begin, int total_at,
                                 int* __restrict__ sb, int* __restrict__ se) {
-    const int lb = level_begin[k], le = level_begin[k + 1];
-    for (int node = lb + blockIdx.x * blockDim.x + threadIdx.x; node < le; node += gridDim.x * blockDim.x) {
+    const int nn = level_begin[total_at];
+    for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < nn; node += gridDim.x * blockDim.x) {
         int fc = first_child[node];
         if (fc < 0) continue;
         int pos = sb[node];
@@ -710,9 +748,13 @@ __global__ void k_fix_empty_dev(const int* __restrict__ first_child, const int* 
     }
 }
 
-__global__ void k_init_root_dev(double3 c, int n0, double* __restrict__ center, int* __restrict__ parent,
-                                int* __restrict__ level, int* __restrict__ first_child, int* __restrict__ sb,
-                                int* __restrict__ se, int* __restrict__ level_begin, int nlb, int* __restrict__ flags) {
+// the root, the level boundaries, the flags and the chunk counts of level 0 (one workgroup)
+__global__ void k_init_root_dev(double3 c, int n0, int thr, int depth, int max_depth, double* __restrict__ center,
+                                int* __restrict__ parent, int* __restrict__ level, int* __restrict__ first_child,
+                                int* __restrict__ sb, int* __restrict__ se, int* __restrict__ level_begin, int nlb,
+                                int* __restrict__ flags, int* __restrict__ part) {
+    for (int q = threadIdx.x; q < 2 * LEVEL_WGS; q += blockDim.x) part[q] = 0;
+    if (threadIdx.x != 0) return;
     center[0] = c.x;
     center[1] = c.y;
     center[2] = c.z;
@@ -723,7 +765,10 @@ __global__ void k_init_root_dev(double3 c, int n0, double* __restrict__ center, 
     se[0] = n0;
     level_begin[0] = 0;
     for (int i = 1; i < nlb; i++) level_begin[i] = 1;
-    flags[0] = flags[1] = 0;
+    const bool sub = n0 >= thr;
+    flags[0] = (sub && level_deeper(0, depth, max_depth)) ? 1 : 0;
+    flags[1] = 0;
+    part[0] = (sub && level_allows(0, depth, max_depth)) ? 1 : 0;   // the root is chunk 0 of level 0
 }
 
 // Same tree as build_from_sorted, all levels queued at once: the node arrays are sized for
@@ -743,25 +788,27 @@ static int build_from_sorted_queued(TreeDev& T, TreeScratch& S, const K* sk, int
     int* dlb = S.dlevel.get<int>(nlb + 2);
     int* flags = dlb + nlb;
     int* node_of = S.node_of.get<int>(std::max(n0, 1));
-    int* partial = S.partial.get<int>(LEVEL_WGS);
-    k_init_root_dev<<<1, 1, 0, st>>>(make_double3(root_c[0], root_c[1], root_c[2]), n0, T.center.as<double>(),
-                                     T.parent.as<int>(), T.level.as<int>(), T.first_child.as<int>(), S.sb.as<int>(),
-                                     S.se.as<int>(), dlb, nlb, flags);
+    int* part = S.partial.get<int>(2 * LEVEL_WGS);   // chunk counts of two consecutive levels
+    k_init_root_dev<<<1, 256, 0, st>>>(make_double3(root_c[0], root_c[1], root_c[2]), n0, thr, depth, max_depth,
+                                       T.center.as<double>(), T.parent.as<int>(), T.level.as<int>(),
+                                       T.first_child.as<int>(), S.sb.as<int>(), S.se.as<int>(), dlb, nlb, flags, part);
     if (n0) k_zero_i32<<<blocks(n0), 256, 0, st>>>(node_of, n0);
     const int last = std::min(max_depth, depth);   // levels 0..last can hold nodes
     for (int k = 0; k <= last; k++) {
         double3 ph = make_double3(T.half[k][0], T.half[k][1], T.half[k][2]);
-        k_level_count<<<LEVEL_WGS, 256, 0, st>>>(k, depth, max_depth, thr, dlb, flags, S.sb.as<int>(),
-                                                 S.se.as<int>(), partial);
-        k_level_children<<<LEVEL_WGS, 256, 0, st>>>(k, depth, max_depth, thr, ph, (int)node_cap, dlb, flags, partial,
+        int* cur = part + (k & 1) * LEVEL_WGS;
+        int* next = part + ((k + 1) & 1) * LEVEL_WGS;
+        k_level_children<<<LEVEL_WGS, 256, 0, st>>>(k, depth, max_depth, thr, ph, (int)node_cap, dlb, flags, cur, next,
                                                     T.center.as<double>(), T.first_child.as<int>(),
                                                     T.parent.as<int>(), T.level.as<int>(), S.sb.as<int>(),
                                                     S.se.as<int>());
         if (k == last) break;
-        if (n0) k_assign_level<K><<<blocks(n0), 256, 0, st>>>(sk, n0, k + 1, depth, T.first_child.as<int>(), node_of,
-                                                               S.sb.as<int>(), S.se.as<int>());
-        k_fix_empty_dev<<<256, 256, 0, st>>>(T.first_child.as<int>(), dlb, k, S.sb.as<int>(), S.se.as<int>());
+        if (n0)
+            k_assign_count<K><<<blocks(n0), 256, 0, st>>>(sk, n0, k, depth, max_depth, thr, dlb, T.first_child.as<int>(),
+                                                          node_of, S.sb.as<int>(), S.se.as<int>(), next, flags);
     }
+    k_fix_empty_all<<<std::max(1, std::min(blocks(node_cap), 4 * num_cus())), 256, 0, st>>>(
+        T.first_child.as<int>(), dlb, last + 2, S.sb.as<int>(), S.se.as<int>());
     std::vector<int> h(nlb + 2);
     FMM_HIP(hipMemcpyAsync(h.data(), dlb, (nlb + 2) * sizeof(int), hipMemcpyDeviceToHost, st));
     FMM_HIP(hipStreamSynchronize(st));
@@ -870,7 +917,7 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
             uint32_t* k32s = S.key2.get<uint32_t>(n0);
             k_compact_paths<uint32_t><<<blocks(n), 256, 0, st>>>(key, inroot, rank, n, 3 * (depth - short_depth), k32,
                                                                  idx);
-            sort_pairs_u32(k32, k32s, idx, order, n0, 3 * short_depth, S.tmp, st);
+            sort_points_u32(k32, k32s, idx, order, n0, 3 * short_depth, S.tmp, st);
             // node bound of the queued build: 8 n0 / thr children per level
             const size_t per_level = 8 * ((size_t)n0 / thr + 1);
             const size_t cap_short = 1 + per_level * (size_t)short_depth;
@@ -915,7 +962,7 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
                                                             T.level.as<int>(), irr, lkey, bad);
                     int bits = 1;
                     while (bits < 32 && (1u << bits) <= (unsigned)n0) bits++;
-                    sort_pairs_u32(lkey, lkeys, idx, slots, n0, bits, S.tmp, st);
+                    sort_points_u32(lkey, lkeys, idx, slots, n0, bits, S.tmp, st);
                 }
                 T.num_slots = n0;
                 T.dropped = n - n0;

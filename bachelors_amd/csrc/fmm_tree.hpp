@@ -19,4 +19,8 @@ struct TreeScratch {
 void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const double root_c[3],
                 const double root_h[3], int thr, int max_depth, bool cubic_scale, hipStream_t st);
 
+// copies the node ranges, levels, parents and centres to the host mirrors (once per build;
+// synchronises the device)
+void fetch_tree_details(TreeDev& T);
+
 }  // namespace fmm

@@ -107,6 +107,20 @@ def test_trees_bit_exact_lattice_faces_and_deep_cluster(fm, orc):
     assert st["source_tree_sorted"] == 1 and st["target_tree_sorted"] == 1
 
 
+@pytest.mark.parametrize("max_depth", [1, 3, 5])
+def test_trees_bit_exact_depth_capped(fm, orc, max_depth):
+    """A depth cap leaves nodes above the leaf capacity unsplit."""
+    src, w, tgt = _inputs(11, 20000, 6000)
+    F = fm.FMM(order=4, source_leaf_capacity=20, target_leaf_capacity=24, frame=1, max_depth=max_depth)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    F.build_trees()
+    S = orc.source_tree(src, 20, max_depth=max_depth)
+    assert S.level.max() == max_depth
+    _cmp_tree(F.tree(0), S)
+    _cmp_tree(F.tree(1), orc.target_tree(tgt, 24, root=SHARED_ROOT, max_depth=max_depth))
+
+
 @pytest.mark.parametrize("frame", [0, 1])
 def test_interaction_lists_bit_exact(fm, orc, frame):
     src, w, tgt = _inputs(4, 12000, 4000)
