@@ -62,12 +62,10 @@ __device__ __forceinline__ void lm_of(int i, int& l, int& m) {
 // nsplit workgroups per operator, each owning a contiguous range of output rows (so a launch
 // of few operators still fills the GPU).  LDS: the harmonic table X(n, k>=0), the degree/order
 // of every coefficient index and the per-degree scale factor of the operator kind.
-// output: 0 = double, 1 = float, 2 = bf16 hi + bf16 lo (of the float value), 3 = the FP16 split
-// of the equilibrated operator (see F16Split; `out` is then the FP32 staging table)
+// output: 0 = double, 1 = float, 2 = bf16 hi + bf16 lo (of the float value)
 template <int MODE>
 __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ specs, int p, int pI_full, int Lp,
-                                                  void* __restrict__ out, __bf16* __restrict__ out_lo, F16Split f16,
-                                                  int nsplit) {
+                                                  void* __restrict__ out, __bf16* __restrict__ out_lo, int nsplit) {
     extern __shared__ cplx X[];   // packed (n, k>=0), n <= pH
     const int opi = blockIdx.x / nsplit, slice = blockIdx.x - opi * nsplit;
     const OpSpec sp = specs[opi];
@@ -120,19 +118,12 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
     __syncthreads();
     const double c = 0.70710678118654752440;
     const size_t base = (size_t)opi * Lp * Lp;
-    // output rows of this workgroup: [r0, r1) of Rout (Lp, or the FP16 split's R)
-    const int Rout = MODE == 3 ? f16.R : Lp, rsl = (Rout + nsplit - 1) / nsplit;
-    const int r0 = min(Rout, slice * rsl), r1 = min(Rout, r0 + rsl);
+    // output rows of this workgroup: [r0, r1)
+    const int rsl = (Lp + nsplit - 1) / nsplit;
+    const int r0 = min(Lp, slice * rsl), r1 = min(Lp, r0 + rsl);
     const int c0 = min(r0, L), c1 = min(r1, L);   // computed rows (< L)
-    __shared__ unsigned rmax[MODE == 3 ? 512 : 1];   // FP16 split: max |entry| of each row (float bits)
-    if (MODE == 3) {
-        for (int r = threadIdx.x; r < f16.R; r += blockDim.x) rmax[r] = 0u;
-        __syncthreads();
-    }
     // the entries (i < L, j < L) of rows [c0, c1): e = (i - c0) L + j, (i, j) advanced by the
-    // stride instead of divided out (the row index may advance by several).  The trip count is
-    // uniform over the workgroup (lanes past the end compute nothing), so a wave can reduce its
-    // row maxima with shuffles.
+    // stride instead of divided out (the row index may advance by several)
     const int LL = (c1 - c0) * L;
     int i = c0 + threadIdx.x / L, j = threadIdx.x - (threadIdx.x / L) * L;
     const int di = blockDim.x / L, dj = blockDim.x - di * L;
@@ -164,108 +155,205 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
             if (in) ((double*)out)[o] = val;
         } else if (MODE == 1) {
             if (in) ((float*)out)[o] = (float)val;
-        } else if (MODE == 2) {
+        } else {
             float v = (float)val;
             __bf16 h = (__bf16)v;
             if (in) {
                 ((__bf16*)out)[o] = h;
                 out_lo[o] = (__bf16)(v - (float)h);
             }
+        }
+    }
+    // zero padding of rows [r0, r1): rows >= L whole, then columns L .. Lp-1 of rows < L
+    const int np = Lp - L, z0 = max(r0, L);
+    for (int e = z0 * Lp + threadIdx.x; e < r1 * Lp; e += blockDim.x) {
+        if (MODE == 0) ((double*)out)[base + e] = 0.0;
+        else if (MODE == 1) ((float*)out)[base + e] = 0.0f;
+        else { ((__bf16*)out)[base + e] = (__bf16)0.0f; out_lo[base + e] = (__bf16)0.0f; }
+    }
+    for (int e = threadIdx.x; e < (c1 - c0) * np; e += blockDim.x) {
+        const int r = c0 + e / np;
+        const size_t o = base + (size_t)r * Lp + L + (e - (r - c0) * np);
+        if (MODE == 0) ((double*)out)[o] = 0.0;
+        else if (MODE == 1) ((float*)out)[o] = 0.0f;
+        else { ((__bf16*)out)[o] = (__bf16)0.0f; out_lo[o] = (__bf16)0.0f; }
+    }
+}
+
+constexpr int M2L_F16_CGROUPS = 2;   // column groups of 256 in the split (Kp <= 512)
+
+// M2L operators as FP16 splits (F16Split), one item per (output degree l, order |m|) x (input
+// degree a, order |b|): the complex kernel entries K(+-m, +-b) = f(+-m, +-b) X(l + a, +-m -+ b)
+// take two harmonics, X(n, m - b) and X(n, m + b), and their conjugates, and give the up to four
+// real entries of rows (l, +-m) and columns (a, +-b) with the same operations, in the same
+// order, as the entry-wise kernel.  Workgroup s of an operator owns the row pairs (l, |m|) of
+// packed index [s pps, (s+1) pps), staged in LDS as FP32 rows (scaled by fac[a] and 1 / dm_j),
+// two per pair (+m, then -m; m = 0 leaves the second unused); each wave then takes whole rows:
+// the row maximum by a shuffle reduction, the power-of-2 row scale and the hi + lo split into
+// the [R][Kp] tables.  Rows L .. R-1 (zero, rinv 1) belong to workgroup 0.
+__global__ __launch_bounds__(256) void k_make_m2l_f16(const OpSpec* __restrict__ specs, int p, int pI_full,
+                                                      F16Split f16, int pps, int nsplit) {
+    extern __shared__ cplx X[];   // packed (n, k>=0), n <= pH
+    const int opi = blockIdx.x / nsplit, slice = blockIdx.x - opi * nsplit;
+    const OpSpec sp = specs[opi];
+    const int pH = sp.m2l_full ? pI_full : p;
+    const int L = (p + 1) * (p + 1), NP = (p + 1) * (p + 2) / 2;
+    const int nX = (pI_full + 1) * (pI_full + 2) / 2;
+    short2* tri_lm = (short2*)(X + nX);                  // (n, k) of packed index t (degree <= p)
+    double* fac = (double*)(tri_lm + ((NP + 1) & ~1));   // (sig_s / sig_t)^a
+    float* stg = (float*)(fac + p + 1);                  // the slice's rows, [row][L]
+    const double ux = sp.t[0] / sp.sig_out, uy = sp.t[1] / sp.sig_out, uz = sp.t[2] / sp.sig_out;
+    const double r2 = ux * ux + uy * uy + uz * uz;
+    // column k of the irregular harmonic table, one thread per k (as k_make_ops)
+    for (int k = threadIdx.x; k <= pH; k += blockDim.x) {
+        const double inv = 1.0 / sqrt(r2);
+        cplx d = {inv, 0};   // Psi_k^k = (2k-1)!! (x+iy)^k / r^(2k+1)
+        for (int i = 1; i <= k; i++) d = cscale(cmul(d, {ux, uy}), (2.0 * i - 1) / r2);
+        cplx pm2 = {0, 0}, pm1 = d;
+        X[tri(k, k)] = cscale(d, sgn_pow(k));
+        for (int n = k + 1; n <= pH; n++) {
+            cplx v = cscale(cadd(cscale(pm1, (2.0 * n - 1) * uz), cscale(pm2, -(double)((n - 1) * (n - 1) - k * k))),
+                            1.0 / r2);
+            X[tri(n, k)] = cscale(v, sgn_pow(n));
+            pm2 = pm1;
+            pm1 = v;
+        }
+    }
+    for (int t = threadIdx.x; t < NP; t += blockDim.x) {
+        int n = 0;
+        while ((n + 1) * (n + 2) / 2 <= t) n++;
+        tri_lm[t] = make_short2((short)n, (short)(t - n * (n + 1) / 2));
+    }
+    const double ratio = sp.sig_in / sp.sig_out;
+    for (int n = threadIdx.x; n <= p; n += blockDim.x) fac[n] = pow(ratio, n);
+    __syncthreads();
+    const double c = 0.70710678118654752440;
+    const int rp0 = slice * pps, nrp = min(NP, rp0 + pps) - rp0;
+    for (int e = threadIdx.x; e < nrp * NP; e += blockDim.x) {
+        const int rp = e / NP, q = e - rp * NP;
+        const short2 lm = tri_lm[rp0 + rp], ab = tri_lm[q];
+        const int l = lm.x, m = lm.y, a = ab.x, b = ab.y, n = l + a;
+        // K(s1 m, s2 b), s = +-1: f(m', b') = (-1)^|(|m'-b'| - |m'| - |b'|) / 2| (-1)^a
+        cplx kpp = {0, 0}, kpm = {0, 0}, kmp = {0, 0}, kmm = {0, 0};
+        if (n <= pH) {
+            const cplx A = harm(X, n, m - b), Bq = harm(X, n, m + b);
+            const double sa = sgn_pow(a);
+            kpp = cscale(A, sgn_pow(abs((abs(m - b) - m - b) / 2)) * sa);
+            kpm = cscale(Bq, sgn_pow(abs((abs(m + b) - m - b) / 2)) * sa);
+            kmp = cscale(cplx{Bq.re, -Bq.im}, sgn_pow(abs((abs(-m - b) - m - b) / 2)) * sa);
+            kmm = cscale(cplx{A.re, -A.im}, sgn_pow(abs((abs(b - m) - m - b) / 2)) * sa);
+        }
+        const double fa = fac[a];
+        const int rp_ = 2 * rp, rm_ = 2 * rp + 1;                           // staged rows (l, +-m)
+        const int jp = a * a + a + b, jm = a * a + a - b;                   // columns (a, +-b)
+        auto put = [&](int r, int j, double val) {
+            val *= fa;
+            stg[r * L + j] = (float)(val * (double)(1.0f / f16.dm[j]));   // exact: dm is 2^k
+        };
+        if (b == 0) {   // column (a, 0): out_c(m') = K(m', 0)
+            if (m == 0) put(rp_, jp, kpp.re);
+            else {
+                put(rp_, jp, c * (kpp.re + kmp.re));
+                put(rm_, jp, c * (kpp.im - kmp.im));
+            }
         } else {
-            // column j of the operator divided by the input coefficient's scale (a power of 2)
-            const float v = in ? (float)(val * (double)(1.0f / f16.dm[j])) : 0.0f;   // exact: dm is 2^k
-            if (in) ((float*)out)[o] = v;
-            const unsigned b = __float_as_uint(fabsf(v));
-            if (L >= 64) {
-                // a wave's 64 consecutive entries span at most two rows: one shuffle reduction and
-                // one LDS atomic per row
-                // (lanes past the end carry 0, whatever their row index)
-                const int lane = threadIdx.x & 63;
-                const int ia = __shfl(i, 0), ib = ia + 1;
-                unsigned ba = i == ia ? b : 0u, bb = i == ib ? b : 0u;
-                for (int off = 32; off > 0; off >>= 1) {
-                    ba = max(ba, (unsigned)__shfl_xor((int)ba, off));
-                    bb = max(bb, (unsigned)__shfl_xor((int)bb, off));
-                }
-                if (lane == 0 && ba && ia < L) atomicMax(&rmax[ia], ba);
-                if (lane == 0 && bb && ib < L) atomicMax(&rmax[ib], bb);
-            } else if (in && b) {
-                atomicMax(&rmax[i], b);
+            // column (a, +b): out_c(m') = c K(m', b) + c K(m', -b)
+            // column (a, -b): out_c(m') = (-c Im K(m', b) + c Im K(m', -b), c Re K(m', b) - c Re K(m', -b))
+            const cplx opP = {c * kpp.re + c * kpm.re, c * kpp.im + c * kpm.im};       // m' = m, col +b
+            const cplx opM = {-(c * kpp.im) + c * kpm.im, c * kpp.re + -(c * kpm.re)};  // m' = m, col -b
+            if (m == 0) {
+                put(rp_, jp, opP.re);
+                put(rp_, jm, opM.re);
+            } else {
+                const cplx omP = {c * kmp.re + c * kmm.re, c * kmp.im + c * kmm.im};       // m' = -m, col +b
+                const cplx omM = {-(c * kmp.im) + c * kmm.im, c * kmp.re + -(c * kmm.re)};  // m' = -m, col -b
+                put(rp_, jp, c * (opP.re + omP.re));
+                put(rm_, jp, c * (opP.im - omP.im));
+                put(rp_, jm, c * (opM.re + omM.re));
+                put(rm_, jm, c * (opM.im - omM.im));
             }
         }
     }
-    if (MODE != 3) {
-        // zero padding of rows [r0, r1): rows >= L whole, then columns L .. Lp-1 of rows < L
-        const int np = Lp - L, z0 = max(r0, L);
-        for (int e = z0 * Lp + threadIdx.x; e < r1 * Lp; e += blockDim.x) {
-            if (MODE == 0) ((double*)out)[base + e] = 0.0;
-            else if (MODE == 1) ((float*)out)[base + e] = 0.0f;
-            else { ((__bf16*)out)[base + e] = (__bf16)0.0f; out_lo[base + e] = (__bf16)0.0f; }
+    __syncthreads();
+    // rows: maximum, scale, split.  Lane g takes the columns 4g .. 4g + 3 of each group of 256.
+    typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+    const int R = f16.R, Kp = f16.Kp, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t ob = (size_t)opi * R * Kp;
+    const int nst = 2 * nrp, r_end = nst + (slice == 0 ? R - L : 0);   // workgroup 0 also zeroes rows L .. R-1
+    for (int r = wave; r < r_end; r += 4) {
+        const bool pad = r >= nst;
+        int i = L + (r - nst);
+        if (!pad) {
+            const short2 lm = tri_lm[rp0 + (r >> 1)];
+            if ((r & 1) && lm.y == 0) continue;   // m = 0: one row per pair (uniform over the wave)
+            i = lm.x * lm.x + lm.x + ((r & 1) ? -lm.y : lm.y);
         }
-        for (int e = threadIdx.x; e < (c1 - c0) * np; e += blockDim.x) {
-            const int r = c0 + e / np;
-            const size_t o = base + (size_t)r * Lp + L + (e - (r - c0) * np);
-            if (MODE == 0) ((double*)out)[o] = 0.0;
-            else if (MODE == 1) ((float*)out)[o] = 0.0f;
-            else { ((__bf16*)out)[o] = (__bf16)0.0f; out_lo[o] = (__bf16)0.0f; }
-        }
-    }
-    if (MODE == 3) {
-        // rows scaled by powers of 2 to a maximum in [0.5, 1), then split hi + lo in FP16; the
-        // FP32 staging entries written above by other threads of this workgroup are visible
-        // after the barrier (global writes and reads of one workgroup)
-        __threadfence_block();
-        __syncthreads();
-        const int R = f16.R, Kp = f16.Kp, G8 = Kp / 8;   // groups of 8 consecutive columns
-        const size_t ob = (size_t)opi * R * Kp;
-        const float* stg = (const float*)out + base;
-        int i = r0 + threadIdx.x / G8, q = threadIdx.x - (threadIdx.x / G8) * G8;
-        const int di = blockDim.x / G8, dq = blockDim.x - di * G8;
-#pragma unroll 4
-        for (int e = threadIdx.x; e < (r1 - r0) * G8; e += blockDim.x, i += di, q += dq) {
-            if (q >= G8) { q -= G8; i++; }
-            const float mx = __uint_as_float(rmax[i]);
-            const float rs = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
-            const float4 a = *(const float4*)(stg + (size_t)i * Lp + 8 * q);
-            const float4 b = *(const float4*)(stg + (size_t)i * Lp + 8 * q + 4);
-            float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        float v[M2L_F16_CGROUPS][4];
+        float mx = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 8; k++)   // the staging holds the L x L block only
-                if (i >= L || 8 * q + k >= L) v[k] = 0.0f;
-            typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-            h8 hi, lo;
+        for (int g = 0; g < M2L_F16_CGROUPS; g++)
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const float x = v[k] * rs;
+            for (int k = 0; k < 4; k++) {
+                const int j = 256 * g + 4 * lane + k;
+                v[g][k] = (!pad && j < L) ? stg[r * L + j] : 0.0f;
+                mx = fmaxf(mx, fabsf(v[g][k]));
+            }
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+        const float rs = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
+#pragma unroll
+        for (int g = 0; g < M2L_F16_CGROUPS; g++) {
+            const int j = 256 * g + 4 * lane;
+            if (j >= Kp) continue;
+            h4 hi, lo;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float x = v[g][k] * rs;
                 const _Float16 h = (_Float16)x;
                 hi[k] = h;
                 lo[k] = (_Float16)(x - (float)h);
             }
-            *(h8*)(f16.hi + ob + (size_t)i * Kp + 8 * q) = hi;
-            *(h8*)(f16.lo + ob + (size_t)i * Kp + 8 * q) = lo;
-            if (q == 0) f16.rinv[(size_t)opi * R + i] = 1.0f / rs;
+            *(h4*)(f16.hi + ob + (size_t)i * Kp + j) = hi;
+            *(h4*)(f16.lo + ob + (size_t)i * Kp + j) = lo;
         }
+        if (lane == 0) f16.rinv[(size_t)opi * R + i] = 1.0f / rs;
     }
 }
 
+// row pairs per workgroup of k_make_m2l_f16: as many as the LDS budget holds (two staged
+// rows of L floats each)
+constexpr size_t M2L_F16_LDS = 56 * 1024;
+static size_t m2l_f16_fixed_lds(int p) {
+    const int NP = (p + 1) * (p + 2) / 2, pI = 2 * p;
+    return (size_t)(pI + 1) * (pI + 2) / 2 * sizeof(cplx) + (size_t)((NP + 1) & ~1) * sizeof(short2) +
+           (size_t)(p + 1) * sizeof(double);
+}
+
+void make_m2l_f16(const OpSpec* d_specs, int nops, int p, const F16Split& f16, hipStream_t st) {
+    if (nops == 0) return;
+    const int L = (p + 1) * (p + 1);
+    if (f16.R < L || f16.Kp < L || f16.Kp % 4 || f16.Kp > 256 * M2L_F16_CGROUPS)
+        throw Error(4, "FP16 split tables: bad row/column padding");
+    const int NP = (p + 1) * (p + 2) / 2;
+    const size_t fixed = m2l_f16_fixed_lds(p), per_pair = 2 * (size_t)L * sizeof(float);
+    if (fixed + per_pair > M2L_F16_LDS) throw Error(4, "FP16 split tables: order too large for the LDS staging");
+    const int pps = (int)std::min<size_t>(NP, (M2L_F16_LDS - fixed) / per_pair);
+    const int nsplit = (NP + pps - 1) / pps;
+    k_make_m2l_f16<<<nops * nsplit, 256, fixed + per_pair * pps, st>>>(d_specs, p, 2 * p, f16, pps, nsplit);
+    FMM_HIP(hipGetLastError());
+}
+
 void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, int out_mode, void* d_out, __bf16* d_out_lo,
-                    hipStream_t st, const F16Split* f16) {
+                    hipStream_t st) {
     if (nops == 0) return;
     int pI = 2 * p, L = (p + 1) * (p + 1);
     size_t lds = (size_t)(pI + 1) * (pI + 2) / 2 * sizeof(cplx) + (size_t)((L + 1) & ~1) * sizeof(short2) +
                  (size_t)(p + 1) * sizeof(double);
-    const F16Split none{};
     // about 2048 workgroups in all, at least 8 rows each
     const int nsplit = std::max(1, std::min((Lp + 7) / 8, 2048 / nops));
     const int grid = nops * nsplit;
-    if (out_mode == 0) k_make_ops<0><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, none, nsplit);
-    else if (out_mode == 1) k_make_ops<1><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, none, nsplit);
-    else if (out_mode == 2) k_make_ops<2><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, d_out_lo, none, nsplit);
-    else {
-        if (!f16 || f16->R > 512 || f16->R > Lp || f16->Kp > Lp || f16->R < L || f16->Kp < L || f16->Kp % 8)
-            throw Error(4, "FP16 split tables: bad row/column padding");
-        k_make_ops<3><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, *f16, nsplit);
-    }
+    if (out_mode == 0) k_make_ops<0><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, nsplit);
+    else if (out_mode == 1) k_make_ops<1><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, nullptr, nsplit);
+    else k_make_ops<2><<<grid, 256, lds, st>>>(d_specs, p, pI, Lp, d_out, d_out_lo, nsplit);
     FMM_HIP(hipGetLastError());
 }
 

@@ -15,7 +15,7 @@ struct OpSpec {
     int m2l_full;   // M2L keeps l + lambda <= 2p (else the reference's l + lambda <= p)
 };
 
-// FP16 split tables (out_mode 3): the operator equilibrated as rs_i T_ij / dm_j -- dm_j the
+// FP16 split tables (make_m2l_f16): the operator equilibrated as rs_i T_ij / dm_j -- dm_j the
 // input coefficient's scale, rs_i the power of 2 that brings row i's maximum into [0.5, 1) --
 // and split into hi + lo FP16 parts, [op][R rows][Kp columns] (R >= L, Kp >= L, zero padded);
 // rinv[op][i] = 1 / rs_i restores the rows of T x.
@@ -29,9 +29,11 @@ struct F16Split {
 
 // nops operators of Lp x Lp, row-major [out][in], zero padded, computed in FP64 and written as
 // out_mode 0: double (d_out), 1: float (d_out), 2: bf16 hi (d_out) + bf16 lo (d_out_lo) of the
-// float value, 3: the FP16 split *f16 (d_out: FP32 staging of nops x Lp x Lp).
+// float value.
 void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, int out_mode, void* d_out, __bf16* d_out_lo,
-                    hipStream_t st, const F16Split* f16 = nullptr);
+                    hipStream_t st);
+// nops M2L operators (specs of kind OPK_M2L) as FP16 splits, computed in FP64
+void make_m2l_f16(const OpSpec* d_specs, int nops, int p, const F16Split& f16, hipStream_t st);
 void make_m2l_specs(const int* class_begin, int nc, const int* far_t, const int* far_s, const double* sc,
                     const double* tc, const int* slev, const int* tlev, const ScaleTable& sigS, const ScaleTable& sigT,
                     int m2l_full, OpSpec* out, hipStream_t st);
