@@ -135,7 +135,7 @@ struct fmm_ctx {
     PairIndex m2l_ix;                            // far pairs by target node
     // F16X3: per-coefficient scales dm, the M2L tables split in FP16 (their FP32 staging), the
     // multipole column scales; down_l2l = L2L operators ahead of the M2L ones in the down tables
-    DevBuf dm, m2l_hi, m2l_lo, m2l_rinv, cscale;
+    DevBuf dm, m2l_hi, m2l_lo, m2l_rinv, m2l_work, cscale;
     int down_l2l = 0;
     const OpSpec* down_specs = nullptr;          // device specs of the down tables
     DevBuf m2l_slab;                             // M2L result per (pair, component)
@@ -601,7 +601,7 @@ void translate_tables(fmm_ctx* c, hipStream_t st) {
             f.lo = c->m2l_lo.get<_Float16>((size_t)nc * f.R * f.Kp);
             f.rinv = c->m2l_rinv.get<float>((size_t)nc * f.R);
             f.dm = c->dm.as<float>();
-            make_m2l_f16(d + n_l2l, nc, c->p, f, st);
+            make_m2l_f16(d + n_l2l, nc, c->p, f, c->m2l_work.reserve(m2l_f16_work_bytes(nc, c->p)), st);
         }
     } else if (O.nops) {
         build_tables(c, O, d, st);

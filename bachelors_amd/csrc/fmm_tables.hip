@@ -182,6 +182,36 @@ __global__ __launch_bounds__(256) void k_make_ops(const OpSpec* __restrict__ spe
 
 constexpr int M2L_F16_CGROUPS = 2;   // column groups of 256 in the split (Kp <= 512)
 
+// per M2L operator: the irregular harmonic table X(n, k >= 0), n <= pH, packed as in k_make_ops
+// (nX entries), and the input-degree factors fac[a] = (sig_s / sig_t)^a.  Thread (op, k) runs
+// column k of the recurrences; threads k <= p also give fac[k].
+__global__ void k_m2l_harmonics(const OpSpec* __restrict__ specs, int nops, int p, int pI_full, int nX,
+                                cplx* __restrict__ Xall, double* __restrict__ facall) {
+    const int cols = pI_full + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nops * cols) return;
+    const int opi = t / cols, k = t - opi * cols;
+    const OpSpec sp = specs[opi];
+    const int pH = sp.m2l_full ? pI_full : p;
+    if (k <= p) facall[(size_t)opi * (p + 1) + k] = pow(sp.sig_in / sp.sig_out, k);
+    if (k > pH) return;
+    cplx* X = Xall + (size_t)opi * nX;
+    const double ux = sp.t[0] / sp.sig_out, uy = sp.t[1] / sp.sig_out, uz = sp.t[2] / sp.sig_out;
+    const double r2 = ux * ux + uy * uy + uz * uz;
+    const double inv = 1.0 / sqrt(r2);
+    cplx d = {inv, 0};   // Psi_k^k = (2k-1)!! (x+iy)^k / r^(2k+1)
+    for (int i = 1; i <= k; i++) d = cscale(cmul(d, {ux, uy}), (2.0 * i - 1) / r2);
+    cplx pm2 = {0, 0}, pm1 = d;
+    X[tri(k, k)] = cscale(d, sgn_pow(k));
+    for (int n = k + 1; n <= pH; n++) {
+        cplx v = cscale(cadd(cscale(pm1, (2.0 * n - 1) * uz), cscale(pm2, -(double)((n - 1) * (n - 1) - k * k))),
+                        1.0 / r2);
+        X[tri(n, k)] = cscale(v, sgn_pow(n));
+        pm2 = pm1;
+        pm1 = v;
+    }
+}
+
 // M2L operators as FP16 splits (F16Split), one item per (output degree l, order |m|) x (input
 // degree a, order |b|): the complex kernel entries K(+-m, +-b) = f(+-m, +-b) X(l + a, +-m -+ b)
 // take two harmonics, X(n, m - b) and X(n, m + b), and their conjugates, and give the up to four
@@ -191,46 +221,38 @@ constexpr int M2L_F16_CGROUPS = 2;   // column groups of 256 in the split (Kp <=
 // two per pair (+m, then -m; m = 0 leaves the second unused); each wave then takes whole rows:
 // the row maximum by a shuffle reduction, the power-of-2 row scale and the hi + lo split into
 // the [R][Kp] tables.  Rows L .. R-1 (zero, rinv 1) belong to workgroup 0.
-__global__ __launch_bounds__(256) void k_make_m2l_f16(const OpSpec* __restrict__ specs, int p, int pI_full,
-                                                      F16Split f16, int pps, int nsplit) {
+__global__ __launch_bounds__(256) void k_make_m2l_f16(const OpSpec* __restrict__ specs, int p, int pI_full, int nX,
+                                                      const cplx* __restrict__ Xall,
+                                                      const double* __restrict__ facall, F16Split f16, int pps,
+                                                      int nsplit) {
     extern __shared__ cplx X[];   // packed (n, k>=0), n <= pH
     const int opi = blockIdx.x / nsplit, slice = blockIdx.x - opi * nsplit;
-    const OpSpec sp = specs[opi];
-    const int pH = sp.m2l_full ? pI_full : p;
+    const int pH = specs[opi].m2l_full ? pI_full : p;
     const int L = (p + 1) * (p + 1), NP = (p + 1) * (p + 2) / 2;
-    const int nX = (pI_full + 1) * (pI_full + 2) / 2;
-    short2* tri_lm = (short2*)(X + nX);                  // (n, k) of packed index t (degree <= p)
-    double* fac = (double*)(tri_lm + ((NP + 1) & ~1));   // (sig_s / sig_t)^a
-    float* stg = (float*)(fac + p + 1);                  // the slice's rows, [row][L]
-    const double ux = sp.t[0] / sp.sig_out, uy = sp.t[1] / sp.sig_out, uz = sp.t[2] / sp.sig_out;
-    const double r2 = ux * ux + uy * uy + uz * uz;
-    // column k of the irregular harmonic table, one thread per k (as k_make_ops)
-    for (int k = threadIdx.x; k <= pH; k += blockDim.x) {
-        const double inv = 1.0 / sqrt(r2);
-        cplx d = {inv, 0};   // Psi_k^k = (2k-1)!! (x+iy)^k / r^(2k+1)
-        for (int i = 1; i <= k; i++) d = cscale(cmul(d, {ux, uy}), (2.0 * i - 1) / r2);
-        cplx pm2 = {0, 0}, pm1 = d;
-        X[tri(k, k)] = cscale(d, sgn_pow(k));
-        for (int n = k + 1; n <= pH; n++) {
-            cplx v = cscale(cadd(cscale(pm1, (2.0 * n - 1) * uz), cscale(pm2, -(double)((n - 1) * (n - 1) - k * k))),
-                            1.0 / r2);
-            X[tri(n, k)] = cscale(v, sgn_pow(n));
-            pm2 = pm1;
-            pm1 = v;
-        }
+    short2* tri_lm = (short2*)(X + nX);                    // (n, k) of packed index t (degree <= p)
+    double* cf = (double*)(tri_lm + ((NP + 1) & ~1));      // column factors: fac[a] and 1 / dm_j
+    double* idm = cf + (p + 1);
+    float* stg = (float*)(idm + L);                        // the slice's rows, [row][L]
+    {
+        const cplx* Xg = Xall + (size_t)opi * nX;
+        const int nXh = (pH + 1) * (pH + 2) / 2;
+        for (int t = threadIdx.x; t < nXh; t += blockDim.x) X[t] = Xg[t];
     }
     for (int t = threadIdx.x; t < NP; t += blockDim.x) {
         int n = 0;
         while ((n + 1) * (n + 2) / 2 <= t) n++;
         tri_lm[t] = make_short2((short)n, (short)(t - n * (n + 1) / 2));
     }
-    const double ratio = sp.sig_in / sp.sig_out;
-    for (int n = threadIdx.x; n <= p; n += blockDim.x) fac[n] = pow(ratio, n);
+    for (int n = threadIdx.x; n <= p; n += blockDim.x) cf[n] = facall[(size_t)opi * (p + 1) + n];
+    for (int j = threadIdx.x; j < L; j += blockDim.x) idm[j] = (double)(1.0f / f16.dm[j]);   // exact: dm is 2^k
     __syncthreads();
     const double c = 0.70710678118654752440;
     const int rp0 = slice * pps, nrp = min(NP, rp0 + pps) - rp0;
-    for (int e = threadIdx.x; e < nrp * NP; e += blockDim.x) {
-        const int rp = e / NP, q = e - rp * NP;
+    // items e = rp NP + q, (rp, q) advanced by the stride instead of divided out
+    int rp = threadIdx.x / NP, q = threadIdx.x - (threadIdx.x / NP) * NP;
+    const int drp = blockDim.x / NP, dq = blockDim.x - drp * NP;
+    for (int e = threadIdx.x; e < nrp * NP; e += blockDim.x, rp += drp, q += dq) {
+        if (q >= NP) { q -= NP; rp++; }
         const short2 lm = tri_lm[rp0 + rp], ab = tri_lm[q];
         const int l = lm.x, m = lm.y, a = ab.x, b = ab.y, n = l + a;
         // K(s1 m, s2 b), s = +-1: f(m', b') = (-1)^|(|m'-b'| - |m'| - |b'|) / 2| (-1)^a
@@ -243,12 +265,12 @@ __global__ __launch_bounds__(256) void k_make_m2l_f16(const OpSpec* __restrict__
             kmp = cscale(cplx{Bq.re, -Bq.im}, sgn_pow(abs((abs(-m - b) - m - b) / 2)) * sa);
             kmm = cscale(cplx{A.re, -A.im}, sgn_pow(abs((abs(b - m) - m - b) / 2)) * sa);
         }
-        const double fa = fac[a];
-        const int rp_ = 2 * rp, rm_ = 2 * rp + 1;                           // staged rows (l, +-m)
-        const int jp = a * a + a + b, jm = a * a + a - b;                   // columns (a, +-b)
+        const double fa = cf[a];
+        const int rp_ = 2 * rp, rm_ = 2 * rp + 1;   // staged rows (l, +-m)
+        const int jp = a * a + a + b, jm = a * a + a - b;   // columns (a, +-b)
         auto put = [&](int r, int j, double val) {
             val *= fa;
-            stg[r * L + j] = (float)(val * (double)(1.0f / f16.dm[j]));   // exact: dm is 2^k
+            stg[r * L + j] = (float)(val * idm[j]);
         };
         if (b == 0) {   // column (a, 0): out_c(m') = K(m', 0)
             if (m == 0) put(rp_, jp, kpp.re);
@@ -319,26 +341,32 @@ __global__ __launch_bounds__(256) void k_make_m2l_f16(const OpSpec* __restrict__
     }
 }
 
-// row pairs per workgroup of k_make_m2l_f16: as many as the LDS budget holds (two staged
-// rows of L floats each)
-constexpr size_t M2L_F16_LDS = 56 * 1024;
-static size_t m2l_f16_fixed_lds(int p) {
-    const int NP = (p + 1) * (p + 2) / 2, pI = 2 * p;
-    return (size_t)(pI + 1) * (pI + 2) / 2 * sizeof(cplx) + (size_t)((NP + 1) & ~1) * sizeof(short2) +
-           (size_t)(p + 1) * sizeof(double);
+// row pairs per workgroup of k_make_m2l_f16: as many as the staging budget holds (two rows of
+// L floats each), so that several workgroups share a CU
+constexpr size_t M2L_F16_STAGE = 24 * 1024;
+
+size_t m2l_f16_work_bytes(int nops, int p) {
+    const int pI = 2 * p, nX = (pI + 1) * (pI + 2) / 2;
+    return (size_t)nops * nX * sizeof(cplx) + (size_t)nops * (p + 1) * sizeof(double);
 }
 
-void make_m2l_f16(const OpSpec* d_specs, int nops, int p, const F16Split& f16, hipStream_t st) {
+void make_m2l_f16(const OpSpec* d_specs, int nops, int p, const F16Split& f16, void* work, hipStream_t st) {
     if (nops == 0) return;
-    const int L = (p + 1) * (p + 1);
+    const int L = (p + 1) * (p + 1), NP = (p + 1) * (p + 2) / 2, pI = 2 * p, nX = (pI + 1) * (pI + 2) / 2;
     if (f16.R < L || f16.Kp < L || f16.Kp % 4 || f16.Kp > 256 * M2L_F16_CGROUPS)
         throw Error(4, "FP16 split tables: bad row/column padding");
-    const int NP = (p + 1) * (p + 2) / 2;
-    const size_t fixed = m2l_f16_fixed_lds(p), per_pair = 2 * (size_t)L * sizeof(float);
-    if (fixed + per_pair > M2L_F16_LDS) throw Error(4, "FP16 split tables: order too large for the LDS staging");
-    const int pps = (int)std::min<size_t>(NP, (M2L_F16_LDS - fixed) / per_pair);
+    cplx* Xall = (cplx*)work;
+    double* facall = (double*)(Xall + (size_t)nops * nX);
+    const int nthr = nops * (pI + 1);
+    k_m2l_harmonics<<<(nthr + 255) / 256, 256, 0, st>>>(d_specs, nops, p, pI, nX, Xall, facall);
+    FMM_HIP(hipGetLastError());
+    const size_t per_pair = 2 * (size_t)L * sizeof(float);
+    const int pps = (int)std::max<size_t>(1, std::min<size_t>(NP, M2L_F16_STAGE / per_pair));
     const int nsplit = (NP + pps - 1) / pps;
-    k_make_m2l_f16<<<nops * nsplit, 256, fixed + per_pair * pps, st>>>(d_specs, p, 2 * p, f16, pps, nsplit);
+    const size_t lds = (size_t)nX * sizeof(cplx) + (size_t)((NP + 1) & ~1) * sizeof(short2) +
+                       (size_t)(p + 1 + L) * sizeof(double) + per_pair * pps;
+    if (lds > 64 * 1024) throw Error(4, "FP16 split tables: order too large for the LDS staging");
+    k_make_m2l_f16<<<nops * nsplit, 256, lds, st>>>(d_specs, p, pI, nX, Xall, facall, f16, pps, nsplit);
     FMM_HIP(hipGetLastError());
 }
 

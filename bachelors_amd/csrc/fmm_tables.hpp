@@ -32,8 +32,10 @@ struct F16Split {
 // float value.
 void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, int out_mode, void* d_out, __bf16* d_out_lo,
                     hipStream_t st);
-// nops M2L operators (specs of kind OPK_M2L) as FP16 splits, computed in FP64
-void make_m2l_f16(const OpSpec* d_specs, int nops, int p, const F16Split& f16, hipStream_t st);
+// nops M2L operators (specs of kind OPK_M2L) as FP16 splits, computed in FP64; work: device
+// scratch of m2l_f16_work_bytes(nops, p) (the harmonic tables)
+size_t m2l_f16_work_bytes(int nops, int p);
+void make_m2l_f16(const OpSpec* d_specs, int nops, int p, const F16Split& f16, void* work, hipStream_t st);
 void make_m2l_specs(const int* class_begin, int nc, const int* far_t, const int* far_s, const double* sc,
                     const double* tc, const int* slev, const int* tlev, const ScaleTable& sigS, const ScaleTable& sigT,
                     int m2l_full, OpSpec* out, hipStream_t st);
