@@ -191,6 +191,8 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
     const int NB = p * (p + 1) / 2;               // (l', m') pairs with l' <= p-1
     double* Ls = sh;                               // [3][L]
     double* beta = sh + 3 * L;                     // [3][NB][2]
+    double* part = beta + 6 * NB;                  // [64][2][3] partial sums of the lanes
+    double* rcp = part + 6 * 64;                   // 1 / n for n < p^2 (the recurrences' divisors)
     const int leaf = leaves[blockIdx.x];
     const int lane = threadIdx.x;
     const T* Lsrc = Lexp + (size_t)leaf * 3 * Lp;
@@ -198,6 +200,7 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
         int c = e / L, k = e - c * L;
         Ls[e] = (double)Lsrc[c * Lp + k];
     }
+    for (int e = lane; e < p * p; e += 64) rcp[e] = recip(e);
     __syncthreads();
     for (int e = lane; e < NB; e += 64) {
         int lp = 0;
@@ -217,55 +220,75 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
     const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
     const double inv = 1.0 / sig.s[level[leaf]];
     const double s2 = inv * inv;
-    // lane (m, g): column m (< p) of the degree <= p-1 table for targets g, g+G, ...
+    // lane (m, g): column m (< p) of the degree <= p-1 table for the targets 2 G it + g and
+    // 2 G it + G + g (two independent recurrence chains sharing the beta loads)
     const int G = 64 / p, m = lane % p, g = lane / p;
-    double* part = beta + 6 * NB;   // [64][3] partial sums of the lanes
     const int kb = begin[leaf], n = end[leaf] - kb;
-    for (int it = 0; it * G < n; it++) {
-        const int t = it * G + g;
-        const bool valid = g < G && t < n;
-        double b0 = 0, b1 = 0, b2 = 0;
-        if (valid) {
-            const int k = kb + t;
-            double ux = (txyz[3 * k] - cx) * inv, uy = (txyz[3 * k + 1] - cy) * inv, uz = (txyz[3 * k + 2] - cz) * inv;
-            double r2 = ux * ux + uy * uy + uz * uz;
-            double dr = 1.0, di = 0.0;
+    for (int it = 0; 2 * it * G < n; it++) {
+        const int t0 = 2 * it * G + g, t1 = t0 + G;
+        const bool v0 = g < G && t0 < n, v1 = g < G && t1 < n;
+        double b0 = 0, b1 = 0, b2 = 0, c0 = 0, c1 = 0, c2 = 0;
+        if (v0) {
+            const int k0 = kb + t0, k1 = kb + (v1 ? t1 : t0);
+            const double ux = (txyz[3 * k0] - cx) * inv, uy = (txyz[3 * k0 + 1] - cy) * inv,
+                         uz = (txyz[3 * k0 + 2] - cz) * inv;
+            const double wx = (txyz[3 * k1] - cx) * inv, wy = (txyz[3 * k1 + 1] - cy) * inv,
+                         wz = (txyz[3 * k1 + 2] - cz) * inv;
+            const double r2 = ux * ux + uy * uy + uz * uz, q2 = wx * wx + wy * wy + wz * wz;
+            double dr = 1.0, di = 0.0, er = 1.0, ei = 0.0;
             for (int i = 1; i <= m; i++) {
-                double s = recip(2 * i);
+                const double s = rcp[2 * i];
                 double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
-                dr = nr;
-                di = ni;
+                double mr = (er * wx - ei * wy) * s, mi = (er * wy + ei * wx) * s;
+                dr = nr; di = ni; er = mr; ei = mi;
             }
             double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
+            double o1r = er, o1i = ei, o2r = 0.0, o2i = 0.0;
             for (int l = m; l < p; l++) {
-                double vr = dr, vi = di;
+                double vr = dr, vi = di, yr = er, yi = ei;
                 if (l > m) {
-                    double f = recip(l * l - m * m);
+                    const double f = rcp[l * l - m * m];
                     vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
                     vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
+                    yr = ((2.0 * l - 1) * wz * o1r - q2 * o2r) * f;
+                    yi = ((2.0 * l - 1) * wz * o1i - q2 * o2i) * f;
                     p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
+                    o2r = o1r; o2i = o1i; o1r = yr; o1i = yi;
                 }
-                int e = l * (l + 1) / 2 + m;
-                b0 += beta[(0 * NB + e) * 2] * vr - beta[(0 * NB + e) * 2 + 1] * vi;
-                b1 += beta[(1 * NB + e) * 2] * vr - beta[(1 * NB + e) * 2 + 1] * vi;
-                b2 += beta[(2 * NB + e) * 2] * vr - beta[(2 * NB + e) * 2 + 1] * vi;
+                const int e = l * (l + 1) / 2 + m;
+                const double x0r = beta[(0 * NB + e) * 2], x0i = beta[(0 * NB + e) * 2 + 1];
+                const double x1r = beta[(1 * NB + e) * 2], x1i = beta[(1 * NB + e) * 2 + 1];
+                const double x2r = beta[(2 * NB + e) * 2], x2i = beta[(2 * NB + e) * 2 + 1];
+                b0 += x0r * vr - x0i * vi;
+                b1 += x1r * vr - x1i * vi;
+                b2 += x2r * vr - x2i * vi;
+                c0 += x0r * yr - x0i * yi;
+                c1 += x1r * yr - x1i * yi;
+                c2 += x2r * yr - x2i * yi;
             }
         }
-        part[3 * lane] = b0;
-        part[3 * lane + 1] = b1;
-        part[3 * lane + 2] = b2;
+        part[6 * lane] = b0;
+        part[6 * lane + 1] = b1;
+        part[6 * lane + 2] = b2;
+        part[6 * lane + 3] = c0;
+        part[6 * lane + 4] = c1;
+        part[6 * lane + 5] = c2;
         __syncthreads();
-        if (valid && m == 0) {
-            double s0 = 0, s1 = 0, sz = 0;
-            for (int mm = 0; mm < p; mm++) {
-                s0 += part[3 * (lane + mm)];
-                s1 += part[3 * (lane + mm) + 1];
-                sz += part[3 * (lane + mm) + 2];
+        if (m == 0 && (v0 || v1)) {
+            const int h = v0 ? 0 : 1;   // lanes m = 0 of group g: target t0 (and t1 below)
+            for (int u = h; u < 2; u++) {
+                if (u == 1 && !v1) break;
+                double s0 = 0, s1 = 0, sz = 0;
+                for (int mm = 0; mm < p; mm++) {
+                    s0 += part[6 * (lane + mm) + 3 * u];
+                    s1 += part[6 * (lane + mm) + 3 * u + 1];
+                    sz += part[6 * (lane + mm) + 3 * u + 2];
+                }
+                const int k = kb + (u ? t1 : t0);
+                Bfar[3 * k] = s0 * s2;
+                Bfar[3 * k + 1] = s1 * s2;
+                Bfar[3 * k + 2] = sz * s2;
             }
-            const int k = kb + t;
-            Bfar[3 * k] = s0 * s2;
-            Bfar[3 * k + 1] = s1 * s2;
-            Bfar[3 * k + 2] = sz * s2;
         }
         __syncthreads();
     }
@@ -501,7 +524,7 @@ void launch_l2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, cons
                 const void* Lexp, int p, int Lp, double* Bfar, hipStream_t st) {
     if (!nleaves) return;
     int L = (p + 1) * (p + 1), NB = p * (p + 1) / 2;
-    size_t lds = (size_t)(3 * L + 6 * NB + 3 * 64) * sizeof(double);
+    size_t lds = (size_t)(3 * L + 6 * NB + 6 * 64 + p * p) * sizeof(double);
     if (p < 1) throw Error(1, "order must be at least 1");
     if (f64)
         k_l2p<double><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),

@@ -843,13 +843,23 @@ __global__ __launch_bounds__(256) void k_reduce_nodes(const uint32_t* __restrict
     if (s0 == s1) return;
     const int nv = 3 * vr;
     V* o = out + (size_t)v * 3 * vl;
+    // the loads of four segments are issued together; the adds keep the segment order
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         const int i = lane + 64 * r;
         if (i >= nv) continue;
         const int c = i / vr, q = c * vl + (i - c * vr);
         V acc = o[q];
-        for (int sg = s0; sg < s1; sg++) acc = acc + partial[(size_t)sg * nv + i];
+        const V* pp = partial + (size_t)s0 * nv + i;
+        int sg = s0;
+        for (; sg + 4 <= s1; sg += 4, pp += 4 * (size_t)nv) {
+            const V x0 = pp[0], x1 = pp[nv], x2 = pp[2 * (size_t)nv], x3 = pp[3 * (size_t)nv];
+            acc = acc + x0;
+            acc = acc + x1;
+            acc = acc + x2;
+            acc = acc + x3;
+        }
+        for (; sg < s1; sg++, pp += nv) acc = acc + pp[0];
         o[q] = acc;
     }
 }
