@@ -141,6 +141,47 @@ def test_interaction_lists_bit_exact(fm, orc, frame):
     assert len(ft) == len(oft) and len(ct) == len(oct_)
 
 
+def _class_codes(S, T, ft, fs):
+    """(level difference, centre offset in units of the finer box half size) of each far pair,
+    from the oracle's trees, as one sortable integer."""
+    hf = np.minimum(S.half[fs, 0], T.half[ft, 0])
+    off = np.rint((S.center[fs] - T.center[ft]) / hf[:, None]).astype(np.int64) + 128
+    d = (S.level[fs] - T.level[ft]).astype(np.int64) + 32
+    return (d << 24) | (off[:, 0] << 16) | (off[:, 1] << 8) | off[:, 2]
+
+
+@pytest.mark.parametrize("near_uniform", [True, False])
+def test_far_pairs_class_order(fm, orc, near_uniform):
+    """Shared frame: far pairs come sorted by (class, target) and the class count matches the
+    distinct (level difference, offset) codes.  Uniform targets as dense as the sources give
+    level differences up to 2 and offsets up to 15 finer half sizes; few targets around a
+    thin shell give level differences up to 4 and offsets up to 51."""
+    if near_uniform:
+        src, w, tgt = _inputs(6, 16000, 16000, kind="uniform")
+        caps = capt = 32
+    else:
+        src, w, tgt = _inputs(6, 12000, 4000)
+        caps, capt = 20, 24
+    F = fm.FMM(order=4, source_leaf_capacity=caps, target_leaf_capacity=capt, frame=1)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    F.build_trees()
+    S = orc.source_tree(src, caps)
+    T = orc.target_tree(tgt, capt, root=SHARED_ROOT)
+    (oft, ofs), _ = orc.interaction_lists(S, T)
+    F.leaf_expansions()
+    F.upward()
+    F.downward()
+    (ft, fs), _, ncls = F.lists()
+    code = _class_codes(S, T, oft, ofs)
+    order = np.lexsort((oft, code))
+    assert np.array_equal(ft, oft[order]) and np.array_equal(fs, ofs[order])
+    assert ncls == len(np.unique(code))
+    d = (code >> 24) - 32
+    o = np.stack([(code >> 16) & 255, (code >> 8) & 255, code & 255], 1) - 128
+    assert (d.max(), np.abs(o).max()) == ((2, 15) if near_uniform else (4, 51))
+
+
 @pytest.mark.parametrize("kind", [0, 1, 2])
 @pytest.mark.parametrize("p", [4, 12])
 def test_operator_tables_fp64(fm, orc, kind, p):
