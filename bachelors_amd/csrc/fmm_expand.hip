@@ -337,11 +337,11 @@ __global__ void k_pack_sources(int n, const double* __restrict__ origin, const d
     P2[j] = (R)sq[3 * j + 2];
 }
 
-// One workgroup (4 waves) per target leaf.  The leaf's close list is dealt round-robin to the
-// waves; each wave stages a close node's sources (64 per pass) in its own LDS slice and
-// its lanes sweep it, G = 64 / 2^ceil(log2 n) lanes per target for a leaf of n <= 64
-// targets (lane = target * G + g takes sources g, g + G, ...).  Lane partials are summed
-// over the G lanes, then over the 4 waves in a fixed order, and written once.
+// One workgroup (4 waves) per target leaf.  The 64-source chunks of the leaf's close list are
+// dealt round-robin to the waves; each wave stages a chunk in its own LDS slice and its lanes
+// sweep it, G = 64 / 2^ceil(log2 n) lanes per target for a leaf of n <= 64 targets
+// (lane = target * G + g takes sources g, g + G, ...).  Lane partials are summed over the G
+// lanes, then over the 4 waves in a fixed order, and written once.
 template <class R>
 __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, const int* __restrict__ tbegin, const int* __restrict__ tend,
                                              const int* __restrict__ close_begin, const int* __restrict__ close_end,
@@ -380,10 +380,14 @@ __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, con
         const R ulx = split ? (R)(ux - (double)uhx) : (R)0, uly = split ? (R)(uy - (double)uhy) : (R)0;
         const R ulz = split ? (R)(uz - (double)uhz) : (R)0;
         double a0 = 0, a1 = 0, a2 = 0;
-        for (int ci = cb + wave; ci < ce; ci += 4) {
+        // the close list's 64-source chunks, in list order, dealt round-robin to the waves (an
+        // internal close node can hold most of a leaf's sources)
+        int run = 0;   // chunks of the close nodes before ci
+        for (int ci = cb; ci < ce; ci++) {
             const int s = close_s[ci];
             const int sb = sbegin[s], se = send[s];
-            for (int j0 = sb; j0 < se; j0 += 64) {
+            const int nch = (se - sb + 63) >> 6;
+            for (int j0 = sb + 64 * ((wave - run) & 3); j0 < se; j0 += 4 * 64) {
                 const int nj = min(64, se - j0);
                 __syncwarp();
                 if (lane < nj) {
@@ -416,6 +420,7 @@ __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, con
                 a1 += f1;
                 a2 += f2;
             }
+            run += nch;
         }
         for (int off = 1; off < G; off <<= 1) {
             a0 += __shfl_xor(a0, off);
