@@ -189,10 +189,12 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
     extern __shared__ double sh[];
     const int L = (p + 1) * (p + 1);
     const int NB = p * (p + 1) / 2;               // (l', m') pairs with l' <= p-1
-    double* Ls = sh;                               // [3][L]
-    double* beta = sh + 3 * L;                     // [3][NB][2]
-    double* part = beta + 6 * NB;                  // [64][2][3] partial sums of the lanes
-    double* rcp = part + 6 * 64;                   // 1 / n for n < p^2 (the recurrences' divisors)
+    // LDS: beta, then one region holding first the local expansion and, once beta is built,
+    // the lanes' partial sums and the reciprocal table (fewer bytes per wave: more waves per CU)
+    double* beta = sh;                             // [3][NB][2]
+    double* Ls = sh + 6 * NB;                      // [3][L]
+    double* part = Ls;                             // [64][2][3] partial sums of the lanes
+    double* rcp = Ls + 6 * 64;                     // 1 / n for n < p^2 (the recurrences' divisors)
     const int leaf = leaves[blockIdx.x];
     const int lane = threadIdx.x;
     const T* Lsrc = Lexp + (size_t)leaf * 3 * Lp;
@@ -200,7 +202,6 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
         int c = e / L, k = e - c * L;
         Ls[e] = (double)Lsrc[c * Lp + k];
     }
-    for (int e = lane; e < p * p; e += 64) rcp[e] = recip(e);
     __syncthreads();
     for (int e = lane; e < NB; e += 64) {
         int lp = 0;
@@ -216,6 +217,8 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
             beta[(2 * NB + e) * 2 + q] = d[1][0][q] - d[0][1][q];
         }
     }
+    __syncthreads();   // the local expansion is no longer read: its space takes the reciprocals
+    for (int e = lane; e < p * p; e += 64) rcp[e] = recip(e);
     __syncthreads();
     const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
     const double inv = 1.0 / sig.s[level[leaf]];
@@ -529,7 +532,7 @@ void launch_l2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, cons
                 const void* Lexp, int p, int Lp, double* Bfar, hipStream_t st) {
     if (!nleaves) return;
     int L = (p + 1) * (p + 1), NB = p * (p + 1) / 2;
-    size_t lds = (size_t)(3 * L + 6 * NB + 6 * 64 + p * p) * sizeof(double);
+    size_t lds = (size_t)(6 * NB + std::max(3 * L, 6 * 64 + p * p)) * sizeof(double);
     if (p < 1) throw Error(1, "order must be at least 1");
     if (f64)
         k_l2p<double><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
