@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void k_make_m2l_f16(const OpSpec* __restrict__
 
 // row pairs per workgroup of k_make_m2l_f16: as many as the staging budget holds (two rows of
 // L floats each), so that several workgroups share a CU
-constexpr size_t M2L_F16_STAGE = 24 * 1024;
+constexpr size_t M2L_F16_STAGE = 16 * 1024;   // p = 12: 12 pairs, 23 KB in all, 7 workgroups per CU
 
 size_t m2l_f16_work_bytes(int nops, int p) {
     const int pI = 2 * p, nX = (pI + 1) * (pI + 2) / 2;
