@@ -123,7 +123,8 @@ int fmm_direct_B_dev(fmm_ctx* ctx, double current, size_t m, const double* xyz, 
 int fmm_get_stats(fmm_ctx* ctx, fmm_stats* stats);
 
 /* ---- parity export (host arrays) ---- */
-/* which: 0 source tree, 1 target tree; arrays in canonical BFS node order */
+/* which: 0 source tree, 1 target tree; arrays in canonical BFS node order.  The node arrays are
+ * copied from the device on the first export after a build (the device is synchronised). */
 int fmm_tree_size(fmm_ctx* ctx, int which, int* num_nodes, int* num_slots);
 int fmm_tree_export(fmm_ctx* ctx, int which, int* level, int* parent, int* first_child, int* begin, int* end,
                     int* npts, double* center, int* slots);
