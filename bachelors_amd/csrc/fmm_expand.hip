@@ -342,9 +342,9 @@ __global__ void k_pack_sources(int n, const double* __restrict__ origin, const d
 
 // One workgroup (4 waves) per target leaf.  The 64-source chunks of the leaf's close list are
 // dealt round-robin to the waves; each wave stages a chunk in its own LDS slice and its lanes
-// sweep it, G = 64 / 2^ceil(log2 n) lanes per target for a leaf of n <= 64 targets
-// (lane = target * G + g takes sources g, g + G, ...).  Lane partials are summed over the G
-// lanes, then over the 4 waves in a fixed order, and written once.
+// sweep it, G = floor(64 / n) lanes per target for a leaf of 8 <= n <= 64 targets (64 / 2^ceil(log2 n)
+// below 8; lane = target * G + g takes sources g, g + G, ...).  Lane partials are summed over
+// the G lanes, then over the 4 waves in a fixed order, and written once.
 template <class R>
 __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, const int* __restrict__ tbegin, const int* __restrict__ tend,
                                              const int* __restrict__ close_begin, const int* __restrict__ close_end,
@@ -369,9 +369,12 @@ __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, con
     }
     for (int k0 = tb; k0 < te; k0 += 64) {
         const int n = min(64, te - k0);
+        // G lanes per target: 64 / n (n >= 8, at most 8 lanes, summed through LDS), else the power
+        // of 2 64 / 2^ceil(log2 n) (summed by shuffles)
+        const bool pow2 = n < 8;
         const int lgG = 6 - (n <= 1 ? 0 : 32 - __clz(n - 1));
-        const int G = 1 << lgG;
-        const int ti = lane >> lgG, g = lane & (G - 1);
+        const int G = pow2 ? 1 << lgG : 64 / n;
+        const int ti = pow2 ? lane >> lgG : lane / G, g = lane - ti * G;
         const bool valid = ti < n;
         double ux = 0, uy = 0, uz = 0;
         if (valid) {
@@ -425,19 +428,28 @@ __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, con
             }
             run += nch;
         }
-        for (int off = 1; off < G; off <<= 1) {
-            a0 += __shfl_xor(a0, off);
-            a1 += __shfl_xor(a1, off);
-            a2 += __shfl_xor(a2, off);
-        }
+        if (pow2)
+            for (int off = 1; off < G; off <<= 1) {
+                a0 += __shfl_xor(a0, off);
+                a1 += __shfl_xor(a1, off);
+                a2 += __shfl_xor(a2, off);
+            }
         red[wave][0][lane] = a0;
         red[wave][1][lane] = a1;
         red[wave][2][lane] = a2;
         __syncthreads();
         if (wave == 0 && valid && g == 0) {
             const int k = k0 + ti;
-            for (int c = 0; c < 3; c++)
-                Bnear[3 * k + c] = ((red[0][c][lane] + red[1][c][lane]) + red[2][c][lane]) + red[3][c][lane];
+            for (int c = 0; c < 3; c++) {
+                double t[4];
+                for (int w = 0; w < 4; w++) {
+                    double sw = red[w][c][lane];
+                    if (!pow2)
+                        for (int gg = 1; gg < G; gg++) sw += red[w][c][lane + gg];
+                    t[w] = sw;
+                }
+                Bnear[3 * k + c] = ((t[0] + t[1]) + t[2]) + t[3];
+            }
         }
         __syncthreads();
     }
