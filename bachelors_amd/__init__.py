@@ -301,7 +301,7 @@ class FastMultipoleSolver:
 
     def __init__(self, quadratures, points, problem=Problem.BioSavartLaplace, quadratureOctreeLeafCapacity=1000,
                  calculationPointOctreeLeafCapacity=100, weights=None, currents=None, order=10,
-                 precision="fp64", frame=FRAME_SHARED, mac_scale=1.0, m2l_full=True, near_field=True):
+                 precision="fp64", frame=FRAME_REFERENCE, mac_scale=1.0, m2l_full=True, near_field=True):
         if problem != Problem.BioSavartLaplace:
             raise FMMError("only Problem.BioSavartLaplace is implemented")
         q = np.asarray(quadratures, np.float64)

@@ -217,6 +217,13 @@ __global__ void k_weights(const int* __restrict__ slots, int n, const double* __
 
 
 void run_adjoint(AdjointState& S, const AdjointArgs& a, const double* dLdB, double* dLdw, hipStream_t st) {
+    // the transposed passes run on class_gemm, whose tiles hold at most 64 columns
+    auto check_width = [](const PassView& v) {
+        if (v.ntiles && v.width > 64) throw Error(1, "adjoint pass planned with tiles wider than 64 columns");
+    };
+    for (const PassView& v : a.m2m_levels) check_width(v);
+    for (const PassView& v : a.l2l_levels) check_width(v);
+    check_width(a.m2l);
     const TreeDev& Ts = *a.S;
     const TreeDev& Tt = *a.T;
     const bool f64 = a.prec == Precision::F64;

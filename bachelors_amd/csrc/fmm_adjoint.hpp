@@ -9,6 +9,7 @@ struct PassView {
     const GemmTile* tiles = nullptr;
     const int* src = nullptr;   // forward source of each pair
     const int* dst = nullptr;   // forward destination of each pair
+    int width = 64;             // columns per tile the pass was planned with (class_gemm takes <= 64)
 };
 
 struct AdjointArgs {

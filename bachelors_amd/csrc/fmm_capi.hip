@@ -87,6 +87,7 @@ struct Staging {
 
 struct Pass {               // one batched GEMM launch: tiles over (src, dst) pairs (staged data)
     int ntiles = 0, npairs = 0;
+    int width = 64;         // columns per tile
     size_t o_src = 0, o_dst = 0, o_tiles = 0;
     const int* src = nullptr;
     const int* dst = nullptr;
@@ -151,7 +152,8 @@ struct fmm_ctx {
     hipStream_t side = nullptr;
     hipEvent_t j_lists = nullptr, j_pack = nullptr, j_p2p = nullptr, j_gemm = nullptr,
                j_start = nullptr, j_tables = nullptr,   // j_tables: everything the side stream builds for M2L
-               j_src = nullptr, j_tgt = nullptr;        // source tree on the main / target side on the side stream
+               j_src = nullptr, j_tgt = nullptr,        // source tree on the main / target side on the side stream
+               j_caller = nullptr;                      // the caller's default-stream work before a _dev read
     long long near_inter = -1;
     bool lists_ready = false;
 
@@ -184,6 +186,14 @@ inline int blocksN(long long n) { return (int)((n + 255) / 256); }
 
 void require(bool c, int code, const char* msg) {
     if (!c) throw Error(code, msg);
+}
+
+// Device-pointer entry points read caller memory on the context stream.  The caller's work on
+// the default (null) stream is ordered before it by an event (no host wait, no device-wide
+// synchronisation); work on other caller streams is the caller's to order (fmm_set_stream).
+void join_default_stream(fmm_ctx* c) {
+    FMM_HIP(hipEventRecord(c->j_caller, nullptr));
+    FMM_HIP(hipStreamWaitEvent(c->stream, c->j_caller, 0));
 }
 
 bool nonempty_host(const TreeDev& T, int n) { return T.h_first_child[n] >= 0 || T.h_npts[n] > 0; }
@@ -618,6 +628,7 @@ void translate_tables(fmm_ctx* c, hipStream_t st) {
         const size_t o_tiles = sg.put(tiles);
         P.npairs = c->lists.num_far;
         P.ntiles = (int)tiles.size();
+        P.width = m2l_tile_cols(c);
         P.tiles = (const GemmTile*)(sg.send(st) + o_tiles);
     }
     size_t n = (size_t)T.num_nodes * 3 * c->Lp;
@@ -853,8 +864,7 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
 
 void set_sources_common(fmm_ctx* c, size_t n, const double* xyz, const double* w, const double* J, hipMemcpyKind kind) {
     require(n < (size_t)INT32_MAX / 3, FMM_ERR_LIMIT, "too many sources");
-    // caller memory may still be written by work on another stream
-    if (kind == hipMemcpyDeviceToDevice) FMM_HIP(hipDeviceSynchronize());
+    if (kind == hipMemcpyDeviceToDevice) join_default_stream(c);
     require(n == 0 || (xyz && w), FMM_ERR_ARG, "null source arrays");
     c->ns = (int)n;
     c->have_J = J != nullptr;
@@ -873,7 +883,7 @@ void set_sources_common(fmm_ctx* c, size_t n, const double* xyz, const double* w
 
 void set_targets_common(fmm_ctx* c, size_t m, const double* xyz, hipMemcpyKind kind) {
     require(m < (size_t)INT32_MAX / 3, FMM_ERR_LIMIT, "too many targets");
-    if (kind == hipMemcpyDeviceToDevice) FMM_HIP(hipDeviceSynchronize());
+    if (kind == hipMemcpyDeviceToDevice) join_default_stream(c);
     require(m == 0 || xyz, FMM_ERR_ARG, "null target array");
     c->nt = (int)m;
     size_t mm = std::max<size_t>(m, 1);
@@ -912,7 +922,7 @@ void fmm_config_init(fmm_config* g) {
     g->source_leaf_capacity = 1000;   // reference defaults (fast_multipole_solver.hpp:29-30)
     g->target_leaf_capacity = 100;
     g->precision = FMM_PREC_FP64;
-    g->frame = FMM_FRAME_SHARED;
+    g->frame = FMM_FRAME_REFERENCE;   // the reference's own trees and output order
     g->m2l_full = 1;
     g->max_depth = 21;
     g->near_field = 1;
@@ -956,6 +966,7 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
         FMM_HIP(hipEventCreateWithFlags(&c->j_tables, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_src, hipEventDisableTiming));
         FMM_HIP(hipEventCreateWithFlags(&c->j_tgt, hipEventDisableTiming));
+        FMM_HIP(hipEventCreateWithFlags(&c->j_caller, hipEventDisableTiming));
     });
     if (rc != FMM_OK) {
         delete c;
@@ -971,7 +982,8 @@ void fmm_destroy(fmm_ctx* c) {
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_ok)
         for (int i = 0; i < E_COUNT; i++) (void)hipEventDestroy(c->ev[i]);
-    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm, c->j_start, c->j_tables, c->j_src, c->j_tgt})
+    for (hipEvent_t e : {c->j_lists, c->j_pack, c->j_p2p, c->j_gemm, c->j_start, c->j_tables, c->j_src, c->j_tgt,
+                         c->j_caller})
         if (e) (void)hipEventDestroy(e);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -987,6 +999,12 @@ int fmm_set_stream(fmm_ctx* c, void* s) {
             if (c->own_stream) FMM_HIP(hipStreamDestroy(c->stream));
             c->own_stream = false;
             c->stream = (hipStream_t)s;
+        } else if (!c->own_stream) {
+            // back to a stream of the context's own (the caller's stream is no longer used)
+            hipStream_t own = nullptr;
+            FMM_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+            c->stream = own;
+            c->own_stream = true;
         }
     });
 }
@@ -1008,7 +1026,7 @@ int fmm_set_targets_dev(fmm_ctx* c, size_t m, const double* xyz) {
 int fmm_set_weights_dev(fmm_ctx* c, const double* w) {
     return guard(c, [&] {
         require(w != nullptr, FMM_ERR_ARG, "null weights");
-        FMM_HIP(hipDeviceSynchronize());
+        join_default_stream(c);
         if (c->ns) FMM_HIP(hipMemcpyAsync(c->src_w.ptr, w, c->ns * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         if (c->stage >= ST_TREES) {
             int nss = c->S.num_slots;
@@ -1065,7 +1083,7 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
         require(c->stage >= ST_DOWN, FMM_ERR_STATE, "run the forward pass first");
         require(dLdB && dLdw, FMM_ERR_ARG, "null adjoint arrays");
         require(c->cfg.frame == FMM_FRAME_SHARED, FMM_ERR_UNSUPPORTED, "the adjoint needs the shared frame");
-        FMM_HIP(hipDeviceSynchronize());
+        join_default_stream(c);
         AdjointArgs a;
         a.prec = c->pass_prec();
         if (c->prec == Precision::F16X3 && c->down_ops.nops > c->down_l2l)
@@ -1114,16 +1132,29 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
             plan_tiles(cb, cop, tiles);
             stage_pass(sg, c->m2m_adj_pass[k], all_s, all_d, tiles);
         }
+        // M2L^T runs on class_gemm (tiles of at most 64 columns); the forward M2L tiles may be
+        // wider (F16X3 plans 128), so the transposed pass gets its own 64-column tiles
+        size_t o_m2l_tiles = 0;
+        int n_m2l_tiles = 0;
+        if (c->m2l_pass.ntiles) {
+            const int nc = c->lists.num_classes;
+            std::vector<int> cop(nc);
+            for (int i = 0; i < nc; i++) cop[i] = c->down_l2l + i;
+            std::vector<GemmTile> tiles;
+            plan_tiles(c->lists.h_class_begin, cop, tiles, 64);
+            n_m2l_tiles = (int)tiles.size();
+            o_m2l_tiles = sg.put(tiles);
+        }
         const char* dev = sg.send(c->stream);
         for (int k = S.num_levels - 1; k >= 1; k--) resolve_pass(c->m2m_adj_pass[k], dev);
         for (int k = 0; k < MAX_LEVELS; k++) {
             const Pass* pm = k < (int)c->m2m_adj_pass.size() ? &c->m2m_adj_pass[k] : nullptr;
             const Pass* pl = k < (int)c->l2l_pass.size() ? &c->l2l_pass[k] : nullptr;
-            a.m2m_levels.push_back(pm ? PassView{pm->ntiles, pm->tiles, pm->src, pm->dst} : PassView{});
-            a.l2l_levels.push_back(pl ? PassView{pl->ntiles, pl->tiles, pl->src, pl->dst} : PassView{});
+            a.m2m_levels.push_back(pm ? PassView{pm->ntiles, pm->tiles, pm->src, pm->dst, pm->width} : PassView{});
+            a.l2l_levels.push_back(pl ? PassView{pl->ntiles, pl->tiles, pl->src, pl->dst, pl->width} : PassView{});
         }
-        a.m2l = PassView{c->m2l_pass.ntiles, c->m2l_pass.tiles, c->lists.far_t.as<int>(),
-                         c->lists.far_s.as<int>()};
+        a.m2l = PassView{n_m2l_tiles, (const GemmTile*)(dev + o_m2l_tiles), c->lists.far_t.as<int>(),
+                         c->lists.far_s.as<int>(), 64};
         a.n_up_ops = c->up_ops.nops;
         a.n_down_ops = c->down_ops.nops;
         a.scale = current * REF_MU0 / (4.0 * REF_PI);
@@ -1136,7 +1167,7 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
 int fmm_direct_B_dev(fmm_ctx* c, double current, size_t m, const double* xyz, double* B) {
     return guard(c, [&] {
         require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built (sources are gathered with them)");
-        FMM_HIP(hipDeviceSynchronize());
+        join_default_stream(c);
         launch_direct(c->sxyz.as<double>(), c->sq.as<double>(), c->S.num_slots, xyz, (int)m,
                       current * REF_MU0 / (4.0 * REF_PI), B, c->stream);
         FMM_HIP(hipStreamSynchronize(c->stream));

@@ -154,8 +154,19 @@ __global__ __launch_bounds__(256) void k_lists_step(const int2* __restrict__ fro
     }
 }
 
-// Far-pair sort key: class code (level difference and centre offset in units of the finer
-// half size) above the target index (target_bits wide); value = source
+// Far-pair sort key: class code (level difference and centre offset, each axis in units of
+// the finer box's half size along that axis) above the target index (target_bits wide);
+// value = source.  hs / ht: per-level half sizes [level][3] of the source / target boxes.
+// Offsets that are not whole multiples of the half size (to 1e-9, relative), or beyond the key's
+// range, set the error flag instead of sharing an operator with another offset.
+__device__ __forceinline__ int lattice_offset(double d, double h, unsigned* err) {
+    const double q = d / h, r = rint(q);
+    if (!(fabs(q - r) <= 1e-9 * fmax(1.0, fabs(q))) || r < -127.0 || r > 127.0) {
+        atomicOr(err, 1u);
+        return 0;
+    }
+    return (int)r;
+}
 __global__ void k_far_keys(const int2* __restrict__ far, int n, const NodeRec* __restrict__ S,
                            const NodeRec* __restrict__ T, const double* __restrict__ hs, const double* __restrict__ ht,
                            int use_classes, int target_bits, uint64_t* __restrict__ key, int* __restrict__ val,
@@ -167,13 +178,12 @@ __global__ void k_far_keys(const int2* __restrict__ far, int n, const NodeRec* _
     if (use_classes) {
         const NodeRec rs = S[s], rt = T[t];
         int ls = lvl(rs), lt = lvl(rt);
-        double hf = fmin(hs[ls], ht[lt]);   // finer of the two boxes (physical size, not tree level)
-        int nx = (int)rint((rs.c[0] - rt.c[0]) / hf);
-        int ny = (int)rint((rs.c[1] - rt.c[1]) / hf);
-        int nz = (int)rint((rs.c[2] - rt.c[2]) / hf);
+        // finer of the two boxes (physical size, not tree level), per axis
+        int nx = lattice_offset(rs.c[0] - rt.c[0], fmin(hs[3 * ls], ht[3 * lt]), err);
+        int ny = lattice_offset(rs.c[1] - rt.c[1], fmin(hs[3 * ls + 1], ht[3 * lt + 1]), err);
+        int nz = lattice_offset(rs.c[2] - rt.c[2], fmin(hs[3 * ls + 2], ht[3 * lt + 2]), err);
         int d = ls - lt;
-        if (nx < -127 || nx > 127 || ny < -127 || ny > 127 || nz < -127 || nz > 127 || d < -31 || d > 31)
-            atomicOr(err, 1u);
+        if (d < -31 || d > 31) atomicOr(err, 1u);
         code = ((uint64_t)(d + 32) << 24) | ((uint64_t)(nx + 128) << 16) | ((uint64_t)(ny + 128) << 8) |
                (uint64_t)(nz + 128);
     }
@@ -335,15 +345,16 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
             uint64_t* key = L.key.get<uint64_t>(std::max(nfar, 1));
             int* val = L.val.get<int>(std::max(nfar, 1));
             unsigned* err = cnt + 2;
-            double* dh = L.hs.get<double>(2 * MAX_LEVELS);
-            for (int k = 0; k < MAX_LEVELS; k++) {
-                L.h_half[k] = S.half[k][0];
-                L.h_half[MAX_LEVELS + k] = T.half[k][0];
-            }
-            FMM_HIP(hipMemcpyAsync(dh, L.h_half, 2 * MAX_LEVELS * sizeof(double), hipMemcpyHostToDevice, st));
+            double* dh = L.hs.get<double>(6 * MAX_LEVELS);
+            for (int k = 0; k < MAX_LEVELS; k++)
+                for (int a = 0; a < 3; a++) {
+                    L.h_half[3 * k + a] = S.half[k][a];
+                    L.h_half[3 * (MAX_LEVELS + k) + a] = T.half[k][a];
+                }
+            FMM_HIP(hipMemcpyAsync(dh, L.h_half, 6 * MAX_LEVELS * sizeof(double), hipMemcpyHostToDevice, st));
             const int tbits = bits_for(T.num_nodes);
             if (nfar > 0)
-                k_far_keys<<<blocks(nfar), 256, 0, st>>>(far, nfar, srec, trec, dh, dh + MAX_LEVELS, classes ? 1 : 0,
+                k_far_keys<<<blocks(nfar), 256, 0, st>>>(far, nfar, srec, trec, dh, dh + 3 * MAX_LEVELS, classes ? 1 : 0,
                                                          tbits, key, val, err);
             int* ft = out.far_t.get<int>(std::max(nfar, 1));
             int* fs = out.far_s.get<int>(std::max(nfar, 1));
@@ -363,7 +374,7 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
                 unsigned hc[2];
                 FMM_HIP(hipMemcpyAsync(hc, cc, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
                 FMM_HIP(hipStreamSynchronize(st));
-                if (hc[1]) throw Error(4, "interaction offset out of the class-key range");
+                if (hc[1]) throw Error(4, "interaction offset off the box lattice or out of the class-key range");
                 int nc = (int)hc[0];
                 int* starts = out.class_begin.get<int>(nc + 1);
                 k_seg_starts<<<blocks(nfar), 256, 0, st>>>(flag, pos, nfar, starts);

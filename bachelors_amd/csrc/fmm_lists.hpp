@@ -9,7 +9,7 @@ struct ListScratch {
     DevBuf far, close, frontA, frontB, counters, key, val, key2, val2, flag, pos, tmp, hs, srec, trec;
     unsigned far_cap = 0, close_cap = 0, front_cap = 0;
     unsigned h_seed[3] = {0, 0, 0};        // host staging of the root pair and its count
-    double h_half[2 * MAX_LEVELS] = {0};   // host staging of the per-level half sizes
+    double h_half[6 * MAX_LEVELS] = {0};   // host staging of the per-level half sizes [tree][level][axis]
 };
 
 // Far pairs sorted by (class, target) -- class = (level difference, centre offset in

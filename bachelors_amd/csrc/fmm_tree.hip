@@ -203,7 +203,8 @@ static void set_level_sizes(TreeDev& T, const double root_h[3], bool cubic_scale
     for (int k = 0; k < MAX_LEVELS; k++) {
         const double* h = T.half[k];
         T.radius[k] = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
-        T.scale[k] = cubic_scale ? h[0] : T.radius[k] / std::sqrt(3.0);
+        // the largest half size keeps every scaled position of a box's points within |u| <= sqrt(3)
+        T.scale[k] = cubic_scale ? std::max(h[0], std::max(h[1], h[2])) : T.radius[k] / std::sqrt(3.0);
     }
 }
 

@@ -13,7 +13,7 @@ struct TreeScratch {
 
 // Builds the reference octree of the n points d_xyz (double3, device) rooted at the box
 // (root_c, root_h).  thr: a node subdivides when its arrivals reach thr.  cubic_scale:
-// expansion scale per level = cube half size (else |half|/sqrt(3)).  One radix sort by
+// expansion scale per level = the largest half size (else |half|/sqrt(3)).  One radix sort by
 // point path when every point's child is unique down to its leaf; otherwise (gaps or
 // overlaps of the float-rounded child boxes) level by level.  T.sorted_build tells which.
 void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const double root_c[3],

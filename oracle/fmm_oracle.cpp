@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <chrono>
 #include <complex>
+#include <omp.h>
 
 namespace orc {
 
@@ -488,7 +489,9 @@ static bool far_enough(const Tree& S, int s, const Tree& T, int t, double mac_sc
     double d2 = dx * dx + dy * dy + dz * dz;
     return md * md < d2;
 }
-static void form_maps(const Tree& S, int s, const Tree& T, int t, double mac, Lists& out) {
+// tie_s: a planted mistake for the tests only (descend the source side on R_t == R_s, i.e.
+// "<=" in the rule below); the restatement is tie_s = false
+static void form_maps(const Tree& S, int s, const Tree& T, int t, double mac, Lists& out, bool tie_s = false) {
     if (!((is_sub(S, s) || is_useful(S, s)) && (is_sub(T, t) || is_useful(T, t)))) return;
     if (far_enough(S, s, T, t, mac)) {
         out.far.insert({t, s});
@@ -498,12 +501,13 @@ static void form_maps(const Tree& S, int s, const Tree& T, int t, double mac, Li
             for (int o = 0; o < 8; o++)
                 if (far_enough(S, S.nodes[s].first_child + o, T, t, mac)) { close_enough = false; break; }
         if (close_enough) out.close.insert({t, s});
-        else for (int o = 0; o < 8; o++) form_maps(S, S.nodes[s].first_child + o, T, t, mac, out);
+        else for (int o = 0; o < 8; o++) form_maps(S, S.nodes[s].first_child + o, T, t, mac, out, tie_s);
     } else {
-        if ((box_radius(T.nodes[t]) < box_radius(S.nodes[s]) || is_useful(T, t)) && !is_useful(S, s)) {
-            for (int o = 0; o < 8; o++) form_maps(S, S.nodes[s].first_child + o, T, t, mac, out);
+        const double rt = box_radius(T.nodes[t]), rs = box_radius(S.nodes[s]);
+        if (((tie_s ? rt <= rs : rt < rs) || is_useful(T, t)) && !is_useful(S, s)) {
+            for (int o = 0; o < 8; o++) form_maps(S, S.nodes[s].first_child + o, T, t, mac, out, tie_s);
         } else {
-            for (int o = 0; o < 8; o++) form_maps(S, s, T, T.nodes[t].first_child + o, mac, out);
+            for (int o = 0; o < 8; o++) form_maps(S, s, T, T.nodes[t].first_child + o, mac, out, tie_s);
         }
     }
 }
@@ -537,6 +541,8 @@ struct Config {
     double mac_scale = 1.0;
     int max_depth = 21;
     double current = 1.0;
+    double fd_eps = FD_EPS;          // FD curl step (reference 1e-3: calculation_point_octree.cpp:217,
+                                     // quadrature_octree.cpp:201)
 };
 struct Run {
     Config cfg;
@@ -546,6 +552,7 @@ struct Run {
     std::vector<double> q;           // per source: w*J (3)
     std::vector<std::vector<double>> M, Lc;   // per storage node: 3*L, component-major
     std::vector<double> B, Bfar, Bnear;       // per target slot (3 each)
+    std::vector<double> A, Afar, Anear;       // vector potential per target slot (calcA(current))
     double t_tree = 0, t_p2m = 0, t_m2m = 0, t_lists = 0, t_m2l = 0, t_l2l = 0, t_eval = 0;
     ~Run() { delete S; delete T; }
 };
@@ -576,7 +583,7 @@ static void m2p_rot(const Run& R, const Tree& S, int s, const double* x, double 
     if (4 * rad * rad < d2) {
         if (R.M[s].empty()) return;
         std::vector<double> h1(L), h2(L), gx(L), gy(L), gz(L);
-        const double e = FD_EPS;
+        const double e = R.cfg.fd_eps;
         solid_harmonics(p, dx + e, dy, dz, false, R.cfg.flags, h1.data());
         solid_harmonics(p, dx - e, dy, dz, false, R.cfg.flags, h2.data());
         for (int i = 0; i < L; i++) gx[i] = h1[i] - h2[i];
@@ -599,6 +606,35 @@ static void m2p_rot(const Run& R, const Tree& S, int s, const double* x, double 
     } else if (n.first_child >= 0) {
         for (int o = 0; o < 8; o++) m2p_rot(R, S, n.first_child + o, x, out);
     }
+}
+// treecode M2P of the vector potential: quadrature_octree.cpp:171-194 (MAC 4R^2 < d^2, recursion
+// into the children, a leaf that fails the MAC adds nothing)
+static void m2p_A(const Run& R, const Tree& S, int s, const double* x, double out[3]) {
+    const int p = R.cfg.p, L = nlen(p);
+    const Node& n = S.nodes[s];
+    double dx = x[0] - n.c[0], dy = x[1] - n.c[1], dz = x[2] - n.c[2];
+    double d2 = dx * dx + dy * dy + dz * dz;
+    double rad = box_radius(n);
+    if (4 * rad * rad < d2) {
+        if (R.M[s].empty()) return;
+        std::vector<double> h(L);
+        solid_harmonics(p, dx, dy, dz, false, R.cfg.flags, h.data());
+        for (int c = 0; c < 3; c++) {
+            double t = 0;
+            for (int i = 0; i < L; i++) t += R.M[s][c * L + i] * h[i];
+            out[c] += t;
+        }
+    } else if (n.first_child >= 0) {
+        for (int o = 0; o < 8; o++) m2p_A(R, S, n.first_child + o, x, out);
+    }
+}
+// direct vector potential of one source: q / |x - y| (the A counterpart of bs_kernel)
+inline void a_kernel(const double* x, const double* y, const double q[3], double acc[3]) {
+    double d[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (r2 == 0) return;
+    double r = std::sqrt(r2);
+    for (int c = 0; c < 3; c++) acc[c] += q[c] / r;
 }
 static void collect_points(const Tree& S, int s, std::vector<int>& out) {
     const Node& n = S.nodes[s];
@@ -630,7 +666,11 @@ static Run* run_fmm(const Config& cfg, size_t ns, const double* src, const doubl
     Tree& T = *R->T;
     // P2M at useful leaves (multipole_solver.cpp:56-95)
     R->M.assign(S.nodes.size(), {});
-    for (size_t s = 0; s < S.nodes.size(); s++)
+    // OpenMP loops here and below split independent outputs only (leaves, parents of one
+    // orientation, targets, levels): every sum keeps the serial order, so the results are the
+    // same for any thread count
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long s = 0; s < (long long)S.nodes.size(); s++)
         if (is_useful(S, (int)s)) p2m_leaf(*R, S, (int)s, R->M[s]);
     auto t2 = clk::now();
     R->t_p2m = secs(t1, t2);
@@ -660,7 +700,9 @@ static Run* run_fmm(const Config& cfg, size_t ns, const double* src, const doubl
                 double t[3] = {S.nodes[s0].c[0] - S.nodes[par0].c[0], S.nodes[s0].c[1] - S.nodes[par0].c[1],
                                S.nodes[s0].c[2] - S.nodes[par0].c[2]};
                 m2m_matrix_operator(p, t, cfg.flags, Top.data());
-                for (int s : by_o[o]) {
+#pragma omp parallel for schedule(static)
+                for (long long si = 0; si < (long long)by_o[o].size(); si++) {
+                    const int s = by_o[o][si];
                     int par = S.nodes[s].parent;
                     for (int c = 0; c < 3; c++)
                         for (int i = 0; i < L; i++) {
@@ -685,15 +727,24 @@ static Run* run_fmm(const Config& cfg, size_t ns, const double* src, const doubl
     for (size_t t = 0; t < T.nodes.size(); t++)
         if (is_sub(T, (int)t) || is_useful(T, (int)t)) R->Lc[t].assign(3 * L, 0.0);
     {
-        std::vector<double> out(L);
+        // far pairs grouped by target (the set is ordered by (target, source))
+        std::vector<std::pair<int, std::vector<int>>> by_t;
         for (auto& pr : R->lists.far) {
-            int t = pr.first, s = pr.second;
-            double tv[3] = {S.nodes[s].c[0] - T.nodes[t].c[0], S.nodes[s].c[1] - T.nodes[t].c[1],
-                            S.nodes[s].c[2] - T.nodes[t].c[2]};
-            Translator tr(OP_M2L, p, tv, cfg.flags);
-            for (int c = 0; c < 3; c++) {
-                tr.apply(R->M[s].data() + c * L, out.data());
-                for (int i = 0; i < L; i++) R->Lc[t][c * L + i] += out[i];
+            if (by_t.empty() || by_t.back().first != pr.first) by_t.push_back({pr.first, {}});
+            by_t.back().second.push_back(pr.second);
+        }
+#pragma omp parallel for schedule(dynamic, 4)
+        for (long long g = 0; g < (long long)by_t.size(); g++) {
+            std::vector<double> out(L);
+            const int t = by_t[g].first;
+            for (int s : by_t[g].second) {
+                double tv[3] = {S.nodes[s].c[0] - T.nodes[t].c[0], S.nodes[s].c[1] - T.nodes[t].c[1],
+                                S.nodes[s].c[2] - T.nodes[t].c[2]};
+                Translator tr(OP_M2L, p, tv, cfg.flags);
+                for (int c = 0; c < 3; c++) {
+                    tr.apply(R->M[s].data() + c * L, out.data());
+                    for (int i = 0; i < L; i++) R->Lc[t][c * L + i] += out[i];
+                }
             }
         }
     }
@@ -701,9 +752,19 @@ static Run* run_fmm(const Config& cfg, size_t ns, const double* src, const doubl
     R->t_m2l = secs(t4, t5);
     // L2L, complex path (calculation_point_octree.cpp:277-299), pre-order
     {
-        std::vector<double> out(L);
+        // level by level (a child receives from its parent only)
+        std::vector<std::vector<int>> by_level;
         for (int b : T.bfs) {
             if (!is_sub(T, b) || R->Lc[b].empty()) continue;
+            int lv = T.nodes[b].level;
+            if ((int)by_level.size() <= lv) by_level.resize(lv + 1);
+            by_level[lv].push_back(b);
+        }
+        for (auto& lvl_nodes : by_level)
+#pragma omp parallel for schedule(dynamic, 8)
+        for (long long bi = 0; bi < (long long)lvl_nodes.size(); bi++) {
+            std::vector<double> out(L);
+            const int b = lvl_nodes[bi];
             for (int o = 0; o < 8; o++) {
                 int ch = T.nodes[b].first_child + o;
                 if (!(is_sub(T, ch) || is_useful(T, ch))) continue;
@@ -724,13 +785,20 @@ static Run* run_fmm(const Config& cfg, size_t ns, const double* src, const doubl
     for (auto& pr : R->lists.close) close_of[pr.first].push_back(pr.second);
     const size_t nslots = T.slots.size();
     R->B.assign(3 * nslots, 0);
+    R->A.assign(3 * nslots, 0);
+    R->Afar.assign(3 * nslots, 0);
+    R->Anear.assign(3 * nslots, 0);
     R->Bfar.assign(3 * nslots, 0);
     R->Bnear.assign(3 * nslots, 0);
     const double scale = cfg.current * MU0 / (4.0 * PI);
-    std::vector<double> h1(L), h2(L), g(3 * L);
-    std::vector<std::vector<double>> gd(3, std::vector<double>(L));
-    for (int b : T.bfs) {
-        if (!is_useful(T, b)) continue;
+    std::vector<int> leaves;
+    for (int b : T.bfs)
+        if (is_useful(T, b)) leaves.push_back(b);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long long li = 0; li < (long long)leaves.size(); li++) {
+        std::vector<double> h1(L), h2(L), g(3 * L);
+        std::vector<std::vector<double>> gd(3, std::vector<double>(L));
+        const int b = leaves[li];
         const Node& n = T.nodes[b];
         const double* Lx = R->Lc[b].data();
         const double* Ly = Lx + L;
@@ -753,7 +821,7 @@ static Run* run_fmm(const Config& cfg, size_t ns, const double* src, const doubl
                     far[2] += Ly[i] * g[3 * i + 0] - Lx[i] * g[3 * i + 1];
                 }
             } else {
-                const double e = FD_EPS;
+                const double e = cfg.fd_eps;
                 double off[3][3] = {{e, 0, 0}, {0, e, 0}, {0, 0, e}};
                 for (int a = 0; a < 3; a++) {
                     solid_harmonics(p, dx + off[a][0], dy + off[a][1], dz + off[a][2], true, cfg.flags, h1.data());
@@ -781,6 +849,29 @@ static Run* run_fmm(const Config& cfg, size_t ns, const double* src, const doubl
                 R->Bnear[3 * k + c] = nearv[c] * scale;
                 R->B[3 * k + c] = (far[c] + nearv[c]) / (4.0 * PI) * cfg.current * MU0;
             }
+            // calcA(current) (fast_multipole_solver.cpp:195-212): far part sum L R(x - c)
+            // (calculation_point_octree.cpp:192-212), close nodes by M2P (quadrature_octree.cpp:
+            // 171-194) or, corrected, the direct 1/|x - y| sum; scaled by current / (4 pi), no MU0
+            {
+                std::vector<double>& Rh = h1;
+                solid_harmonics(p, dx, dy, dz, true, cfg.flags, Rh.data());
+                double af[3] = {0, 0, 0}, an[3] = {0, 0, 0};
+                for (int c = 0; c < 3; c++)
+                    for (int i = 0; i < L; i++) af[c] += R->Lc[b][c * L + i] * Rh[i];
+                if (cfg.flags & P2P) {
+                    for (int j : near_pts) a_kernel(x, S.P + 3 * (size_t)j, R->q.data() + 3 * (size_t)j, an);
+                } else {
+                    auto it = close_of.find(b);
+                    if (it != close_of.end())
+                        for (int s : it->second) m2p_A(*R, S, s, x, an);
+                }
+                const double sa = cfg.current / (4.0 * PI);
+                for (int c = 0; c < 3; c++) {
+                    R->Afar[3 * k + c] = af[c] * sa;
+                    R->Anear[3 * k + c] = an[c] * sa;
+                    R->A[3 * k + c] = (af[c] + an[c]) / (4.0 * PI) * cfg.current;
+                }
+            }
         }
     }
     auto t7 = clk::now();
@@ -797,6 +888,9 @@ using namespace orc;
 extern "C" {
 
 int orc_nlen(int p) { return nlen(p); }
+// OpenMP threads of the parallel loops (results do not depend on the count)
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads() { return omp_get_max_threads(); }
 
 void orc_solid_harmonics(int order, const double* v, int regular, int flags, double* out) {
     solid_harmonics(order, v[0], v[1], v[2], regular != 0, flags, out);
@@ -885,6 +979,12 @@ void* orc_lists_build(void* S, void* T, double mac_scale) {
     form_maps(*(Tree*)S, 0, *(Tree*)T, 0, mac_scale, *L);
     return L;
 }
+// the same with the planted tie mistake (tests of the list checker only)
+void* orc_lists_build_tie_s(void* S, void* T, double mac_scale) {
+    Lists* L = new Lists();
+    form_maps(*(Tree*)S, 0, *(Tree*)T, 0, mac_scale, *L, true);
+    return L;
+}
 void orc_lists_free(void* h) { delete (Lists*)h; }
 void orc_lists_counts(void* h, size_t* nfar, size_t* nclose) {
     *nfar = ((Lists*)h)->far.size();
@@ -905,7 +1005,7 @@ void orc_lists_get(void* h, void* S, void* T, int* far_t, int* far_s, int* close
 struct orc_config {
     int p, cap_s, cap_t, flags, target_root_mode, max_depth;
     double target_root_c[3], target_root_h[3];
-    double mac_scale, current;
+    double mac_scale, current, fd_eps;
 };
 void* orc_fmm_run(const orc_config* c, size_t ns, const double* src, const double* w, const double* J, size_t nt,
                   const double* tgt) {
@@ -919,6 +1019,7 @@ void* orc_fmm_run(const orc_config* c, size_t ns, const double* src, const doubl
     for (int a = 0; a < 3; a++) { cfg.target_root_c[a] = c->target_root_c[a]; cfg.target_root_h[a] = c->target_root_h[a]; }
     cfg.mac_scale = c->mac_scale;
     cfg.current = c->current;
+    cfg.fd_eps = c->fd_eps > 0 ? c->fd_eps : FD_EPS;
     return run_fmm(cfg, ns, src, w, J, nt, tgt);
 }
 void orc_fmm_free(void* h) { delete (Run*)h; }
@@ -930,6 +1031,30 @@ void orc_fmm_get(void* h, int* slots, double* B, double* Bfar, double* Bnear) {
     std::memcpy(B, R.B.data(), 3 * n * sizeof(double));
     std::memcpy(Bfar, R.Bfar.data(), 3 * n * sizeof(double));
     std::memcpy(Bnear, R.Bnear.data(), 3 * n * sizeof(double));
+}
+void orc_fmm_get_A(void* h, double* A, double* Afar, double* Anear) {
+    Run& R = *(Run*)h;
+    size_t n = R.T->slots.size();
+    std::memcpy(A, R.A.data(), 3 * n * sizeof(double));
+    std::memcpy(Afar, R.Afar.data(), 3 * n * sizeof(double));
+    std::memcpy(Anear, R.Anear.data(), 3 * n * sizeof(double));
+}
+// treecode single-point evaluation over the run's source tree from node bfs_id (0 = root):
+// MultipoleSolver::calcB(current, point) = root calcRot * current MU0 / (4 pi) and
+// calcA(current, point) = root calcA * current / (4 pi) (multipole_solver.cpp:686-700)
+void orc_treecode(void* h, int bfs_id, size_t n, const double* x, double* B, double* A) {
+    Run& R = *(Run*)h;
+    const int s = R.S->bfs[bfs_id];
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long i = 0; i < (long long)n; i++) {
+        double b[3] = {0, 0, 0}, a[3] = {0, 0, 0};
+        m2p_rot(R, *R.S, s, x + 3 * i, b);
+        m2p_A(R, *R.S, s, x + 3 * i, a);
+        for (int c = 0; c < 3; c++) {
+            B[3 * i + c] = b[c] / (4.0 * PI) * R.cfg.current * MU0;
+            A[3 * i + c] = a[c] / (4.0 * PI) * R.cfg.current;
+        }
+    }
 }
 void orc_fmm_times(void* h, double* t) {
     Run& R = *(Run*)h;
@@ -973,6 +1098,25 @@ void orc_direct_B(size_t ns, const double* src, const double* w, const double* J
         double acc[3] = {0, 0, 0};
         for (size_t i = 0; i < ns; i++) bs_kernel(tgt + 3 * t, src + 3 * i, q.data() + 3 * i, acc);
         for (int c = 0; c < 3; c++) B[3 * t + c] = acc[c] * scale;
+    }
+}
+// A(x) = current / (4 PI) sum_q w_q J_q / |x - y_q| (the calcA scaling of the reference:
+// multipole_solver.cpp:686-692, fast_multipole_solver.cpp:195-212; no MU0)
+void orc_direct_A(size_t ns, const double* src, const double* w, const double* J, size_t nt, const double* tgt,
+                  double current, double* A) {
+    std::vector<double> q(3 * ns);
+    for (size_t i = 0; i < ns; i++) {
+        double j[3];
+        if (J) { j[0] = J[3 * i]; j[1] = J[3 * i + 1]; j[2] = J[3 * i + 2]; }
+        else phi_hat(src + 3 * i, j);
+        for (int c = 0; c < 3; c++) q[3 * i + c] = w[i] * j[c];
+    }
+    const double scale = current / (4 * PI);
+#pragma omp parallel for schedule(static)
+    for (long long t = 0; t < (long long)nt; t++) {
+        double acc[3] = {0, 0, 0};
+        for (size_t i = 0; i < ns; i++) a_kernel(tgt + 3 * t, src + 3 * i, q.data() + 3 * i, acc);
+        for (int c = 0; c < 3; c++) A[3 * t + c] = acc[c] * scale;
     }
 }
 // dL/dw_j for L = sum_i |B_i - Bstar_i|^2 with B from the direct sum.

@@ -46,12 +46,22 @@ def lib():
         L.orc_lists_build.restype = ctypes.c_void_p
         L.orc_fmm_run.restype = ctypes.c_void_p
         for name in ("orc_tree_free", "orc_tree_num_nodes", "orc_tree_num_slots", "orc_tree_get",
-                     "orc_lists_free", "orc_lists_counts", "orc_lists_get", "orc_fmm_free",
+                     "orc_lists_free", "orc_lists_counts", "orc_lists_get", "orc_fmm_free", "orc_fmm_get_A",
+                     "orc_treecode",
                      "orc_fmm_num_slots", "orc_fmm_get", "orc_fmm_times", "orc_fmm_counts",
                      "orc_fmm_local", "orc_fmm_multipole"):
             getattr(L, name).argtypes = None
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (the results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
 
 
 def _d(a):
@@ -173,9 +183,13 @@ def target_tree(P, cap, root=None, max_depth=21):
     return Tree(P, cap + 1, root=root, max_depth=max_depth)
 
 
-def interaction_lists(S, T, mac_scale=1.0):
+def interaction_lists(S, T, mac_scale=1.0, planted_tie_mistake=False):
+    """Far and close pairs (target, source) in BFS ids.  planted_tie_mistake: the rule with "<="
+    in place of "<" (a deliberate error, for showing that the list checks catch it)."""
     L = lib()
-    h = ctypes.c_void_p(L.orc_lists_build(S.h, T.h, ctypes.c_double(mac_scale)))
+    build = L.orc_lists_build_tie_s if planted_tie_mistake else L.orc_lists_build
+    build.restype = ctypes.c_void_p
+    h = ctypes.c_void_p(build(S.h, T.h, ctypes.c_double(mac_scale)))
     nf = ctypes.c_size_t()
     nc = ctypes.c_size_t()
     L.orc_lists_counts(h, ctypes.byref(nf), ctypes.byref(nc))
@@ -192,14 +206,14 @@ class _Cfg(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("cap_s", ctypes.c_int), ("cap_t", ctypes.c_int), ("flags", ctypes.c_int),
                 ("target_root_mode", ctypes.c_int), ("max_depth", ctypes.c_int),
                 ("target_root_c", ctypes.c_double * 3), ("target_root_h", ctypes.c_double * 3),
-                ("mac_scale", ctypes.c_double), ("current", ctypes.c_double)]
+                ("mac_scale", ctypes.c_double), ("current", ctypes.c_double), ("fd_eps", ctypes.c_double)]
 
 
 class FMMRun:
     """One reference FMM evaluation (FastMultipoleSolver restated)."""
 
     def __init__(self, src, w, tgt, p=10, cap_s=32, cap_t=32, flags=FAITHFUL, J=None, target_root=None,
-                 mac_scale=1.0, current=1.0, max_depth=21):
+                 mac_scale=1.0, current=1.0, max_depth=21, fd_eps=1e-3):
         src = np.ascontiguousarray(src, dtype=np.float64)
         tgt = np.ascontiguousarray(tgt, dtype=np.float64)
         w = np.ascontiguousarray(w, dtype=np.float64)
@@ -215,6 +229,7 @@ class FMMRun:
                 c.target_root_h[a] = target_root[1][a]
         c.mac_scale = mac_scale
         c.current = current
+        c.fd_eps = fd_eps
         Jp = None
         if J is not None:
             J = np.ascontiguousarray(J, dtype=np.float64)
@@ -229,6 +244,10 @@ class FMMRun:
         self.Bfar = np.zeros((n, 3))
         self.Bnear = np.zeros((n, 3))
         L.orc_fmm_get(self.h, _i(self.slots), _d(self.B), _d(self.Bfar), _d(self.Bnear))
+        self.A = np.zeros((n, 3))
+        self.Afar = np.zeros((n, 3))
+        self.Anear = np.zeros((n, 3))
+        L.orc_fmm_get_A(self.h, _d(self.A), _d(self.Afar), _d(self.Anear))
         t = np.zeros(7)
         L.orc_fmm_times(self.h, _d(t))
         self.times = dict(zip(["tree", "p2m", "m2m", "lists", "m2l", "l2l", "eval"], t.tolist()))
@@ -251,6 +270,20 @@ class FMMRun:
         out[self.slots] = self.B
         return out
 
+    def A_input_order(self, nt):
+        out = np.full((nt, 3), np.nan)
+        out[self.slots] = self.A
+        return out
+
+    def treecode(self, x, node=0):
+        """MultipoleSolver::calcB / calcA (current, point) at each row of x: the treecode M2P
+        recursion from source node `node` (BFS id; 0 = root), FD curl for B."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        B = np.zeros((len(x), 3))
+        A = np.zeros((len(x), 3))
+        lib().orc_treecode(self.h, int(node), ctypes.c_size_t(len(x)), _d(x), _d(B), _d(A))
+        return B, A
+
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:
             _lib.orc_fmm_free(self.h)
@@ -269,6 +302,21 @@ def direct_B(src, w, tgt, J=None, current=1.0):
     lib().orc_direct_B(ctypes.c_size_t(len(src)), _d(src), _d(w), Jp, ctypes.c_size_t(len(tgt)), _d(tgt),
                        ctypes.c_double(current), _d(B))
     return B
+
+
+def direct_A(src, w, tgt, J=None, current=1.0):
+    """A(x) = current / (4 pi) sum w J / |x - y| (the reference's calcA scaling, no MU0)."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    tgt = np.ascontiguousarray(tgt, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    A = np.zeros((len(tgt), 3))
+    Jp = None
+    if J is not None:
+        J = np.ascontiguousarray(J, dtype=np.float64)
+        Jp = _d(J)
+    lib().orc_direct_A(ctypes.c_size_t(len(src)), _d(src), _d(w), Jp, ctypes.c_size_t(len(tgt)), _d(tgt),
+                       ctypes.c_double(current), _d(A))
+    return A
 
 
 def direct_grad_w(src, w, tgt, Bstar, J=None, current=1.0):

@@ -38,6 +38,19 @@ def test_config_defaults_match_reference():
     assert c.target_leaf_capacity == 100     # fast_multipole_solver.hpp:30
     assert c.mac_scale == 1.0
     assert list(c.source_root_half) == [3.0, 3.0, 3.0]   # multipole_solver.cpp:662-663
+    assert list(c.source_root_center) == [0.0, 0.0, 0.0]
+    # the reference's target tree (bounding-box root, math.cpp:39-57) and leaf order
+    assert c.frame == bachelors_amd.FRAME_REFERENCE
+    assert c.max_depth == 21 and c.precision == bachelors_amd.PREC_FP64
+
+
+def test_drop_in_defaults_to_the_reference_frame():
+    import inspect
+    import bachelors_amd
+    sig = inspect.signature(bachelors_amd.FastMultipoleSolver.__init__)
+    assert sig.parameters["frame"].default == bachelors_amd.FRAME_REFERENCE
+    assert sig.parameters["quadratureOctreeLeafCapacity"].default == 1000   # fast_multipole_solver.hpp:29
+    assert sig.parameters["calculationPointOctreeLeafCapacity"].default == 100
 
 
 def test_create_rejects_bad_arguments():

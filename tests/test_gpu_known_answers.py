@@ -19,7 +19,8 @@ def test_fmm_torus_against_oracle_direct_and_telma(orc, precision, tol):
     pts, Bt = telma()
     # the reference's own settings of comparisonToTelmaIntegrals: capS 1000, capT 32 (main.cpp:72)
     s = fm.FastMultipoleSolver(np.column_stack([P, W]), pts, quadratureOctreeLeafCapacity=1000,
-                               calculationPointOctreeLeafCapacity=32, order=10, precision=precision)
+                               calculationPointOctreeLeafCapacity=32, order=10, precision=precision,
+                               frame=fm.FRAME_SHARED)
     s.calcMultipoleExpansionsAtLeaves()
     s.calclMultipoleExpansions(fm.M2MAlg.Matrices, fm.Device.GPU)
     s.calcLocalMultipoleExpansions(fm.M2LAlg.ComplexTranslation, fm.Device.CPU)

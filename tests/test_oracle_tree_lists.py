@@ -83,6 +83,39 @@ def test_target_tree_capacity_and_bbox_root(orc):
     assert T.first_child[0] >= 0
 
 
+def _target_rule_violations(T, P, cap, max_depth=21):
+    """Nodes of a target tree on a dyadic root (children tile their parent exactly, so a node's
+    arrivals are the points in its box) that break calculation_point_octree.cpp:41: a node
+    subdivides exactly when its population reaches cap + 1 (a leaf holds at most cap)."""
+    bad = []
+    for b in range(len(T.level)):
+        cnt = int(_in_box(P, T.center[b], T.half[b]).sum())
+        if (T.first_child[b] >= 0) != (cnt >= cap + 1 and T.level[b] < max_depth) or T.end[b] - T.begin[b] != cnt:
+            bad.append((b, cnt))
+    return bad
+
+
+@pytest.mark.parametrize("kind", ["uniform", "adversarial"])
+def test_target_tree_threshold_by_brute_force(orc, kind):
+    """Target tree on the dyadic root [-3, 3)^3: subdivided <=> population >= cap + 1, checked by
+    recounting every node's box; the data holds nodes of exactly cap and cap + 1 points, so an
+    off-by-one threshold cannot pass (planted below)."""
+    rng = np.random.default_rng(31)
+    P = rng.uniform(-2, 2, (4000, 3)) if kind == "uniform" else _adversarial(rng, 4000)
+    cap = 12
+    root = (np.zeros(3), np.full(3, 3.0))
+    T = orc.target_tree(P, cap, root=root)
+    assert sorted(T.slots.tolist()) == list(range(len(P)))
+    assert _target_rule_violations(T, P, cap) == []
+    pops = [int(_in_box(P, T.center[b], T.half[b]).sum()) for b in range(len(T.level))]
+    leaf_pops = {c for b, c in enumerate(pops) if T.first_child[b] < 0}
+    sub_pops = {c for b, c in enumerate(pops) if T.first_child[b] >= 0}
+    assert cap in leaf_pops and cap + 1 in sub_pops        # both sides of the threshold occur
+    # planted mistake: the source tree's threshold (subdivide at cap) on the same points
+    wrong = orc.Tree(P, cap, root=root)
+    assert _target_rule_violations(wrong, P, cap) != []
+
+
 def _leaf_of(T, n):
     leaf = np.full(n, -1)
     for b in range(len(T.level)):
@@ -153,3 +186,75 @@ def test_gap_overlap_targets_are_duplicated_or_dropped(orc):
     counts = np.bincount(T.slots, minlength=len(aug))
     extra = counts[len(tgt):]
     assert np.any(extra != 1)
+
+
+def _rule_violations(S, T, ft, fs, ct, cs, mac_scale=1.0):
+    """Checks every far and close pair against the descent rule of formInteractionMaps
+    (fast_multipole_solver.cpp:105-165) bottom-up: the pair's own fate is far / close, and some
+    chain of generating pairs leads back to the root pair, each link being a pair whose fate under
+    the rule is to descend into the side that produced the next link.  The rule at a pair (t, s)
+    that is neither far nor close: descend into s's children when (R_t < R_s or t is a useful
+    leaf) and s is not a useful leaf, else into t's children; at a useful t it descends into s when
+    some child of s is far."""
+    rad_s = np.linalg.norm(S.half, axis=1)
+    rad_t = np.linalg.norm(T.half, axis=1)
+    useful_s = (S.first_child < 0) & (S.npts > 0)
+    useful_t = (T.first_child < 0) & (T.npts > 0)
+
+    def far(s, t):
+        md = (2 * max(rad_s[s], rad_t[t]) + min(rad_s[s], rad_t[t])) * mac_scale
+        d = S.center[s] - T.center[t]
+        return md * md < d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+
+    def fate(t, s):
+        if far(s, t):
+            return "far"
+        if useful_t[t]:
+            fc = S.first_child[s]
+            if fc < 0 or not any(far(fc + o, t) for o in range(8)):
+                return "close"
+            return "desc_s"
+        if (rad_t[t] < rad_s[s] or useful_t[t]) and not useful_s[s]:
+            return "desc_s"
+        return "desc_t"
+
+    memo = {}
+
+    def reached(t, s):
+        if t == 0 and s == 0:
+            return True
+        if (t, s) in memo:
+            return memo[(t, s)]
+        ok = False
+        if s != 0 and fate(t, S.parent[s]) == "desc_s" and reached(t, S.parent[s]):
+            ok = True
+        elif t != 0 and fate(T.parent[t], s) == "desc_t" and reached(T.parent[t], s):
+            ok = True
+        memo[(t, s)] = ok
+        return ok
+
+    bad = [("far", t, s) for t, s in zip(ft, fs) if fate(t, s) != "far" or not reached(t, s)]
+    bad += [("close", t, s) for t, s in zip(ct, cs) if fate(t, s) != "close" or not reached(t, s)]
+    return bad
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_interaction_lists_follow_the_descent_rule(orc, shared):
+    """In the shared frame source and target boxes of one level tie (R_t == R_s), the case where
+    the rule's strict '<' decides which side descends; the planted '<=' mistake must change the
+    lists and be caught."""
+    rng = np.random.default_rng(14)
+    src = shell_sources(rng, 2500)
+    tgt = rng.uniform(-2, 2, (800, 3))
+    S = orc.source_tree(src, 12)
+    T = orc.target_tree(tgt, 10, root=(np.zeros(3), np.full(3, 3.0)) if shared else None)
+    (ft, fs), (ct, cs) = orc.interaction_lists(S, T)
+    assert len(ft) > 1000 and len(ct) > 100
+    assert _rule_violations(S, T, ft, fs, ct, cs) == []
+    if shared:
+        rad_s = np.linalg.norm(S.half, axis=1)
+        rad_t = np.linalg.norm(T.half, axis=1)
+        assert np.sum(rad_s[fs] == rad_t[ft]) > 100        # same-size boxes tie in this frame
+        (wt, ws), (wct, wcs) = orc.interaction_lists(S, T, planted_tie_mistake=True)
+        assert set(zip(wt, ws)) != set(zip(ft, fs))
+        assert _rule_violations(S, T, wt, ws, wct, wcs) != []
