@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libbachelors_fmm.so")
 FMM_OK = 0
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_F16X3 = 0, 1, 2, 3
 FRAME_REFERENCE, FRAME_SHARED = 0, 1
+NEAR_NONE, NEAR_P2P, NEAR_TREECODE = 0, 1, 2
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "f16x3": PREC_F16X3}
 
 
@@ -92,6 +93,7 @@ def lib():
         L.fmm_config_init.argtypes = [ctypes.POINTER(Config)]
         for name in ("fmm_build_trees", "fmm_leaf_expansions", "fmm_upward", "fmm_downward", "fmm_synchronize"):
             getattr(L, name).argtypes = [_vp]
+        L.fmm_upward_with.argtypes = [_vp, ctypes.c_int]
         L.fmm_set_sources.argtypes = [_vp, ctypes.c_size_t, _vp, _vp, _vp]
         L.fmm_set_sources_dev.argtypes = [_vp, ctypes.c_size_t, _vp, _vp, _vp]
         L.fmm_set_targets.argtypes = [_vp, ctypes.c_size_t, _vp]
@@ -104,6 +106,11 @@ def lib():
         L.fmm_leaf_order.argtypes = [_vp, _vp]
         L.fmm_adjoint_weights_dev.argtypes = [_vp, ctypes.c_double, _vp, _vp]
         L.fmm_direct_B_dev.argtypes = [_vp, ctypes.c_double, ctypes.c_size_t, _vp, _vp]
+        L.fmm_direct_A_dev.argtypes = [_vp, ctypes.c_double, ctypes.c_size_t, _vp, _vp]
+        L.fmm_evaluate_A.argtypes = [_vp, ctypes.c_double, _vp]
+        L.fmm_evaluate_A_dev.argtypes = [_vp, ctypes.c_double, _vp]
+        L.fmm_treecode.argtypes = [_vp, ctypes.c_double, ctypes.c_size_t, _vp, _vp, _vp]
+        L.fmm_treecode_dev.argtypes = [_vp, ctypes.c_double, ctypes.c_size_t, _vp, _vp, _vp]
         L.fmm_get_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
         L.fmm_tree_size.argtypes = [_vp, ctypes.c_int, _ip, _ip]
         L.fmm_tree_export.argtypes = [_vp, ctypes.c_int] + [_vp] * 8
@@ -195,8 +202,12 @@ class FMM:
     def leaf_expansions(self):
         self._check(lib().fmm_leaf_expansions(self.h))
 
-    def upward(self):
-        self._check(lib().fmm_upward(self.h))
+    def upward(self, algorithm=None):
+        """M2M; algorithm: an M2MAlg (default Matrices, the MFMA GEMM path)."""
+        if algorithm is None:
+            self._check(lib().fmm_upward(self.h))
+        else:
+            self._check(lib().fmm_upward_with(self.h, int(algorithm)))
 
     def downward(self):
         self._check(lib().fmm_downward(self.h))
@@ -208,6 +219,26 @@ class FMM:
 
     def evaluate_B_dev(self, current, B_ptr):
         self._check(lib().fmm_evaluate_B_dev(self.h, current, B_ptr))
+
+    def evaluate_A(self, current=1.0):
+        """Vector potential at every target, input order (calcA(current) scaling: no MU0)."""
+        A = np.zeros((self.nt, 3))
+        self._check(lib().fmm_evaluate_A(self.h, current, _ptr(A)))
+        return A
+
+    def evaluate_A_dev(self, current, A_ptr):
+        self._check(lib().fmm_evaluate_A_dev(self.h, current, A_ptr))
+
+    def treecode(self, x, current=1.0):
+        """Single-point treecode B and A (MultipoleSolver::calcB / calcA(current, point)) at each row."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        B = np.zeros((len(x), 3))
+        A = np.zeros((len(x), 3))
+        self._check(lib().fmm_treecode(self.h, current, len(x), _ptr(x), _ptr(B), _ptr(A)))
+        return B, A
+
+    def treecode_dev(self, current, m, xyz_ptr, B_ptr, A_ptr):
+        self._check(lib().fmm_treecode_dev(self.h, current, m, xyz_ptr, B_ptr, A_ptr))
 
     def run_B_dev(self, current, B_ptr):
         self._check(lib().fmm_run_B_dev(self.h, current, B_ptr))
@@ -227,6 +258,9 @@ class FMM:
 
     def direct_B_dev(self, current, m, xyz_ptr, B_ptr):
         self._check(lib().fmm_direct_B_dev(self.h, current, m, xyz_ptr, B_ptr))
+
+    def direct_A_dev(self, current, m, xyz_ptr, A_ptr):
+        self._check(lib().fmm_direct_A_dev(self.h, current, m, xyz_ptr, A_ptr))
 
     def leaf_order(self):
         n = self.tree(1)["slots"].shape[0]
@@ -301,7 +335,8 @@ class FastMultipoleSolver:
 
     def __init__(self, quadratures, points, problem=Problem.BioSavartLaplace, quadratureOctreeLeafCapacity=1000,
                  calculationPointOctreeLeafCapacity=100, weights=None, currents=None, order=10,
-                 precision="fp64", frame=FRAME_REFERENCE, mac_scale=1.0, m2l_full=True, near_field=True):
+                 precision="fp64", frame=FRAME_REFERENCE, mac_scale=1.0, m2l_full=True, near_field=NEAR_P2P,
+                 max_depth=None):
         if problem != Problem.BioSavartLaplace:
             raise FMMError("only Problem.BioSavartLaplace is implemented")
         q = np.asarray(quadratures, np.float64)
@@ -317,7 +352,8 @@ class FastMultipoleSolver:
         self._points = np.ascontiguousarray(points, np.float64)
         self._fmm = FMM(order=order, source_leaf_capacity=quadratureOctreeLeafCapacity,
                         target_leaf_capacity=calculationPointOctreeLeafCapacity, precision=precision, frame=frame,
-                        mac_scale=mac_scale, m2l_full=int(m2l_full), near_field=int(near_field))
+                        mac_scale=mac_scale, m2l_full=int(m2l_full), near_field=int(near_field),
+                        **({} if max_depth is None else {"max_depth": max_depth}))
         self._fmm.set_sources(xyz, w, currents)
         self._fmm.set_targets(self._points)
         self._fmm.build_trees()              # the constructor builds both trees (initTrees)
@@ -330,11 +366,13 @@ class FastMultipoleSolver:
         self._leaves_ready = True
 
     def calclMultipoleExpansions(self, algorithm=M2MAlg.Matrices, device=Device.CPU):
-        # every M2M algorithm of the reference computes the same expansions; here all run
-        # as batched MFMA GEMMs on the GPU
+        """M2M by the reference's algorithm choice (multipole_solver.cpp:102-139), all on the GPU:
+        Matrices = MFMA GEMMs of per-(level, orientation) operators; ComplexTranslation,
+        RealTranslation, Layers = the per-node translation formula; NoTranslation = P2M at every
+        node.  The device argument is accepted for source compatibility."""
         if not self._leaves_ready:
             self.calcMultipoleExpansionsAtLeaves()
-        self._fmm.upward()
+        self._fmm.upward(M2MAlg(algorithm))
         self._multipoles_ready = True
 
     def calcLocalMultipoleExpansions(self, algorithm=M2LAlg.ComplexTranslation, device=Device.CPU):
@@ -343,11 +381,44 @@ class FastMultipoleSolver:
         self._fmm.downward()
         self._locals_ready = True
 
-    def calcB(self, current):
-        """Reference output: list of (point, B) in target-leaf order (FMMResult order)."""
+    def getQuadratureOctreeNodeCount(self):
+        """All source-tree nodes, empty children included (multipole_solver.cpp:97-100)."""
+        return self._fmm.tree(0)["level"].shape[0]
+
+    def calcB(self, current, point=None):
+        """calcB(current): list of (point, B) in target-leaf order (FMMResult order,
+        fast_multipole_solver.cpp:214-231).  calcB(current, point): the treecode B at one point
+        (multipole_solver.cpp:694-700), after calclMultipoleExpansions."""
+        if point is not None:
+            self._need_multipoles()
+            return self._fmm.treecode(np.asarray(point, np.float64), current)[0][0]
         B = self._fmm.evaluate_B(current)
         order = self._fmm.leaf_order()
         return [(self._points[i], B[i]) for i in order]
+
+    def calcA(self, current, point=None):
+        """calcA(current): list of (point, A) in target-leaf order (fast_multipole_solver.cpp:195-212,
+        A = current / (4 pi) sum w J / |x - y|).  calcA(current, point): the treecode A at one point
+        (multipole_solver.cpp:686-692)."""
+        if point is not None:
+            self._need_multipoles()
+            return self._fmm.treecode(np.asarray(point, np.float64), current)[1][0]
+        A = self._fmm.evaluate_A(current)
+        order = self._fmm.leaf_order()
+        return [(self._points[i], A[i]) for i in order]
+
+    def calcA_array(self, current):
+        """A at every target in input order, (M, 3)."""
+        return self._fmm.evaluate_A(current)
+
+    def treecode(self, current, points):
+        """calcB and calcA (current, point) at many points at once: (B, A), each (n, 3)."""
+        self._need_multipoles()
+        return self._fmm.treecode(points, current)
+
+    def _need_multipoles(self):
+        if not self._multipoles_ready:
+            raise FMMError("Multipoles are not ready!")      # multipole_solver.cpp:689,697
 
     def calcB_array(self, current):
         """B at every target in input order, (M, 3)."""

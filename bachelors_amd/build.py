@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbachelors_fmm.so")
 SOURCES = ["fmm_prims.hip", "fmm_tree.hip", "fmm_lists.hip", "fmm_tables.hip", "fmm_gemm.hip", "fmm_expand.hip",
-           "fmm_adjoint.hip", "fmm_capi.hip"]
+           "fmm_treecode.hip", "fmm_adjoint.hip", "fmm_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

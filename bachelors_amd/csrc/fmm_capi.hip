@@ -24,6 +24,7 @@
 #include "fmm_prims.hpp"
 #include "fmm_tables.hpp"
 #include "fmm_tree.hpp"
+#include "fmm_treecode.hpp"
 
 using namespace fmm;
 
@@ -142,6 +143,8 @@ struct fmm_ctx {
     DevBuf m2l_slab;                             // M2L result per (pair, component)
     // evaluation
     DevBuf Bfar, Bnear, Bout;
+    DevBuf Afar, Anear, Aout;                    // vector potential (calcA)
+    DevBuf direct_part, pts, tc_B, tc_A;         // direct-sum partials; host query points and results
     double current = 1.0;
     // adjoint state
     AdjointState adj;
@@ -415,9 +418,39 @@ void stage_leaves(fmm_ctx* c) {
     c->stage = ST_LEAVES;
 }
 
+// M2M of the non-matrix algorithms (MultipoleSolver::calclMultipoleExpansions, multipole_solver.cpp:
+// 102-339): NoTranslation = P2M at every subdivided node from all its points (the slot ranges of
+// a subtree are contiguous); ComplexTranslation, RealTranslation and Layers = the per-node
+// translation formula, level by level, without operator tables
+void upward_without_matrices(fmm_ctx* c, int alg) {
+    const TreeDev& S = c->S;
+    hipStream_t st = c->stream;
+    std::vector<int> internal;
+    std::vector<int> level_begin(1, 0);
+    for (int k = 0; k + 1 < S.num_levels; k++) {
+        for (int n = S.level_begin[k]; n < S.level_begin[k + 1]; n++)
+            if (S.h_first_child[n] >= 0) internal.push_back(n);
+        level_begin.push_back((int)internal.size());
+    }
+    if (internal.empty()) return;
+    Staging& sg = c->up_stage;
+    sg.begin();
+    const size_t o = sg.put(internal);
+    const int* d = (const int*)(sg.send(st) + o);
+    if (alg == FMM_M2M_NO_TRANSLATION) {
+        launch_p2m(c->f64(), d, (int)internal.size(), S, c->sigS, c->sxyz.as<double>(), c->sq.as<double>(), c->p, c->Lp,
+                   c->M.ptr, st);
+        return;
+    }
+    for (int k = S.num_levels - 2; k >= 0; k--)   // parents of level k, deepest first
+        m2m_per_node(c->f64(), d + level_begin[k], level_begin[k + 1] - level_begin[k], S, c->sigS, c->p, c->Lp,
+                     c->M.ptr, st);
+}
+
 // M2M: per level, 8 orientation classes of (child -> parent) pairs (multipole_solver.cpp:341-423)
-void stage_upward(fmm_ctx* c) {
+void stage_upward(fmm_ctx* c, int alg = FMM_M2M_MATRICES) {
     require(c->stage >= ST_LEAVES, FMM_ERR_STATE, "leaf multipoles are not ready");
+    require(alg >= FMM_M2M_NO_TRANSLATION && alg <= FMM_M2M_MATRICES, FMM_ERR_ARG, "unknown M2M algorithm");
     const TreeDev& S = c->S;
     int nlev = S.num_levels;
     OpTables& O = c->up_ops;
@@ -436,11 +469,17 @@ void stage_upward(fmm_ctx* c) {
     // (one K segment per orientation operator; empty children hold zero multipoles), written
     // once without atomics; few parents -> the 8 orientations as parallel classes of
     // (child, parent) pairs with atomic accumulation (a long serial K loop would dominate)
-    if (O.nops) {
+    if (O.nops) {   // built for every algorithm: the adjoint applies them transposed
         Staging& sp = c->up_specs;
         sp.begin();
         const size_t o = sp.put(specs);
         build_tables(c, O, (const OpSpec*)(sp.send(c->stream) + o), c->stream);
+    }
+    if (alg != FMM_M2M_MATRICES) {
+        upward_without_matrices(c, alg);
+        c->rec(E_M2M);
+        c->stage = ST_UP;
+        return;
     }
     Staging& sg = c->up_stage;
     sg.begin();
@@ -739,7 +778,13 @@ void stage_p2p(fmm_ctx* c, hipStream_t ps) {
     int nts = c->T.num_slots;
     double* Bnear = c->Bnear.get<double>(3 * (size_t)std::max(nts, 1));
     c->rec_on(E_P2P0, ps);
-    if (c->cfg.near_field) {
+    if (c->cfg.near_field == FMM_NEAR_TREECODE) {
+        // the reference's close list: M2P walks from each close node, failing leaves dropped
+        FMM_HIP(hipMemsetAsync(Bnear, 0, 3 * (size_t)std::max(nts, 1) * sizeof(double), ps));
+        launch_m2p_close(c->f64(), c->S, c->sigS, c->M.ptr, c->p, c->Lp, c->tgt_leaves.as<int>(),
+                         (int)c->h_tgt_leaves.size(), c->T, c->lists.close_begin.as<int>(), c->lists.close_end.as<int>(),
+                         c->lists.close_s.as<int>(), c->txyz.as<double>(), Bnear, nullptr, ps);
+    } else if (c->cfg.near_field) {
         launch_p2p(c->f64(), c->tgt_leaves.as<int>(), (int)c->h_tgt_leaves.size(), c->T, c->S, c->lists.close_begin.as<int>(),
                    c->lists.close_end.as<int>(), c->lists.close_s.as<int>(), c->nP0.ptr, c->nP1.ptr, c->nP2.ptr,
                    c->txyz.as<double>(), Bnear, ps);
@@ -775,6 +820,51 @@ void stage_evaluate(fmm_ctx* c, double current, double* Bdev) {
     stage_l2p(c);
     stage_p2p(c, c->stream);
     stage_assemble(c, current, Bdev);
+}
+
+// calcA(current) (fast_multipole_solver.cpp:195-212): A = current / (4 pi) (sum L R(x - c) over
+// the target's leaf + the close list), input order; targets outside the target tree get NaN
+void stage_evaluate_A(fmm_ctx* c, double current, double* Adev) {
+    require(c->stage >= ST_DOWN, FMM_ERR_STATE, "local expansions are not ready");
+    hipStream_t st = c->stream;
+    const int nts = c->T.num_slots, nl = (int)c->h_tgt_leaves.size();
+    const size_t n3 = 3 * (size_t)std::max(nts, 1);
+    double* Af = c->Afar.get<double>(n3);
+    double* An = c->Anear.get<double>(n3);
+    launch_l2p_A(c->f64(), c->tgt_leaves.as<int>(), nl, c->T, c->sigT, c->txyz.as<double>(), c->Lx.ptr, c->p, c->Lp,
+                 Af, st);
+    FMM_HIP(hipMemsetAsync(An, 0, n3 * sizeof(double), st));
+    if (c->cfg.near_field == FMM_NEAR_TREECODE)
+        launch_m2p_close(c->f64(), c->S, c->sigS, c->M.ptr, c->p, c->Lp, c->tgt_leaves.as<int>(), nl, c->T,
+                         c->lists.close_begin.as<int>(), c->lists.close_end.as<int>(), c->lists.close_s.as<int>(),
+                         c->txyz.as<double>(), nullptr, An, st);
+    else if (c->cfg.near_field)
+        launch_p2p_A(c->f64(), c->tgt_leaves.as<int>(), nl, c->T, c->S, c->lists.close_begin.as<int>(),
+                     c->lists.close_end.as<int>(), c->lists.close_s.as<int>(), c->nP0.ptr, c->nP1.ptr, c->nP2.ptr,
+                     c->txyz.as<double>(), An, st);
+    const size_t nout = 3 * (size_t)c->nt;
+    if (nout) k_fill_nan<<<(unsigned)((nout + 255) / 256), 256, 0, st>>>(Adev, nout);
+    launch_assemble(c->T.slots.as<int>(), nts, Af, An, current / (4.0 * REF_PI), Adev, st);
+    FMM_HIP(hipGetLastError());
+}
+
+// the single-point treecode (multipole_solver.cpp:686-700) at m device points: B scaled by
+// current MU0 / (4 pi), A by current / (4 pi); either may be null
+void stage_treecode(fmm_ctx* c, double current, size_t m, const double* xyz, double* B, double* A) {
+    require(c->stage >= ST_UP, FMM_ERR_STATE, "Multipoles are not ready!");
+    require(m < (size_t)INT32_MAX / 3, FMM_ERR_LIMIT, "too many points");
+    require(m == 0 || xyz, FMM_ERR_ARG, "null point array");
+    launch_treecode_points(c->f64(), c->S, c->sigS, c->M.ptr, c->p, c->Lp, xyz, (int)m,
+                           current * REF_MU0 / (4.0 * REF_PI), current / (4.0 * REF_PI), B, A, c->stream);
+}
+
+// FP64 direct sum of B (pot: A) at m device targets into out, partial sums in part (nsplit *
+// m * 3 doubles)
+void direct_sum(fmm_ctx* c, bool pot, double current, size_t m, const double* xyz, double* out, double* part,
+                int nsplit) {
+    const double scale = pot ? current / (4.0 * REF_PI) : current * REF_MU0 / (4.0 * REF_PI);
+    launch_direct(pot, c->sxyz.as<double>(), c->sq.as<double>(), c->S.num_slots, xyz, (int)m, scale, out, part,
+                  nsplit, c->stream);
 }
 
 // The whole evaluation with the side stream: the lists are built (on a host thread, since
@@ -937,6 +1027,7 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
     if (cfg->source_leaf_capacity < 2 || cfg->target_leaf_capacity < 1) return FMM_ERR_ARG;
     if (cfg->precision < 0 || cfg->precision > 3 || cfg->frame < 0 || cfg->frame > 1) return FMM_ERR_ARG;
     if (!(cfg->mac_scale >= 1.0)) return FMM_ERR_ARG;
+    if (cfg->near_field < 0 || cfg->near_field > FMM_NEAR_TREECODE) return FMM_ERR_ARG;
     fmm_ctx* c = new fmm_ctx();
     c->cfg = *cfg;
     if (c->cfg.max_depth <= 0 || c->cfg.max_depth > MAX_LEVELS - 1) c->cfg.max_depth = MAX_LEVELS - 1;
@@ -1046,6 +1137,14 @@ int fmm_upward(fmm_ctx* c) {
         stage_upward(c);
     });
 }
+int fmm_upward_with(fmm_ctx* c, int algorithm) {
+    return guard(c, [&] {
+        require(algorithm >= FMM_M2M_NO_TRANSLATION && algorithm <= FMM_M2M_MATRICES, FMM_ERR_ARG,
+                "unknown M2M algorithm");
+        if (c->stage == ST_TREES) stage_leaves(c);
+        stage_upward(c, algorithm);
+    });
+}
 int fmm_downward(fmm_ctx* c) { return guard(c, [&] { stage_downward(c); }); }
 
 int fmm_evaluate_B_dev(fmm_ctx* c, double current, double* B) {
@@ -1083,6 +1182,8 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
         require(c->stage >= ST_DOWN, FMM_ERR_STATE, "run the forward pass first");
         require(dLdB && dLdw, FMM_ERR_ARG, "null adjoint arrays");
         require(c->cfg.frame == FMM_FRAME_SHARED, FMM_ERR_UNSUPPORTED, "the adjoint needs the shared frame");
+        require(c->cfg.near_field != FMM_NEAR_TREECODE, FMM_ERR_UNSUPPORTED,
+                "the adjoint has no transposed treecode near field");
         join_default_stream(c);
         AdjointArgs a;
         a.prec = c->pass_prec();
@@ -1164,12 +1265,60 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
     });
 }
 
-int fmm_direct_B_dev(fmm_ctx* c, double current, size_t m, const double* xyz, double* B) {
+static int direct_entry(fmm_ctx* c, bool pot, double current, size_t m, const double* xyz, double* out) {
     return guard(c, [&] {
         require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built (sources are gathered with them)");
+        require(m < (size_t)INT32_MAX / 3, FMM_ERR_LIMIT, "too many targets");
+        require(m == 0 || (xyz && out), FMM_ERR_ARG, "null target or output array");
+        if (!m) return;
         join_default_stream(c);
-        launch_direct(c->sxyz.as<double>(), c->sq.as<double>(), c->S.num_slots, xyz, (int)m,
-                      current * REF_MU0 / (4.0 * REF_PI), B, c->stream);
+        const int nsplit = direct_splits(c->S.num_slots, (int)m);
+        double* part = c->direct_part.get<double>((size_t)nsplit * 3 * m);
+        direct_sum(c, pot, current, m, xyz, out, part, nsplit);
+        FMM_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+int fmm_direct_B_dev(fmm_ctx* c, double current, size_t m, const double* xyz, double* B) {
+    return direct_entry(c, false, current, m, xyz, B);
+}
+int fmm_direct_A_dev(fmm_ctx* c, double current, size_t m, const double* xyz, double* A) {
+    return direct_entry(c, true, current, m, xyz, A);
+}
+int fmm_evaluate_A_dev(fmm_ctx* c, double current, double* A) {
+    return guard(c, [&] {
+        require(A != nullptr || c->nt == 0, FMM_ERR_ARG, "null output");
+        stage_evaluate_A(c, current, A);
+    });
+}
+int fmm_evaluate_A(fmm_ctx* c, double current, double* A) {
+    return guard(c, [&] {
+        require(A != nullptr || c->nt == 0, FMM_ERR_ARG, "null output");
+        double* d = c->Aout.get<double>(3 * (size_t)std::max(c->nt, 1));
+        stage_evaluate_A(c, current, d);
+        if (c->nt) FMM_HIP(hipMemcpyAsync(A, d, 3 * (size_t)c->nt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        FMM_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int fmm_treecode_dev(fmm_ctx* c, double current, size_t m, const double* xyz, double* B, double* A) {
+    return guard(c, [&] {
+        join_default_stream(c);
+        stage_treecode(c, current, m, xyz, B, A);
+        FMM_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+int fmm_treecode(fmm_ctx* c, double current, size_t m, const double* xyz, double* B, double* A) {
+    return guard(c, [&] {
+        require(m == 0 || xyz, FMM_ERR_ARG, "null point array");
+        require(m < (size_t)INT32_MAX / 3, FMM_ERR_LIMIT, "too many points");
+        const size_t n3 = 3 * std::max<size_t>(m, 1);
+        double* dx = c->pts.get<double>(n3);
+        double* dB = B ? c->tc_B.get<double>(n3) : nullptr;
+        double* dA = A ? c->tc_A.get<double>(n3) : nullptr;
+        if (m) FMM_HIP(hipMemcpyAsync(dx, xyz, 3 * m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        stage_treecode(c, current, m, dx, dB, dA);
+        if (m && B) FMM_HIP(hipMemcpyAsync(B, dB, 3 * m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (m && A) FMM_HIP(hipMemcpyAsync(A, dA, 3 * m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         FMM_HIP(hipStreamSynchronize(c->stream));
     });
 }

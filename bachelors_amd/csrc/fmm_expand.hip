@@ -180,7 +180,9 @@ __device__ __forceinline__ void deriv_coef(const double* Ls, int L, int c, int a
     br = sr; bi = si;
 }
 
-template <class T>
+// POT: the vector potential A = sigma^-1 Re sum alpha_c(l,m) Phi_l^m(u) over degree <= p (the
+// reference's calcA, calculation_point_octree.cpp:192-212) instead of B = curl A (degree <= p-1).
+template <class T, bool POT>
 __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, const double* __restrict__ center,
                                             const int* __restrict__ begin, const int* __restrict__ end,
                                             const int* __restrict__ level, ScaleTable sig,
@@ -188,13 +190,14 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
                                             double* __restrict__ Bfar) {
     extern __shared__ double sh[];
     const int L = (p + 1) * (p + 1);
-    const int NB = p * (p + 1) / 2;               // (l', m') pairs with l' <= p-1
+    const int D = POT ? p : p - 1;                // highest degree evaluated
+    const int NB = (D + 1) * (D + 2) / 2;         // (l', m') pairs with l' <= D
     // LDS: beta, then one region holding first the local expansion and, once beta is built,
     // the lanes' partial sums and the reciprocal table (fewer bytes per wave: more waves per CU)
     double* beta = sh;                             // [3][NB][2]
     double* Ls = sh + 6 * NB;                      // [3][L]
     double* part = Ls;                             // [64][2][3] partial sums of the lanes
-    double* rcp = Ls + 6 * 64;                     // 1 / n for n < p^2 (the recurrences' divisors)
+    double* rcp = Ls + 6 * 64;                     // 1 / n for n <= max(D^2, 2D) (the recurrences' divisors)
     const int leaf = leaves[blockIdx.x];
     const int lane = threadIdx.x;
     const T* Lsrc = Lexp + (size_t)leaf * 3 * Lp;
@@ -207,6 +210,10 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
         int lp = 0;
         while ((lp + 1) * (lp + 2) / 2 <= e) lp++;
         int mp = e - lp * (lp + 1) / 2;
+        if (POT) {
+            for (int c = 0; c < 3; c++) alpha_of(Ls, L, c, lp, mp, beta[(c * NB + e) * 2], beta[(c * NB + e) * 2 + 1]);
+            continue;
+        }
         double d[3][3][2];   // d[comp][axis]
         for (int c = 0; c < 3; c++)
             for (int a = 0; a < 3; a++) deriv_coef(Ls, L, c, a, lp, mp, d[c][a][0], d[c][a][1]);
@@ -218,14 +225,14 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
         }
     }
     __syncthreads();   // the local expansion is no longer read: its space takes the reciprocals
-    for (int e = lane; e < p * p; e += 64) rcp[e] = recip(e);
+    for (int e = lane; e <= max(D * D, 2 * D); e += 64) rcp[e] = recip(e);
     __syncthreads();
     const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
     const double inv = 1.0 / sig.s[level[leaf]];
-    const double s2 = inv * inv;
-    // lane (m, g): column m (< p) of the degree <= p-1 table for the targets 2 G it + g and
+    const double s2 = POT ? inv : inv * inv;
+    // lane (m, g): column m (<= D) of the degree <= D table for the targets 2 G it + g and
     // 2 G it + G + g (two independent recurrence chains sharing the beta loads)
-    const int G = 64 / p, m = lane % p, g = lane / p;
+    const int NM = D + 1, G = 64 / NM, m = lane % NM, g = lane / NM;
     const int kb = begin[leaf], n = end[leaf] - kb;
     for (int it = 0; 2 * it * G < n; it++) {
         const int t0 = 2 * it * G + g, t1 = t0 + G;
@@ -247,7 +254,7 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
             }
             double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
             double o1r = er, o1i = ei, o2r = 0.0, o2i = 0.0;
-            for (int l = m; l < p; l++) {
+            for (int l = m; l <= D; l++) {
                 double vr = dr, vi = di, yr = er, yi = ei;
                 if (l > m) {
                     const double f = rcp[l * l - m * m];
@@ -282,7 +289,7 @@ __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, cons
             for (int u = h; u < 2; u++) {
                 if (u == 1 && !v1) break;
                 double s0 = 0, s1 = 0, sz = 0;
-                for (int mm = 0; mm < p; mm++) {
+                for (int mm = 0; mm < NM; mm++) {
                     s0 += part[6 * (lane + mm) + 3 * u];
                     s1 += part[6 * (lane + mm) + 3 * u + 1];
                     sz += part[6 * (lane + mm) + 3 * u + 2];
@@ -345,7 +352,8 @@ __global__ void k_pack_sources(int n, const double* __restrict__ origin, const d
 // sweep it, G = floor(64 / n) lanes per target for a leaf of 8 <= n <= 64 targets (64 / 2^ceil(log2 n)
 // below 8; lane = target * G + g takes sources g, g + G, ...).  Lane partials are summed over
 // the G lanes, then over the 4 waves in a fixed order, and written once.
-template <class R>
+// POT: the vector potential sum_j q_j / |x - y_j| instead of B.
+template <class R, bool POT>
 __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, const int* __restrict__ tbegin, const int* __restrict__ tend,
                                              const int* __restrict__ close_begin, const int* __restrict__ close_end,
                                              const int* __restrict__ close_s, const double* __restrict__ origin,
@@ -417,10 +425,16 @@ __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, con
                     // fused multiply-adds (explicit: this library is built without contraction)
                     const R r2 = fma_t(dx, dx, fma_t(dy, dy, dz * dz));
                     const R ri = r2 > (R)0 ? rsqrt_t(r2) : (R)0;
-                    const R ri3 = ri * ri * ri;
-                    f0 = fma_t(fma_t(qy, dz, -qz * dy), ri3, f0);
-                    f1 = fma_t(fma_t(qz, dx, -qx * dz), ri3, f1);
-                    f2 = fma_t(fma_t(qx, dy, -qy * dx), ri3, f2);
+                    if (POT) {
+                        f0 = fma_t(qx, ri, f0);
+                        f1 = fma_t(qy, ri, f1);
+                        f2 = fma_t(qz, ri, f2);
+                    } else {
+                        const R ri3 = ri * ri * ri;
+                        f0 = fma_t(fma_t(qy, dz, -qz * dy), ri3, f0);
+                        f1 = fma_t(fma_t(qz, dx, -qx * dz), ri3, f1);
+                        f2 = fma_t(fma_t(qx, dy, -qy * dx), ri3, f2);
+                    }
                 }
                 a0 += f0;
                 a1 += f1;
@@ -465,43 +479,87 @@ __global__ void k_assemble(const int* __restrict__ slots, int n, const double* _
 }
 
 // ---------------------------------------------------------------- direct sum (FP64)
-__global__ __launch_bounds__(256) void k_direct(const double* __restrict__ sxyz, const double* __restrict__ sq, int ns,
-                                                const double* __restrict__ txyz, int nt, double scale,
-                                                double* __restrict__ B) {
+// B(x) = scale sum_j q_j x (x - y_j) / |x - y_j|^3 (POT: A(x) = scale sum_j q_j / |x - y_j|),
+// math::calcBioSavartLaplace (integration.cpp:101-120).  The sources are cut into nsplit
+// contiguous ranges: workgroup (b, s) sums range s for the 256 targets of block b (sources staged
+// in LDS) into partial[s][t][3]; k_direct_reduce adds the nsplit partials of each target in range
+// order (deterministic) and scales.  Splitting the sources keeps every CU busy when the targets
+// are few (16,384 targets are 64 workgroups).
+template <bool POT>
+__global__ __launch_bounds__(256) void k_direct_partial(const double* __restrict__ sxyz, const double* __restrict__ sq,
+                                                        int ns, const double* __restrict__ txyz, int nt,
+                                                        double* __restrict__ partial) {
     __shared__ double3 sp[256];
     __shared__ double3 sqq[256];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, split = blockIdx.y, nsplit = gridDim.y;
+    const long long per = ((long long)ns + nsplit - 1) / nsplit;
+    const int j_begin = (int)min((long long)ns, per * split), j_end = (int)min((long long)ns, per * (split + 1));
     double x = 0, y = 0, z = 0;
     if (t < nt) { x = txyz[3 * t]; y = txyz[3 * t + 1]; z = txyz[3 * t + 2]; }
     double a0 = 0, a1 = 0, a2 = 0;
-    for (int j0 = 0; j0 < ns; j0 += 256) {
+    for (int j0 = j_begin; j0 < j_end; j0 += 256) {
         int j = j0 + threadIdx.x;
         __syncthreads();
-        if (j < ns) {
+        if (j < j_end) {
             sp[threadIdx.x] = make_double3(sxyz[3 * j], sxyz[3 * j + 1], sxyz[3 * j + 2]);
             sqq[threadIdx.x] = make_double3(sq[3 * j], sq[3 * j + 1], sq[3 * j + 2]);
         }
         __syncthreads();
-        int nj = min(256, ns - j0);
+        int nj = min(256, j_end - j0);
         for (int jj = 0; jj < nj; jj++) {
             double dx = x - sp[jj].x, dy = y - sp[jj].y, dz = z - sp[jj].z;
             double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 == 0.0) continue;
             double r = sqrt(r2);
-            double r3 = 1.0 / (r2 * r);
             double3 q = sqq[jj];
-            a0 += (q.y * dz - q.z * dy) * r3;
-            a1 += (q.z * dx - q.x * dz) * r3;
-            a2 += (q.x * dy - q.y * dx) * r3;
+            if (POT) {
+                double ri = 1.0 / r;
+                a0 += q.x * ri;
+                a1 += q.y * ri;
+                a2 += q.z * ri;
+            } else {
+                double r3 = 1.0 / (r2 * r);
+                a0 += (q.y * dz - q.z * dy) * r3;
+                a1 += (q.z * dx - q.x * dz) * r3;
+                a2 += (q.x * dy - q.y * dx) * r3;
+            }
         }
     }
     if (t < nt) {
-        B[3 * t] = a0 * scale;
-        B[3 * t + 1] = a1 * scale;
-        B[3 * t + 2] = a2 * scale;
+        double* o = partial + ((size_t)split * nt + t) * 3;
+        o[0] = a0;
+        o[1] = a1;
+        o[2] = a2;
     }
 }
 
+__global__ void k_direct_reduce(const double* __restrict__ partial, int nt, int nsplit, double scale,
+                                double* __restrict__ B) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // target * 3 + component
+    if (i >= 3 * nt) return;
+    double acc = 0;
+    for (int s = 0; s < nsplit; s++) acc += partial[(size_t)s * 3 * nt + i];
+    B[i] = acc * scale;
+}
+
+int direct_splits(int ns, int nt) {
+    // about 4 workgroups per CU in total, each range at least 4,096 sources
+    const int tb = std::max(1, blocks(nt));
+    int s = std::max(1, 4 * num_cus() / tb);
+    s = std::min(s, std::max(1, ns / 4096));
+    return s;
+}
+
+void launch_direct(bool pot, const double* sxyz, const double* sq, int ns, const double* txyz, int nt, double scale,
+                   double* out, double* partial, int nsplit, hipStream_t st) {
+    if (!nt) return;
+    if (nsplit < 1) throw Error(1, "direct sum: at least one source range");
+    dim3 grid(blocks(nt), nsplit);
+    if (pot) k_direct_partial<true><<<grid, 256, 0, st>>>(sxyz, sq, ns, txyz, nt, partial);
+    else k_direct_partial<false><<<grid, 256, 0, st>>>(sxyz, sq, ns, txyz, nt, partial);
+    k_direct_reduce<<<blocks(3 * (long long)nt), 256, 0, st>>>(partial, nt, nsplit, scale, out);
+    FMM_HIP(hipGetLastError());
+}
 
 void launch_charges(const int* slots, int n, const double* xyz, const double* w, const double* J, double* sxyz,
                     double* sq, double* sJ, hipStream_t st) {
@@ -540,19 +598,30 @@ void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, cons
     else p2m_dispatch<float>(p, leaves, nleaves, S, sig, sxyz, sq, Lp, (float*)M, st);
     FMM_HIP(hipGetLastError());
 }
+template <bool POT>
+static void l2p_launch(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig,
+                       const double* txyz, const void* Lexp, int p, int Lp, double* out, hipStream_t st) {
+    const int L = (p + 1) * (p + 1), D = POT ? p : p - 1, NB = (D + 1) * (D + 2) / 2;
+    // degree <= 63 columns: 64 lanes hold at least one target group
+    if (p < 1 || D + 1 > 64) throw Error(1, "L2P: order out of range");
+    size_t lds = (size_t)(6 * NB + std::max(3 * L, 6 * 64 + std::max(D * D, 2 * D) + 1)) * sizeof(double);
+    if (f64)
+        k_l2p<double, POT><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
+                                                     T.level.as<int>(), sig, txyz, (const double*)Lexp, p, Lp, out);
+    else
+        k_l2p<float, POT><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
+                                                    T.level.as<int>(), sig, txyz, (const float*)Lexp, p, Lp, out);
+    FMM_HIP(hipGetLastError());
+}
 void launch_l2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig, const double* txyz,
                 const void* Lexp, int p, int Lp, double* Bfar, hipStream_t st) {
     if (!nleaves) return;
-    int L = (p + 1) * (p + 1), NB = p * (p + 1) / 2;
-    size_t lds = (size_t)(6 * NB + std::max(3 * L, 6 * 64 + p * p)) * sizeof(double);
-    if (p < 1) throw Error(1, "order must be at least 1");
-    if (f64)
-        k_l2p<double><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
-                                                T.level.as<int>(), sig, txyz, (const double*)Lexp, p, Lp, Bfar);
-    else
-        k_l2p<float><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
-                                               T.level.as<int>(), sig, txyz, (const float*)Lexp, p, Lp, Bfar);
-    FMM_HIP(hipGetLastError());
+    l2p_launch<false>(f64, leaves, nleaves, T, sig, txyz, Lexp, p, Lp, Bfar, st);
+}
+void launch_l2p_A(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig, const double* txyz,
+                  const void* Lexp, int p, int Lp, double* Afar, hipStream_t st) {
+    if (!nleaves) return;
+    l2p_launch<true>(f64, leaves, nleaves, T, sig, txyz, Lexp, p, Lp, Afar, st);
 }
 void launch_pack_sources(bool f64, int n, const double* origin, const double* sxyz, const double* sq, void* P0,
                          void* P1, void* P2, hipStream_t st) {
@@ -563,30 +632,36 @@ void launch_pack_sources(bool f64, int n, const double* origin, const double* sx
         k_pack_sources<float><<<blocks(n), 256, 0, st>>>(n, origin, sxyz, sq, (float4*)P0, (float4*)P1, (float*)P2);
     FMM_HIP(hipGetLastError());
 }
+template <bool POT>
+static void p2p_launch(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S,
+                       const int* close_begin, const int* close_end, const int* close_s, const void* P0,
+                       const void* P1, const void* P2, const double* txyz, double* out, hipStream_t st) {
+    if (!nleaves) return;
+    if (f64)
+        k_p2p<double, POT><<<nleaves, 256, 0, st>>>(leaves, T.begin.as<int>(), T.end.as<int>(),
+                                                    close_begin, close_end, close_s, S.center.as<double>(),
+                                                    S.begin.as<int>(), S.end.as<int>(), (const double4*)P0,
+                                                    (const double4*)P1, (const double*)P2, txyz, out);
+    else
+        k_p2p<float, POT><<<nleaves, 256, 0, st>>>(leaves, T.begin.as<int>(), T.end.as<int>(),
+                                                   close_begin, close_end, close_s, S.center.as<double>(),
+                                                   S.begin.as<int>(), S.end.as<int>(), (const float4*)P0,
+                                                   (const float4*)P1, (const float*)P2, txyz, out);
+    FMM_HIP(hipGetLastError());
+}
 void launch_p2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
                 const int* close_end, const int* close_s, const void* P0, const void* P1, const void* P2,
                 const double* txyz, double* Bnear, hipStream_t st) {
-    if (!nleaves) return;
-    if (f64)
-        k_p2p<double><<<nleaves, 256, 0, st>>>(leaves, T.begin.as<int>(), T.end.as<int>(),
-                                               close_begin, close_end, close_s, S.center.as<double>(),
-                                               S.begin.as<int>(), S.end.as<int>(), (const double4*)P0,
-                                               (const double4*)P1, (const double*)P2, txyz, Bnear);
-    else
-        k_p2p<float><<<nleaves, 256, 0, st>>>(leaves, T.begin.as<int>(), T.end.as<int>(),
-                                              close_begin, close_end, close_s, S.center.as<double>(),
-                                              S.begin.as<int>(), S.end.as<int>(), (const float4*)P0,
-                                              (const float4*)P1, (const float*)P2, txyz, Bnear);
-    FMM_HIP(hipGetLastError());
+    p2p_launch<false>(f64, leaves, nleaves, T, S, close_begin, close_end, close_s, P0, P1, P2, txyz, Bnear, st);
+}
+void launch_p2p_A(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
+                  const int* close_end, const int* close_s, const void* P0, const void* P1, const void* P2,
+                  const double* txyz, double* Anear, hipStream_t st) {
+    p2p_launch<true>(f64, leaves, nleaves, T, S, close_begin, close_end, close_s, P0, P1, P2, txyz, Anear, st);
 }
 void launch_assemble(const int* slots, int n, const double* Bfar, const double* Bnear, double scale, double* B,
                      hipStream_t st) {
     if (n) k_assemble<<<blocks(n), 256, 0, st>>>(slots, n, Bfar, Bnear, scale, B);
-}
-void launch_direct(const double* sxyz, const double* sq, int ns, const double* txyz, int nt, double scale, double* B,
-                   hipStream_t st) {
-    if (nt) k_direct<<<blocks(nt), 256, 0, st>>>(sxyz, sq, ns, txyz, nt, scale, B);
-    FMM_HIP(hipGetLastError());
 }
 
 }  // namespace fmm

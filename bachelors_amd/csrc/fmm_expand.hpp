@@ -19,7 +19,17 @@ void launch_p2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, cons
                 const double* txyz, double* Bnear, hipStream_t st);
 void launch_assemble(const int* slots, int n, const double* Bfar, const double* Bnear, double scale, double* B,
                      hipStream_t st);
-void launch_direct(const double* sxyz, const double* sq, int ns, const double* txyz, int nt, double scale, double* B,
-                   hipStream_t st);
+// L2P / P2P of the vector potential A (the calcA counterparts of launch_l2p / launch_p2p)
+void launch_l2p_A(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig, const double* txyz,
+                  const void* Lexp, int p, int Lp, double* Afar, hipStream_t st);
+void launch_p2p_A(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
+                  const int* close_end, const int* close_s, const void* P0, const void* P1, const void* P2,
+                  const double* txyz, double* Anear, hipStream_t st);
+// FP64 direct sum of B (pot: A) at nt targets over the ns slot-order sources, scaled; the sources
+// are cut into nsplit ranges whose partial sums (partial: nsplit * nt * 3 doubles) are added in
+// range order.  direct_splits: the number of ranges that fills the GPU.
+int direct_splits(int ns, int nt);
+void launch_direct(bool pot, const double* sxyz, const double* sq, int ns, const double* txyz, int nt, double scale,
+                   double* out, double* partial, int nsplit, hipStream_t st);
 
 }  // namespace fmm

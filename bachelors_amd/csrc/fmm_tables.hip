@@ -370,6 +370,108 @@ void make_m2l_f16(const OpSpec* d_specs, int nops, int p, const F16Split& f16, v
     FMM_HIP(hipGetLastError());
 }
 
+// Per-node M2M: the reference's ComplexTranslation / RealTranslation / Layers algorithms
+// (multipole_solver.cpp:141-339; the translator multipole_translator.cpp:3-32), which translate
+// each child's expansion with the formula out(l, m) += sum R(l - a, m - b) f(m, m - b) in(a, b)
+// evaluated from the child offset's harmonics, without operator tables.  One workgroup per
+// parent: per non-empty child, the complex harmonics X of u = (c_child - c_parent) / sigma_child
+// and the child's expansion in the complex basis go to LDS, then every thread owns output
+// coefficients i = tid + 256 r, sums the complex formula for (l, +-m), converts to the real basis
+// and scales by (sigma_child / sigma_parent)^l; the sums over the children stay in registers and
+// are added to the parent once.
+template <class T>
+__global__ __launch_bounds__(256) void k_m2m_nodes(const int* __restrict__ parents, const int* __restrict__ first_child,
+                                                   const int* __restrict__ npts, const double* __restrict__ center,
+                                                   const int* __restrict__ level, ScaleTable sig, int p, int Lp,
+                                                   T* __restrict__ M) {
+    extern __shared__ cplx sh[];
+    constexpr int RMAX = (MAX_ORDER + 1) * (MAX_ORDER + 1) / 256 + 1;
+    const int L = (p + 1) * (p + 1), NX = (p + 1) * (p + 2) / 2;
+    cplx* X = sh;            // packed (n, k >= 0), n <= p
+    cplx* C = sh + NX;       // [3][L] complex input, index l^2 + l + m
+    const int parent = parents[blockIdx.x], fc = first_child[parent];
+    const double c = 0.70710678118654752440;
+    double acc[RMAX][3];
+    for (int r = 0; r < RMAX; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
+    const double sp = sig.s[level[parent]];
+    for (int o = 0; o < 8; o++) {
+        const int ch = fc + o;
+        if (first_child[ch] < 0 && npts[ch] == 0) continue;   // uniform over the workgroup
+        const double sc = sig.s[level[ch]];
+        __syncthreads();   // the previous child's tables are no longer read
+        const double ux = (center[3 * ch] - center[3 * parent]) / sc, uy = (center[3 * ch + 1] - center[3 * parent + 1]) / sc,
+                     uz = (center[3 * ch + 2] - center[3 * parent + 2]) / sc;
+        const double r2 = ux * ux + uy * uy + uz * uz;
+        for (int k = threadIdx.x; k <= p; k += blockDim.x) {   // Phi_k^k = (x+iy)^k / (2^k k!), column k
+            cplx d = {1, 0};
+            for (int i = 1; i <= k; i++) d = cscale(cmul(d, {ux, uy}), recip(2 * i));
+            cplx pm2 = {0, 0}, pm1 = d;
+            X[tri(k, k)] = cscale(d, sgn_pow(k));
+            for (int n = k + 1; n <= p; n++) {
+                cplx v = cscale(cadd(cscale(pm1, (2.0 * n - 1) * uz), cscale(pm2, -r2)), recip(n * n - k * k));
+                X[tri(n, k)] = cscale(v, sgn_pow(n));
+                pm2 = pm1;
+                pm1 = v;
+            }
+        }
+        const T* Mc = M + (size_t)ch * 3 * Lp;
+        for (int e = threadIdx.x; e < 3 * L; e += blockDim.x) {   // real -> complex (harmonics.cpp:281-302)
+            const int comp = e / L, i = e - comp * L;
+            int a, b;
+            lm_of(i, a, b);
+            const double re = (double)Mc[comp * Lp + a * a + a + abs(b)];
+            const double im = b == 0 ? 0.0 : (double)Mc[comp * Lp + a * a + a - abs(b)];
+            C[e] = b == 0 ? cplx{re, 0.0} : b > 0 ? cplx{c * re, c * im} : cplx{c * re, -c * im};
+        }
+        __syncthreads();
+        const double ratio = sc / sp;
+        for (int r = 0; r < RMAX; r++) {
+            const int i = threadIdx.x + 256 * r;
+            if (i >= L) break;
+            int l, m;
+            lm_of(i, l, m);
+            const int am = abs(m);
+            const double f = pow(ratio, l);
+            for (int comp = 0; comp < 3; comp++) {
+                const cplx* Cc = C + comp * L;
+                cplx op = {0, 0}, om = {0, 0};   // complex outputs at (l, |m|) and (l, -|m|)
+                for (int a = 0; a <= l; a++)
+                    for (int b = -a; b <= a; b++) {
+                        const cplx in = Cc[a * a + a + b];
+                        op = cadd(op, cmul(kernel_entry(OPK_M2M, X, p, l, am, a, b), in));
+                        if (am) om = cadd(om, cmul(kernel_entry(OPK_M2M, X, p, l, -am, a, b), in));
+                    }
+                // complex -> real (harmonics.cpp:304-320)
+                const double v = m == 0 ? op.re : m > 0 ? c * (op.re + om.re) : c * (op.im - om.im);
+                acc[r][comp] += v * f;
+            }
+        }
+    }
+    T* Mp = M + (size_t)parent * 3 * Lp;
+    for (int r = 0; r < RMAX; r++) {
+        const int i = threadIdx.x + 256 * r;
+        if (i >= L) break;
+        for (int comp = 0; comp < 3; comp++) Mp[comp * Lp + i] += (T)acc[r][comp];
+    }
+}
+
+void m2m_per_node(bool f64, const int* parents, int nparents, const TreeDev& S, const ScaleTable& sig, int p, int Lp,
+                  void* M, hipStream_t st) {
+    if (nparents <= 0) return;
+    if (p > MAX_ORDER) throw Error(4, "expansion order above 20");
+    const int L = (p + 1) * (p + 1), NX = (p + 1) * (p + 2) / 2;
+    const size_t lds = (size_t)(NX + 3 * L) * sizeof(cplx);
+    if (f64)
+        k_m2m_nodes<double><<<nparents, 256, lds, st>>>(parents, S.first_child.as<int>(), S.npts.as<int>(),
+                                                        S.center.as<double>(), S.level.as<int>(), sig, p, Lp,
+                                                        (double*)M);
+    else
+        k_m2m_nodes<float><<<nparents, 256, lds, st>>>(parents, S.first_child.as<int>(), S.npts.as<int>(),
+                                                       S.center.as<double>(), S.level.as<int>(), sig, p, Lp,
+                                                       (float*)M);
+    FMM_HIP(hipGetLastError());
+}
+
 void make_operators(const OpSpec* d_specs, int nops, int p, int Lp, int out_mode, void* d_out, __bf16* d_out_lo,
                     hipStream_t st) {
     if (nops == 0) return;

@@ -39,6 +39,11 @@ void make_m2l_f16(const OpSpec* d_specs, int nops, int p, const F16Split& f16, v
 void make_m2l_specs(const int* class_begin, int nc, const int* far_t, const int* far_s, const double* sc,
                     const double* tc, const int* slev, const int* tlev, const ScaleTable& sigS, const ScaleTable& sigT,
                     int m2l_full, OpSpec* out, hipStream_t st);
+// M2M of one tree level without operator tables: every parent in parents[0 .. nparents) adds its
+// non-empty children's expansions translated by the complex formula (M: [node][3][Lp] scaled
+// expansions, double if f64 else float)
+void m2m_per_node(bool f64, const int* parents, int nparents, const TreeDev& S, const ScaleTable& sig, int p, int Lp,
+                  void* M, hipStream_t st);
 void make_pair_specs(const int* src, const int* dst, int n, const double* csrc, const double* cdst, const int* lsrc,
                      const int* ldst, const ScaleTable& sig_src, const ScaleTable& sig_dst, int kind, int m2l_full,
                      OpSpec* out, hipStream_t st);

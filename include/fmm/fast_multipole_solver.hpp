@@ -41,11 +41,14 @@ public:
     const size_t quadratureOctreeLeafCapacity;
     const size_t calculationPointOctreeLeafCapacity;
 
-    // fast_multipole_solver.hpp:25-30; order / precision are additive runtime options
+    // fast_multipole_solver.hpp:25-30; order, precision, tree frame and near-field rule are additive
+    // runtime options whose defaults keep the reference's trees and output order (the reference
+    // frame) with the corrected numerics (direct near field, analytic curl)
     FastMultipoleSolver(std::vector<Quadrature>& quadratures, std::vector<Vector3>& points,
                         Problem problem = Problem::BioSavartLaplace, size_t quadratureOctreeLeafCapacity_ = 1000,
                         size_t calculationPointOctreeLeafCapacity_ = 100, int order = 10,
-                        int precision = FMM_PREC_FP64)
+                        int precision = FMM_PREC_FP64, int frame = FMM_FRAME_REFERENCE,
+                        int near_field = FMM_NEAR_P2P)
         : harmonicOrder(order), harmonicLength((order + 1) * (order + 1)),
           quadratureOctreeLeafCapacity(quadratureOctreeLeafCapacity_),
           calculationPointOctreeLeafCapacity(calculationPointOctreeLeafCapacity_), _points(points) {
@@ -56,6 +59,8 @@ public:
         cfg.source_leaf_capacity = (int)quadratureOctreeLeafCapacity_;
         cfg.target_leaf_capacity = (int)calculationPointOctreeLeafCapacity_;
         cfg.precision = precision;
+        cfg.frame = frame;
+        cfg.near_field = near_field;
         check(fmm_create(&_ctx, &cfg), "fmm_create");
         std::vector<double> xyz(3 * quadratures.size()), w(quadratures.size());
         for (size_t i = 0; i < quadratures.size(); i++) {
@@ -82,10 +87,18 @@ public:
         check(fmm_leaf_expansions(_ctx), "calcMultipoleExpansionsAtLeaves");
         _leaves = true;
     }
-    // every M2M algorithm and device of the reference yields the same expansions; all run on MFMA here
-    void calclMultipoleExpansions(M2MAlg = M2MAlg::Matrices, Device = Device::CPU) {   // :102-139
+    // multipole_solver.cpp:97-100: every node of the source tree, empty children included
+    size_t getQuadratureOctreeNodeCount() {
+        int nodes = 0, slots = 0;
+        check(fmm_tree_size(_ctx, 0, &nodes, &slots), "getQuadratureOctreeNodeCount");
+        return (size_t)nodes;
+    }
+    // multipole_solver.cpp:102-139: every algorithm runs on the GPU (Matrices: MFMA GEMMs;
+    // Complex/Real/Layers: the per-node translation formula; NoTranslation: P2M at every node)
+    // and gives the same expansions; the device argument is kept for source compatibility
+    void calclMultipoleExpansions(M2MAlg alg = M2MAlg::Matrices, Device = Device::CPU) {
         if (!_leaves) calcMultipoleExpansionsAtLeaves();
-        check(fmm_upward(_ctx), "calclMultipoleExpansions");
+        check(fmm_upward_with(_ctx, (int)alg), "calclMultipoleExpansions");
     }
     void calcLocalMultipoleExpansions(M2LAlg = M2LAlg::ComplexTranslation, Device = Device::CPU) {   // :62-102
         check(fmm_downward(_ctx), "calcLocalMultipoleExpansions");
@@ -94,18 +107,38 @@ public:
     std::vector<std::pair<Vector3, Vector3>> calcB(double current) {
         std::vector<double> B(3 * _points.size());
         check(fmm_evaluate_B(_ctx, current, B.data()), "calcB");
-        fmm_stats st;
-        check(fmm_get_stats(_ctx, &st), "fmm_get_stats");
-        std::vector<int64_t> order(st.target_slots);
-        check(fmm_leaf_order(_ctx, order.data()), "fmm_leaf_order");
-        std::vector<std::pair<Vector3, Vector3>> res;
-        res.reserve(order.size());
-        for (int64_t i : order) res.emplace_back(_points[i], Vector3(B[3 * i], B[3 * i + 1], B[3 * i + 2]));
-        return res;
+        return leaf_ordered(B);
+    }
+    // fast_multipole_solver.cpp:195-212: A = current / (4 pi) sum w J / |x - y|, target-leaf order
+    std::vector<std::pair<Vector3, Vector3>> calcA(double current) {
+        std::vector<double> A(3 * _points.size());
+        check(fmm_evaluate_A(_ctx, current, A.data()), "calcA");
+        return leaf_ordered(A);
+    }
+    // multipole_solver.cpp:686-700: the treecode at one point (after calclMultipoleExpansions)
+    Vector3 calcB(double current, const Vector3& point) {
+        double x[3] = {point.x, point.y, point.z}, B[3];
+        check(fmm_treecode(_ctx, current, 1, x, B, nullptr), "calcB(current, point)");
+        return Vector3(B[0], B[1], B[2]);
+    }
+    Vector3 calcA(double current, const Vector3& point) {
+        double x[3] = {point.x, point.y, point.z}, A[3];
+        check(fmm_treecode(_ctx, current, 1, x, nullptr, A), "calcA(current, point)");
+        return Vector3(A[0], A[1], A[2]);
     }
     fmm_ctx* context() { return _ctx; }
 
 private:
+    std::vector<std::pair<Vector3, Vector3>> leaf_ordered(const std::vector<double>& v) {
+        int nodes = 0, slots = 0;
+        check(fmm_tree_size(_ctx, 1, &nodes, &slots), "fmm_tree_size");
+        std::vector<int64_t> order(slots);
+        check(fmm_leaf_order(_ctx, order.data()), "fmm_leaf_order");
+        std::vector<std::pair<Vector3, Vector3>> res;
+        res.reserve(order.size());
+        for (int64_t i : order) res.emplace_back(_points[i], Vector3(v[3 * i], v[3 * i + 1], v[3 * i + 2]));
+        return res;
+    }
     std::vector<Vector3>& _points;   // non-owning, as the reference (fast_multipole_solver.hpp:16)
     fmm_ctx* _ctx = nullptr;
     bool _leaves = false;

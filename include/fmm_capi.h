@@ -8,12 +8,18 @@
  *     fast_multipole_solver.hpp:25-30, multipole_solver.cpp:20-36    + fmm_set_targets + fmm_build_trees
  *   MultipoleSolver::calcMultipoleExpansionsAtLeaves()             fmm_leaf_expansions
  *     multipole_solver.hpp:39, multipole_solver.cpp:56-95
- *   MultipoleSolver::calclMultipoleExpansions(M2MAlg, Device)      fmm_upward
+ *   MultipoleSolver::calclMultipoleExpansions(M2MAlg, Device)      fmm_upward_with (fmm_upward: Matrices)
  *     multipole_solver.hpp:42-44, multipole_solver.cpp:102-139
  *   FastMultipoleSolver::calcLocalMultipoleExpansions(M2LAlg, Dev) fmm_downward
  *     fast_multipole_solver.hpp:39-41, fast_multipole_solver.cpp:62-102
  *   FastMultipoleSolver::calcB(real current)                       fmm_evaluate_B (+ fmm_leaf_order)
  *     fast_multipole_solver.hpp:46, fast_multipole_solver.cpp:214-231
+ *   FastMultipoleSolver::calcA(real current)                       fmm_evaluate_A (+ fmm_leaf_order)
+ *     fast_multipole_solver.hpp:45, fast_multipole_solver.cpp:195-212
+ *   MultipoleSolver::calcA / calcB(real current, const Vector3&)   fmm_treecode (many points at once)
+ *     multipole_solver.hpp:46-47, multipole_solver.cpp:686-700
+ *   MultipoleSolver::getQuadratureOctreeNodeCount()                fmm_tree_size(ctx, 0, ...)
+ *     multipole_solver.hpp:40, multipole_solver.cpp:97-100
  *   math::calcBioSavartLaplace(current, point, quadratures)         fmm_direct_B_dev
  *     integration.cpp:101-111
  * and it replaces the reference's device boundary (kernels::translateAll*,
@@ -23,9 +29,10 @@
  *
  * Conventions: every function returns FMM_OK (0) or an error code and never exits; the
  * message is in fmm_last_error(ctx).  The context owns all device memory.  Host arrays
- * are copied in; *_dev entry points take device pointers (the device is synchronised
- * before caller memory is read, then it is copied device-to-device) and run on the
- * context's HIP stream.  One context per host thread.
+ * are copied in; *_dev entry points take device pointers and run on the context's HIP
+ * stream, ordered after the caller's work on the default (null) stream by an event (no
+ * device-wide synchronisation): work on other caller streams must be finished by the caller,
+ * or the caller's stream handed over with fmm_set_stream.  One context per host thread.
  */
 #ifndef FMM_CAPI_H
 #define FMM_CAPI_H
@@ -53,6 +60,11 @@ extern "C" {
 #define FMM_FRAME_REFERENCE 0  /* target root = reference bounding box (math.cpp:39-57) */
 #define FMM_FRAME_SHARED 1     /* target root = source root: M2L operators shared per offset */
 
+#define FMM_NEAR_NONE 0        /* close lists add nothing */
+#define FMM_NEAR_P2P 1         /* close lists summed particle by particle (exact near field) */
+#define FMM_NEAR_TREECODE 2    /* the reference: M2P walks from each close node, failing leaves
+                                  add nothing (quadrature_octree.cpp:171-258) */
+
 typedef struct fmm_ctx fmm_ctx;
 
 typedef struct fmm_config {
@@ -63,7 +75,7 @@ typedef struct fmm_config {
     int frame;                    /* FMM_FRAME_* */
     int m2l_full;                 /* 1: M2L keeps l+lambda <= 2p; 0: reference l+lambda <= p */
     int max_depth;                /* depth cap (<= 21); the reference has none */
-    int near_field;               /* 1: direct P2P over close lists; 0: none (reference drops close leaves) */
+    int near_field;               /* FMM_NEAR_*: P2P (default), none, or the reference's treecode M2P */
     double mac_scale;             /* 1.0 = reference MAC (2Rmax+Rmin)^2 < d^2 (fast_multipole_solver.cpp:167-184) */
     double source_root_center[3]; /* reference Box((0,0,0),(3,3,3)) (multipole_solver.cpp:662-663) */
     double source_root_half[3];
@@ -102,10 +114,21 @@ int fmm_set_sources_dev(fmm_ctx* ctx, size_t n, const double* xyz, const double*
 int fmm_set_weights_dev(fmm_ctx* ctx, const double* w);   /* new weights, same sources and trees */
 int fmm_set_targets_dev(fmm_ctx* ctx, size_t m, const double* xyz);
 
+/* M2M algorithms of MultipoleSolver::calclMultipoleExpansions (multipole_solver_enums.hpp:3-10,
+ * multipole_solver.cpp:102-339); all give the same multipoles.  NO_TRANSLATION: P2M at every node
+ * from its points; COMPLEX / REAL / LAYERS: the per-node translation formula without operator
+ * tables; MATRICES: the per-(level, orientation) operators applied as MFMA GEMMs (fmm_upward). */
+#define FMM_M2M_NO_TRANSLATION 0
+#define FMM_M2M_COMPLEX_TRANSLATION 1
+#define FMM_M2M_REAL_TRANSLATION 2
+#define FMM_M2M_LAYERS 3
+#define FMM_M2M_MATRICES 4
+
 /* stages, in reference order */
 int fmm_build_trees(fmm_ctx* ctx);
 int fmm_leaf_expansions(fmm_ctx* ctx);
 int fmm_upward(fmm_ctx* ctx);
+int fmm_upward_with(fmm_ctx* ctx, int m2m_algorithm);   /* calclMultipoleExpansions(M2MAlg, Device) */
 int fmm_downward(fmm_ctx* ctx);
 int fmm_evaluate_B(fmm_ctx* ctx, double current, double* B);         /* host, m x 3, input order */
 int fmm_evaluate_B_dev(fmm_ctx* ctx, double current, double* B);     /* device, m x 3, input order */
@@ -114,11 +137,26 @@ int fmm_run_B_dev(fmm_ctx* ctx, double current, double* B);
 /* reference output order (target-leaf DFS order): order[k] = input index of the k-th result */
 int fmm_leaf_order(fmm_ctx* ctx, int64_t* order);
 
+/* vector potential A = current / (4 pi) sum w J / |x - y| (the reference's scaling: no MU0), m x 3,
+ * input order; far part from the local expansions, close lists by the near_field rule */
+int fmm_evaluate_A(fmm_ctx* ctx, double current, double* A);         /* host */
+int fmm_evaluate_A_dev(fmm_ctx* ctx, double current, double* A);     /* device */
+
+/* treecode at m points (n x 3): B = current MU0 / (4 pi) curl A and A = current / (4 pi) sum M I
+ * by M2P walks from the source root (4 R^2 < d^2 accepts a box, failing leaves add nothing), after
+ * fmm_upward; B or A may be NULL.  Host and device-pointer variants. */
+int fmm_treecode(fmm_ctx* ctx, double current, size_t m, const double* xyz, double* B, double* A);
+int fmm_treecode_dev(fmm_ctx* ctx, double current, size_t m, const double* xyz, double* B, double* A);
+
 /* transposed FMM: dLdw[j] = sum_i dLdB[i] . dB_i/dw_j (device pointers, input order) */
 int fmm_adjoint_weights_dev(fmm_ctx* ctx, double current, const double* dLdB, double* dLdw);
 
-/* FP64 direct Biot-Savart sum over the context's sources at m device targets */
+/* FP64 direct sums over the context's sources at m device targets (B: Biot-Savart, A: vector
+ * potential); the sources are split into ranges whose partial sums (a library workspace) are added
+ * in range order */
 int fmm_direct_B_dev(fmm_ctx* ctx, double current, size_t m, const double* xyz, double* B);
+int fmm_direct_A_dev(fmm_ctx* ctx, double current, size_t m, const double* xyz, double* A);
+
 
 int fmm_get_stats(fmm_ctx* ctx, fmm_stats* stats);
 
