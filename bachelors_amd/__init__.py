@@ -99,6 +99,10 @@ def lib():
         L.fmm_set_targets.argtypes = [_vp, ctypes.c_size_t, _vp]
         L.fmm_set_targets_dev.argtypes = [_vp, ctypes.c_size_t, _vp]
         L.fmm_set_weights_dev.argtypes = [_vp, _vp]
+        L.fmm_set_bem_sources.argtypes = [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]
+        L.fmm_set_bem_sources_dev.argtypes = [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]
+        L.fmm_evaluate_bem.argtypes = [_vp, _vp]
+        L.fmm_evaluate_bem_dev.argtypes = [_vp, _vp]
         L.fmm_set_stream.argtypes = [_vp, _vp]
         L.fmm_evaluate_B.argtypes = [_vp, ctypes.c_double, _vp]
         L.fmm_evaluate_B_dev.argtypes = [_vp, ctypes.c_double, _vp]
@@ -174,6 +178,19 @@ class FMM:
         self._keep = (xyz, w, J)
         self.ns = len(xyz)
         self._check(lib().fmm_set_sources(self.h, len(xyz), _ptr(xyz), _ptr(w), _ptr(J)))
+
+    def set_bem_sources(self, xyz, w, B, normal):
+        """Problem::BEM sources (BEMQuadrature): weight, field B and normal per point."""
+        xyz, w, B, normal = (np.ascontiguousarray(a, np.float64) for a in (xyz, w, B, normal))
+        self._keep = (xyz, w, B, normal)
+        self.ns = len(xyz)
+        self._check(lib().fmm_set_bem_sources(self.h, len(xyz), _ptr(xyz), _ptr(w), _ptr(B), _ptr(normal)))
+
+    def evaluate_bem(self):
+        """calcBEM at every target, input order."""
+        out = np.zeros((self.nt, 3))
+        self._check(lib().fmm_evaluate_bem(self.h, _ptr(out)))
+        return out
 
     def set_targets(self, xyz):
         xyz = np.ascontiguousarray(xyz, np.float64)
@@ -336,9 +353,14 @@ class FastMultipoleSolver:
     def __init__(self, quadratures, points, problem=Problem.BioSavartLaplace, quadratureOctreeLeafCapacity=1000,
                  calculationPointOctreeLeafCapacity=100, weights=None, currents=None, order=10,
                  precision="fp64", frame=FRAME_REFERENCE, mac_scale=1.0, m2l_full=True, near_field=NEAR_P2P,
-                 max_depth=None):
-        if problem != Problem.BioSavartLaplace:
-            raise FMMError("only Problem.BioSavartLaplace is implemented")
+                 max_depth=None, bem_B=None, bem_normal=None):
+        """Problem.BEM takes BEMQuadrature sources: quadratures (N, 4) or positions + weights, with
+        bem_B and bem_normal (N, 3); their tree is rooted at Box((0,0,0), (1.1, 1.1, 1.6))
+        (multipole_solver.cpp:668-669)."""
+        problem = Problem(problem)
+        if problem == Problem.BEM and (bem_B is None or bem_normal is None):
+            raise FMMError("Problem.BEM needs bem_B and bem_normal for every quadrature")
+        self.problem = problem
         q = np.asarray(quadratures, np.float64)
         if weights is None:
             xyz, w = q[:, :3], q[:, 3]
@@ -350,11 +372,16 @@ class FastMultipoleSolver:
         self.calculationPointOctreeLeafCapacity = calculationPointOctreeLeafCapacity
         self.log = False
         self._points = np.ascontiguousarray(points, np.float64)
+        extra = {} if max_depth is None else {"max_depth": max_depth}
+        if problem == Problem.BEM:
+            extra.update(source_root_center=(0.0, 0.0, 0.0), source_root_half=(1.1, 1.1, 1.6))
         self._fmm = FMM(order=order, source_leaf_capacity=quadratureOctreeLeafCapacity,
                         target_leaf_capacity=calculationPointOctreeLeafCapacity, precision=precision, frame=frame,
-                        mac_scale=mac_scale, m2l_full=int(m2l_full), near_field=int(near_field),
-                        **({} if max_depth is None else {"max_depth": max_depth}))
-        self._fmm.set_sources(xyz, w, currents)
+                        mac_scale=mac_scale, m2l_full=int(m2l_full), near_field=int(near_field), **extra)
+        if problem == Problem.BEM:
+            self._fmm.set_bem_sources(xyz, w, bem_B, bem_normal)
+        else:
+            self._fmm.set_sources(xyz, w, currents)
         self._fmm.set_targets(self._points)
         self._fmm.build_trees()              # the constructor builds both trees (initTrees)
         self._leaves_ready = False
@@ -406,6 +433,16 @@ class FastMultipoleSolver:
         A = self._fmm.evaluate_A(current)
         order = self._fmm.leaf_order()
         return [(self._points[i], A[i]) for i in order]
+
+    def calcBEM(self, current=1.0):
+        """calcBEM (fast_multipole_solver.cpp:233-250): list of (point, value) in target-leaf order;
+        as in the reference, current does not enter."""
+        V = self._fmm.evaluate_bem()
+        order = self._fmm.leaf_order()
+        return [(self._points[i], V[i]) for i in order]
+
+    def calcBEM_array(self, current=1.0):
+        return self._fmm.evaluate_bem()
 
     def calcA_array(self, current):
         """A at every target in input order, (M, 3)."""

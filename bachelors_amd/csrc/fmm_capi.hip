@@ -112,6 +112,8 @@ struct fmm_ctx {
     // inputs (input order)
     int ns = 0, nt = 0;
     bool have_J = false;
+    bool bem = false;                            // Problem::BEM sources (B and normal per point)
+    DevBuf src_B, src_N, sbd;                    // BEM inputs (input order); w (B . n) per slot
     double tmin[3] = {1e+16, 1e+16, 1e+16}, tmax[3] = {-1e+16, -1e+16, -1e+16};   // target extent
     DevBuf src_xyz, src_w, src_J, tgt_xyz;
     // trees
@@ -357,9 +359,17 @@ void build_source_side(fmm_ctx* c) {
                g.source_leaf_capacity, g.max_depth, true, st);
     for (int k = 0; k < MAX_LEVELS; k++) c->sigS.s[k] = c->S.scale[k];
     int nss = c->S.num_slots;
-    launch_charges(c->S.slots.as<int>(), nss, c->src_xyz.as<double>(), c->src_w.as<double>(),
-                   c->have_J ? c->src_J.as<double>() : nullptr, c->sxyz.get<double>(3 * (size_t)std::max(nss, 1)),
-                   c->sq.get<double>(3 * (size_t)std::max(nss, 1)), c->sJ.get<double>(3 * (size_t)std::max(nss, 1)), st);
+    if (c->bem)
+        launch_bem_charges(c->S.slots.as<int>(), nss, c->src_xyz.as<double>(), c->src_w.as<double>(),
+                           c->src_B.as<double>(), c->src_N.as<double>(),
+                           c->sxyz.get<double>(3 * (size_t)std::max(nss, 1)),
+                           c->sq.get<double>(3 * (size_t)std::max(nss, 1)), c->sbd.get<double>((size_t)std::max(nss, 1)),
+                           st);
+    else
+        launch_charges(c->S.slots.as<int>(), nss, c->src_xyz.as<double>(), c->src_w.as<double>(),
+                       c->have_J ? c->src_J.as<double>() : nullptr, c->sxyz.get<double>(3 * (size_t)std::max(nss, 1)),
+                       c->sq.get<double>(3 * (size_t)std::max(nss, 1)), c->sJ.get<double>(3 * (size_t)std::max(nss, 1)),
+                       st);
     upload_leaves(c->S, c->h_src_leaves, c->src_leaves, st);
 }
 
@@ -407,8 +417,12 @@ void stage_leaves(fmm_ctx* c) {
     size_t n = (size_t)c->S.num_nodes * 3 * c->Lp;
     void* M = c->M.reserve(n * c->esize());
     FMM_HIP(hipMemsetAsync(M, 0, n * c->esize(), c->stream));
-    launch_p2m(c->f64(), c->src_leaves.as<int>(), (int)c->h_src_leaves.size(), c->S, c->sigS, c->sxyz.as<double>(),
-               c->sq.as<double>(), c->p, c->Lp, M, c->stream);
+    if (c->bem)   // Problem::BEM (multipole_solver.cpp:78-83 -> integration.cpp:295-353)
+        launch_p2m_bem(c->f64(), c->src_leaves.as<int>(), (int)c->h_src_leaves.size(), c->S, c->sigS,
+                       c->sxyz.as<double>(), c->sq.as<double>(), c->sbd.as<double>(), c->p, c->Lp, M, c->stream);
+    else
+        launch_p2m(c->f64(), c->src_leaves.as<int>(), (int)c->h_src_leaves.size(), c->S, c->sigS,
+                   c->sxyz.as<double>(), c->sq.as<double>(), c->p, c->Lp, M, c->stream);
     size_t nss = (size_t)std::max(c->S.num_slots, 1), rs = c->f64() ? sizeof(double) : sizeof(float);
     // origin = source root centre (node 0); the P2P kernel reads the same three doubles
     launch_pack_sources(c->f64(), c->S.num_slots, c->S.center.as<double>(), c->sxyz.as<double>(), c->sq.as<double>(),
@@ -823,7 +837,8 @@ void stage_evaluate(fmm_ctx* c, double current, double* Bdev) {
 }
 
 // calcA(current) (fast_multipole_solver.cpp:195-212): A = current / (4 pi) (sum L R(x - c) over
-// the target's leaf + the close list), input order; targets outside the target tree get NaN
+// the target's leaf + the close list), input order; targets outside the target tree get NaN.  For
+// BEM sources this is calcBEM (:233-250) with current = 1.
 void stage_evaluate_A(fmm_ctx* c, double current, double* Adev) {
     require(c->stage >= ST_DOWN, FMM_ERR_STATE, "local expansions are not ready");
     hipStream_t st = c->stream;
@@ -838,6 +853,10 @@ void stage_evaluate_A(fmm_ctx* c, double current, double* Adev) {
         launch_m2p_close(c->f64(), c->S, c->sigS, c->M.ptr, c->p, c->Lp, c->tgt_leaves.as<int>(), nl, c->T,
                          c->lists.close_begin.as<int>(), c->lists.close_end.as<int>(), c->lists.close_s.as<int>(),
                          c->txyz.as<double>(), nullptr, An, st);
+    else if (c->cfg.near_field && c->bem)
+        launch_p2p_bem(c->tgt_leaves.as<int>(), nl, c->T, c->S, c->lists.close_begin.as<int>(),
+                       c->lists.close_end.as<int>(), c->lists.close_s.as<int>(), c->sxyz.as<double>(),
+                       c->sq.as<double>(), c->sbd.as<double>(), c->txyz.as<double>(), An, st);
     else if (c->cfg.near_field)
         launch_p2p_A(c->f64(), c->tgt_leaves.as<int>(), nl, c->T, c->S, c->lists.close_begin.as<int>(),
                      c->lists.close_end.as<int>(), c->lists.close_s.as<int>(), c->nP0.ptr, c->nP1.ptr, c->nP2.ptr,
@@ -958,6 +977,7 @@ void set_sources_common(fmm_ctx* c, size_t n, const double* xyz, const double* w
     require(n == 0 || (xyz && w), FMM_ERR_ARG, "null source arrays");
     c->ns = (int)n;
     c->have_J = J != nullptr;
+    c->bem = false;
     size_t nn = std::max<size_t>(n, 1);
     if (n) {
         FMM_HIP(hipMemcpyAsync(c->src_xyz.get<double>(3 * nn), xyz, 3 * n * sizeof(double), kind, c->stream));
@@ -969,6 +989,22 @@ void set_sources_common(fmm_ctx* c, size_t n, const double* xyz, const double* w
     }
     FMM_HIP(hipStreamSynchronize(c->stream));
     c->stage = ST_EMPTY;
+}
+
+// Problem::BEM sources: position, weight, field B and normal n per point (BEMQuadrature)
+void set_bem_sources_common(fmm_ctx* c, size_t n, const double* xyz, const double* w, const double* Bv,
+                            const double* nrm, hipMemcpyKind kind) {
+    require(n == 0 || (Bv && nrm), FMM_ERR_ARG, "null BEM source arrays");
+    set_sources_common(c, n, xyz, w, nullptr, kind);
+    c->bem = true;
+    size_t nn = std::max<size_t>(n, 1);
+    double* dB = c->src_B.get<double>(3 * nn);
+    double* dN = c->src_N.get<double>(3 * nn);
+    if (n) {
+        FMM_HIP(hipMemcpyAsync(dB, Bv, 3 * n * sizeof(double), kind, c->stream));
+        FMM_HIP(hipMemcpyAsync(dN, nrm, 3 * n * sizeof(double), kind, c->stream));
+    }
+    FMM_HIP(hipStreamSynchronize(c->stream));
 }
 
 void set_targets_common(fmm_ctx* c, size_t m, const double* xyz, hipMemcpyKind kind) {
@@ -1114,9 +1150,18 @@ int fmm_set_sources_dev(fmm_ctx* c, size_t n, const double* xyz, const double* w
 int fmm_set_targets_dev(fmm_ctx* c, size_t m, const double* xyz) {
     return guard(c, [&] { set_targets_common(c, m, xyz, hipMemcpyDeviceToDevice); });
 }
+int fmm_set_bem_sources(fmm_ctx* c, size_t n, const double* xyz, const double* w, const double* B,
+                        const double* normal) {
+    return guard(c, [&] { set_bem_sources_common(c, n, xyz, w, B, normal, hipMemcpyHostToDevice); });
+}
+int fmm_set_bem_sources_dev(fmm_ctx* c, size_t n, const double* xyz, const double* w, const double* B,
+                            const double* normal) {
+    return guard(c, [&] { set_bem_sources_common(c, n, xyz, w, B, normal, hipMemcpyDeviceToDevice); });
+}
 int fmm_set_weights_dev(fmm_ctx* c, const double* w) {
     return guard(c, [&] {
         require(w != nullptr, FMM_ERR_ARG, "null weights");
+        require(!c->bem, FMM_ERR_UNSUPPORTED, "new weights for BEM sources: set the sources again");
         join_default_stream(c);
         if (c->ns) FMM_HIP(hipMemcpyAsync(c->src_w.ptr, w, c->ns * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         if (c->stage >= ST_TREES) {
@@ -1184,6 +1229,7 @@ int fmm_adjoint_weights_dev(fmm_ctx* c, double current, const double* dLdB, doub
         require(c->cfg.frame == FMM_FRAME_SHARED, FMM_ERR_UNSUPPORTED, "the adjoint needs the shared frame");
         require(c->cfg.near_field != FMM_NEAR_TREECODE, FMM_ERR_UNSUPPORTED,
                 "the adjoint has no transposed treecode near field");
+        require(!c->bem, FMM_ERR_UNSUPPORTED, "the adjoint is for current weights, not BEM sources");
         join_default_stream(c);
         AdjointArgs a;
         a.prec = c->pass_prec();
@@ -1297,6 +1343,24 @@ int fmm_evaluate_A(fmm_ctx* c, double current, double* A) {
         stage_evaluate_A(c, current, d);
         if (c->nt) FMM_HIP(hipMemcpyAsync(A, d, 3 * (size_t)c->nt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         FMM_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int fmm_evaluate_bem(fmm_ctx* c, double* out) {
+    return guard(c, [&] {
+        require(c->bem, FMM_ERR_STATE, "the sources are not BEM sources (fmm_set_bem_sources)");
+        require(out != nullptr || c->nt == 0, FMM_ERR_ARG, "null output");
+        double* d = c->Aout.get<double>(3 * (size_t)std::max(c->nt, 1));
+        stage_evaluate_A(c, 1.0, d);
+        if (c->nt) FMM_HIP(hipMemcpyAsync(out, d, 3 * (size_t)c->nt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        FMM_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+int fmm_evaluate_bem_dev(fmm_ctx* c, double* out) {
+    return guard(c, [&] {
+        require(c->bem, FMM_ERR_STATE, "the sources are not BEM sources (fmm_set_bem_sources)");
+        require(out != nullptr || c->nt == 0, FMM_ERR_ARG, "null output");
+        stage_evaluate_A(c, 1.0, out);
     });
 }
 

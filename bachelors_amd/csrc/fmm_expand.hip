@@ -469,6 +469,182 @@ __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, con
     }
 }
 
+// ---------------------------------------------------------------- Problem::BEM
+// BEMQuadrature sources (bem_quadrature.hpp): weight w, field B and normal n per point.  The
+// source strengths in slot order are d = w (B . n) and v = w (B x n) (3); their multipoles are
+// M_c(l, m) = -sum_q (dR_lm/dy_c d_q + (grad R_lm x v_q)_c) (integration.cpp:295-353, with the
+// exact gradient in place of the reference's central differences), the potential they give is
+// sum_q ((y - x) d + (y - x) x v) / |y - x|^3 (integration.cpp:191-202).
+__global__ void k_bem_charges(const int* __restrict__ slots, int n, const double* __restrict__ xyz,
+                              const double* __restrict__ w, const double* __restrict__ Bv,
+                              const double* __restrict__ nrm, double* __restrict__ sxyz, double* __restrict__ sv,
+                              double* __restrict__ sd) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int i = slots[k];
+    const double b0 = Bv[3 * i], b1 = Bv[3 * i + 1], b2 = Bv[3 * i + 2];
+    const double n0 = nrm[3 * i], n1 = nrm[3 * i + 1], n2 = nrm[3 * i + 2], ww = w[i];
+    sxyz[3 * k] = xyz[3 * i]; sxyz[3 * k + 1] = xyz[3 * i + 1]; sxyz[3 * k + 2] = xyz[3 * i + 2];
+    sd[k] = ww * (b0 * n0 + b1 * n1 + b2 * n2);
+    sv[3 * k] = ww * (b1 * n2 - b2 * n1);
+    sv[3 * k + 1] = ww * (b2 * n0 - b0 * n2);
+    sv[3 * k + 2] = ww * (b0 * n1 - b1 * n0);
+}
+
+// One workgroup (64 threads) per useful source leaf.  Points go through LDS in chunks of
+// BEM_CHUNK: thread t builds the complex table Phi_n^k (n <= p - 1, k >= 0) of point t; then each
+// thread owns coefficients i = tid + 64 r and adds the gradient of R_i at every point of the chunk:
+// dz Phi_l^m = Phi_{l-1}^m, dx Phi_l^m = (Phi_{l-1}^{m-1} - Phi_{l-1}^{m+1}) / 2,
+// dy Phi_l^m = i (Phi_{l-1}^{m-1} + Phi_{l-1}^{m+1}) / 2, Phi_n^{-k} = (-1)^k conj Phi_n^k.
+constexpr int BEM_CHUNK = 8;
+template <class T>
+__global__ __launch_bounds__(64) void k_p2m_bem(const int* __restrict__ leaves, const double* __restrict__ center,
+                                                const int* __restrict__ begin, const int* __restrict__ end,
+                                                const int* __restrict__ level, ScaleTable sig,
+                                                const double* __restrict__ sxyz, const double* __restrict__ sv,
+                                                const double* __restrict__ sd, int p, int Lp, T* __restrict__ M) {
+    extern __shared__ double2 tab[];   // [BEM_CHUNK][p (p + 1) / 2]
+    __shared__ double str[BEM_CHUNK][4];
+    const int NT = p * (p + 1) / 2, L = (p + 1) * (p + 1);
+    const int leaf = leaves[blockIdx.x], tid = threadIdx.x;
+    const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
+    const double inv = 1.0 / sig.s[level[leaf]];
+    constexpr int RMAX = (MAX_ORDER + 1) * (MAX_ORDER + 1) / 64 + 1;
+    double acc[RMAX][3];
+    for (int r = 0; r < RMAX; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
+    const int jb = begin[leaf], je = end[leaf];
+    for (int j0 = jb; j0 < je; j0 += BEM_CHUNK) {
+        const int nj = min(BEM_CHUNK, je - j0);
+        __syncthreads();
+        if (tid < nj) {
+            const int j = j0 + tid;
+            const double ux = (sxyz[3 * j] - cx) * inv, uy = (sxyz[3 * j + 1] - cy) * inv,
+                         uz = (sxyz[3 * j + 2] - cz) * inv, r2 = ux * ux + uy * uy + uz * uz;
+            double2* t = tab + tid * NT;
+            double dr = 1.0, di = 0.0;   // Phi_k^k = (x + iy)^k / (2^k k!)
+            for (int k = 0; k < p; k++) {
+                if (k > 0) {
+                    const double s = recip(2 * k);
+                    const double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
+                    dr = nr; di = ni;
+                }
+                double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
+                t[k * (k + 1) / 2 + k] = make_double2(dr, di);
+                for (int n = k + 1; n < p; n++) {
+                    const double f = recip(n * n - k * k);
+                    const double vr = ((2.0 * n - 1) * uz * p1r - r2 * p2r) * f;
+                    const double vi = ((2.0 * n - 1) * uz * p1i - r2 * p2i) * f;
+                    p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
+                    t[n * (n + 1) / 2 + k] = make_double2(vr, vi);
+                }
+            }
+            str[tid][0] = sd[j] * inv;
+            str[tid][1] = sv[3 * j] * inv;
+            str[tid][2] = sv[3 * j + 1] * inv;
+            str[tid][3] = sv[3 * j + 2] * inv;
+        }
+        __syncthreads();
+        for (int r = 0; r < RMAX; r++) {
+            const int i = tid + 64 * r;
+            if (i >= L) break;
+            int l = 0;
+            while ((l + 1) * (l + 1) <= i) l++;
+            const int m = i - l * l - l, am = m < 0 ? -m : m;
+            if (l == 0) continue;
+            const int n = l - 1;
+            const double sg = ((l & 1) ? -1.0 : 1.0) * (am == 0 ? 1.0 : 1.41421356237309504880);
+            for (int q = 0; q < nj; q++) {
+                const double2* t = tab + q * NT;
+                // Phi_{l-1}^{k} for k = am - 1, am, am + 1 (signed orders through the conjugate rule)
+                auto ph = [&](int k) -> double2 {
+                    if (k > n || k < -n) return make_double2(0.0, 0.0);
+                    if (k >= 0) return t[n * (n + 1) / 2 + k];
+                    const double2 v = t[n * (n + 1) / 2 - k];
+                    const double s = (k & 1) ? -1.0 : 1.0;
+                    return make_double2(s * v.x, -s * v.y);
+                };
+                const double2 lo = ph(am - 1), mid = ph(am), hi = ph(am + 1);
+                const double gxr = 0.5 * (lo.x - hi.x), gxi = 0.5 * (lo.y - hi.y);
+                const double gyr = -0.5 * (lo.y + hi.y), gyi = 0.5 * (lo.x + hi.x);
+                const double gzr = mid.x, gzi = mid.y;
+                const bool im = m < 0;
+                const double gx = sg * (im ? gxi : gxr), gy = sg * (im ? gyi : gyr), gz = sg * (im ? gzi : gzr);
+                const double d = str[q][0], v0 = str[q][1], v1 = str[q][2], v2 = str[q][3];
+                acc[r][0] -= gx * d + (gy * v2 - gz * v1);
+                acc[r][1] -= gy * d + (gz * v0 - gx * v2);
+                acc[r][2] -= gz * d + (gx * v1 - gy * v0);
+            }
+        }
+    }
+    T* out = M + (size_t)leaf * 3 * Lp;
+    for (int e = tid; e < 3 * Lp; e += 64) out[e] = (T)0;
+    __syncthreads();
+    for (int r = 0; r < RMAX; r++) {
+        const int i = tid + 64 * r;
+        if (i >= L) break;
+        for (int c = 0; c < 3; c++) out[c * Lp + i] = (T)acc[r][c];
+    }
+}
+
+// near field of the BEM potential, FP64: one thread per target of a useful target leaf, over the
+// sources of every node of the leaf's close list
+__global__ __launch_bounds__(64) void k_p2p_bem(const int* __restrict__ leaves, const int* __restrict__ tbegin,
+                                                const int* __restrict__ tend, const int* __restrict__ close_begin,
+                                                const int* __restrict__ close_end, const int* __restrict__ close_s,
+                                                const int* __restrict__ sbegin, const int* __restrict__ send,
+                                                const double* __restrict__ sxyz, const double* __restrict__ sv,
+                                                const double* __restrict__ sd, const double* __restrict__ txyz,
+                                                double* __restrict__ out) {
+    const int leaf = leaves[blockIdx.x];
+    for (int k = tbegin[leaf] + threadIdx.x; k < tend[leaf]; k += blockDim.x) {
+        const double x = txyz[3 * k], y = txyz[3 * k + 1], z = txyz[3 * k + 2];
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (int ci = close_begin[leaf]; ci < close_end[leaf]; ci++) {
+            const int s = close_s[ci];
+            for (int j = sbegin[s]; j < send[s]; j++) {
+                const double q0 = sxyz[3 * j] - x, q1 = sxyz[3 * j + 1] - y, q2 = sxyz[3 * j + 2] - z;
+                const double r2 = q0 * q0 + q1 * q1 + q2 * q2;
+                if (r2 == 0.0) continue;
+                const double r3 = 1.0 / (r2 * sqrt(r2)), d = sd[j];
+                const double v0 = sv[3 * j], v1 = sv[3 * j + 1], v2 = sv[3 * j + 2];
+                a0 += (q0 * d + (q1 * v2 - q2 * v1)) * r3;
+                a1 += (q1 * d + (q2 * v0 - q0 * v2)) * r3;
+                a2 += (q2 * d + (q0 * v1 - q1 * v0)) * r3;
+            }
+        }
+        out[3 * k] = a0;
+        out[3 * k + 1] = a1;
+        out[3 * k + 2] = a2;
+    }
+}
+
+void launch_bem_charges(const int* slots, int n, const double* xyz, const double* w, const double* Bv,
+                        const double* nrm, double* sxyz, double* sv, double* sd, hipStream_t st) {
+    if (n) k_bem_charges<<<blocks(n), 256, 0, st>>>(slots, n, xyz, w, Bv, nrm, sxyz, sv, sd);
+    FMM_HIP(hipGetLastError());
+}
+void launch_p2m_bem(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig,
+                    const double* sxyz, const double* sv, const double* sd, int p, int Lp, void* M, hipStream_t st) {
+    if (!nleaves) return;
+    if (p < 1 || p > MAX_ORDER) throw Error(1, "BEM P2M: order out of range");
+    const size_t lds = (size_t)BEM_CHUNK * (p * (p + 1) / 2) * sizeof(double2);
+    if (f64)
+        k_p2m_bem<double><<<nleaves, 64, lds, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
+                                                    S.level.as<int>(), sig, sxyz, sv, sd, p, Lp, (double*)M);
+    else
+        k_p2m_bem<float><<<nleaves, 64, lds, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
+                                                   S.level.as<int>(), sig, sxyz, sv, sd, p, Lp, (float*)M);
+    FMM_HIP(hipGetLastError());
+}
+void launch_p2p_bem(const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
+                    const int* close_end, const int* close_s, const double* sxyz, const double* sv, const double* sd,
+                    const double* txyz, double* out, hipStream_t st) {
+    if (!nleaves) return;
+    k_p2p_bem<<<nleaves, 64, 0, st>>>(leaves, T.begin.as<int>(), T.end.as<int>(), close_begin, close_end, close_s,
+                                      S.begin.as<int>(), S.end.as<int>(), sxyz, sv, sd, txyz, out);
+    FMM_HIP(hipGetLastError());
+}
+
 // ---------------------------------------------------------------- assembly
 __global__ void k_assemble(const int* __restrict__ slots, int n, const double* __restrict__ Bfar,
                            const double* __restrict__ Bnear, double scale, double* __restrict__ B) {

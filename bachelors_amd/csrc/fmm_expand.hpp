@@ -25,6 +25,15 @@ void launch_l2p_A(bool f64, const int* leaves, int nleaves, const TreeDev& T, co
 void launch_p2p_A(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
                   const int* close_end, const int* close_s, const void* P0, const void* P1, const void* P2,
                   const double* txyz, double* Anear, hipStream_t st);
+// Problem::BEM: slot-order strengths v = w (B x n) (3) and d = w (B . n) of BEMQuadrature sources;
+// their multipoles (P2M of the gradient of R); the near field of their potential (FP64, unscaled)
+void launch_bem_charges(const int* slots, int n, const double* xyz, const double* w, const double* Bv,
+                        const double* nrm, double* sxyz, double* sv, double* sd, hipStream_t st);
+void launch_p2m_bem(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig,
+                    const double* sxyz, const double* sv, const double* sd, int p, int Lp, void* M, hipStream_t st);
+void launch_p2p_bem(const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
+                    const int* close_end, const int* close_s, const double* sxyz, const double* sv, const double* sd,
+                    const double* txyz, double* out, hipStream_t st);
 // FP64 direct sum of B (pot: A) at nt targets over the ns slot-order sources, scaled; the sources
 // are cut into nsplit ranges whose partial sums (partial: nsplit * nt * 3 doubles) are added in
 // range order.  direct_splits: the number of ranges that fills the GPU.

@@ -28,6 +28,13 @@ struct Quadrature : Vector3 {   // quadrature.hpp:4-10: weight = volume * basis 
     Quadrature(const Vector3& c, double volume, double w) : Vector3(c), weight(volume * w) {}
 };
 
+struct BEMQuadrature : Quadrature {   // bem_quadrature.hpp: a quadrature with a field B and a normal
+    Vector3 B, normal;
+    BEMQuadrature() = default;
+    BEMQuadrature(const Vector3& c, double area, double w, const Vector3& B_, const Vector3& normal_)
+        : Quadrature(c, area, w), B(B_), normal(normal_) {}
+};
+
 enum class M2MAlg { NoTranslation = 0, ComplexTranslation, RealTranslation, Layers, Matrices };   // enums :3-10
 enum class Device { CPU = 0, GPU, Adaptive };
 enum class M2LAlg { ComplexTranslation = 0, Matrices };
@@ -79,6 +86,45 @@ public:
         check(fmm_set_targets(_ctx, points.size(), t.data()), "fmm_set_targets");
         check(fmm_build_trees(_ctx), "fmm_build_trees");   // initTrees (fast_multipole_solver.cpp:43-60)
     }
+    // fast_multipole_solver.hpp:32-37: Problem::BEM with BEMQuadrature sources; their tree is rooted
+    // at Box((0,0,0), (1.1, 1.1, 1.6)) (multipole_solver.cpp:668-669)
+    FastMultipoleSolver(std::vector<BEMQuadrature>& quadratures, std::vector<Vector3>& points,
+                        Problem problem = Problem::BEM, size_t quadratureOctreeLeafCapacity_ = 1000,
+                        size_t calculationPointOctreeLeafCapacity_ = 100, int order = 10,
+                        int precision = FMM_PREC_FP64, int frame = FMM_FRAME_REFERENCE,
+                        int near_field = FMM_NEAR_P2P)
+        : harmonicOrder(order), harmonicLength((order + 1) * (order + 1)),
+          quadratureOctreeLeafCapacity(quadratureOctreeLeafCapacity_),
+          calculationPointOctreeLeafCapacity(calculationPointOctreeLeafCapacity_), _points(points) {
+        if (problem != Problem::BEM) throw std::runtime_error("BEMQuadrature sources are for Problem::BEM");
+        fmm_config cfg;
+        fmm_config_init(&cfg);
+        cfg.order = order;
+        cfg.source_leaf_capacity = (int)quadratureOctreeLeafCapacity_;
+        cfg.target_leaf_capacity = (int)calculationPointOctreeLeafCapacity_;
+        cfg.precision = precision;
+        cfg.frame = frame;
+        cfg.near_field = near_field;
+        const double half[3] = {1.1, 1.1, 1.6};
+        for (int a = 0; a < 3; a++) { cfg.source_root_center[a] = 0.0; cfg.source_root_half[a] = half[a]; }
+        check(fmm_create(&_ctx, &cfg), "fmm_create");
+        const size_t n = quadratures.size();
+        std::vector<double> xyz(3 * n), w(n), B(3 * n), nrm(3 * n);
+        for (size_t i = 0; i < n; i++) {
+            const BEMQuadrature& q = quadratures[i];
+            xyz[3 * i] = q.x; xyz[3 * i + 1] = q.y; xyz[3 * i + 2] = q.z;
+            w[i] = q.weight;
+            B[3 * i] = q.B.x; B[3 * i + 1] = q.B.y; B[3 * i + 2] = q.B.z;
+            nrm[3 * i] = q.normal.x; nrm[3 * i + 1] = q.normal.y; nrm[3 * i + 2] = q.normal.z;
+        }
+        std::vector<double> t(3 * points.size());
+        for (size_t i = 0; i < points.size(); i++) {
+            t[3 * i] = points[i].x; t[3 * i + 1] = points[i].y; t[3 * i + 2] = points[i].z;
+        }
+        check(fmm_set_bem_sources(_ctx, n, xyz.data(), w.data(), B.data(), nrm.data()), "fmm_set_bem_sources");
+        check(fmm_set_targets(_ctx, points.size(), t.data()), "fmm_set_targets");
+        check(fmm_build_trees(_ctx), "fmm_build_trees");
+    }
     FastMultipoleSolver(const FastMultipoleSolver&) = delete;
     FastMultipoleSolver& operator=(const FastMultipoleSolver&) = delete;
     ~FastMultipoleSolver() { fmm_destroy(_ctx); }
@@ -114,6 +160,12 @@ public:
         std::vector<double> A(3 * _points.size());
         check(fmm_evaluate_A(_ctx, current, A.data()), "calcA");
         return leaf_ordered(A);
+    }
+    // fast_multipole_solver.cpp:233-250: BEM sources, target-leaf order (current does not enter)
+    std::vector<std::pair<Vector3, Vector3>> calcBEM(double /*current*/) {
+        std::vector<double> V(3 * _points.size());
+        check(fmm_evaluate_bem(_ctx, V.data()), "calcBEM");
+        return leaf_ordered(V);
     }
     // multipole_solver.cpp:686-700: the treecode at one point (after calclMultipoleExpansions)
     Vector3 calcB(double current, const Vector3& point) {

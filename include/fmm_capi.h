@@ -18,6 +18,10 @@
  *     fast_multipole_solver.hpp:45, fast_multipole_solver.cpp:195-212
  *   MultipoleSolver::calcA / calcB(real current, const Vector3&)   fmm_treecode (many points at once)
  *     multipole_solver.hpp:46-47, multipole_solver.cpp:686-700
+ *   FastMultipoleSolver(vector<BEMQuadrature>&, ..., Problem::BEM)  fmm_set_bem_sources
+ *     fast_multipole_solver.hpp:32-37, multipole_solver.cpp:38-54
+ *   FastMultipoleSolver::calcBEM(real current)                     fmm_evaluate_bem (+ fmm_leaf_order)
+ *     fast_multipole_solver.hpp:47, fast_multipole_solver.cpp:233-250
  *   MultipoleSolver::getQuadratureOctreeNodeCount()                fmm_tree_size(ctx, 0, ...)
  *     multipole_solver.hpp:40, multipole_solver.cpp:97-100
  *   math::calcBioSavartLaplace(current, point, quadratures)         fmm_direct_B_dev
@@ -112,6 +116,14 @@ int fmm_set_sources(fmm_ctx* ctx, size_t n, const double* xyz, const double* w, 
 int fmm_set_targets(fmm_ctx* ctx, size_t m, const double* xyz);
 int fmm_set_sources_dev(fmm_ctx* ctx, size_t n, const double* xyz, const double* w, const double* J);
 int fmm_set_weights_dev(fmm_ctx* ctx, const double* w);   /* new weights, same sources and trees */
+/* Problem::BEM sources (BEMQuadrature, bem_quadrature.hpp; FastMultipoleSolver(vector<BEMQuadrature>&,
+ * ...), fast_multipole_solver.hpp:32-37): n points with weight w, field B (n x 3) and normal (n x 3).
+ * The reference roots their tree at Box((0,0,0), (1.1, 1.1, 1.6)) (multipole_solver.cpp:668-669):
+ * set source_root_center / source_root_half of the config to that box. */
+int fmm_set_bem_sources(fmm_ctx* ctx, size_t n, const double* xyz, const double* w, const double* B,
+                        const double* normal);
+int fmm_set_bem_sources_dev(fmm_ctx* ctx, size_t n, const double* xyz, const double* w, const double* B,
+                            const double* normal);
 int fmm_set_targets_dev(fmm_ctx* ctx, size_t m, const double* xyz);
 
 /* M2M algorithms of MultipoleSolver::calclMultipoleExpansions (multipole_solver_enums.hpp:3-10,
@@ -141,6 +153,12 @@ int fmm_leaf_order(fmm_ctx* ctx, int64_t* order);
  * input order; far part from the local expansions, close lists by the near_field rule */
 int fmm_evaluate_A(fmm_ctx* ctx, double current, double* A);         /* host */
 int fmm_evaluate_A_dev(fmm_ctx* ctx, double current, double* A);     /* device */
+
+/* FastMultipoleSolver::calcBEM (fast_multipole_solver.hpp:47, fast_multipole_solver.cpp:233-250) of BEM
+ * sources: sum w ((y - x)(B . n) + (y - x) x (B x n)) / |y - x|^3 / (4 pi) at every target (m x 3, input
+ * order; the reference's current argument does not enter) */
+int fmm_evaluate_bem(fmm_ctx* ctx, double* out);       /* host */
+int fmm_evaluate_bem_dev(fmm_ctx* ctx, double* out);   /* device */
 
 /* treecode at m points (n x 3): B = current MU0 / (4 pi) curl A and A = current / (4 pi) sum M I
  * by M2P walks from the source root (4 R^2 < d^2 accepts a box, failing leaves add nothing), after

@@ -543,6 +543,7 @@ struct Config {
     double current = 1.0;
     double fd_eps = FD_EPS;          // FD curl step (reference 1e-3: calculation_point_octree.cpp:217,
                                      // quadrature_octree.cpp:201)
+    int problem = 0;                 // 0 BioSavartLaplace, 1 BEM (multipole_solver_enums.hpp:26-30)
 };
 struct Run {
     Config cfg;
@@ -550,6 +551,7 @@ struct Run {
     Tree* T = nullptr;
     Lists lists;
     std::vector<double> q;           // per source: w*J (3)
+    std::vector<double> bem_d, bem_v;   // BEM per source: w (B.n) and w (B x n) (3)
     std::vector<std::vector<double>> M, Lc;   // per storage node: 3*L, component-major
     std::vector<double> B, Bfar, Bnear;       // per target slot (3 each)
     std::vector<double> A, Afar, Anear;       // vector potential per target slot (calcA(current))
@@ -559,8 +561,52 @@ struct Run {
 typedef std::chrono::steady_clock clk;
 static double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
+// BEM P2M: integration.cpp:295-353, M_c -= w (gradR_c (B.n) + (gradR x (B x n))_c) with gradR
+// by central differences of step eps (reference) or, with ANALYTIC_CURL, the exact gradient
+static void p2m_leaf_bem(const Run& R, const Tree& S, int s, std::vector<double>& out) {
+    const int p = R.cfg.p, L = nlen(p);
+    out.assign(3 * L, 0.0);
+    std::vector<double> h1(L), h2(L), g(3 * L), ga(3 * L);
+    const double e = R.cfg.fd_eps;
+    for (int qi : S.nodes[s].pts) {
+        const double* y = S.P + 3 * (size_t)qi;
+        const double dx = y[0] - S.nodes[s].c[0], dy = y[1] - S.nodes[s].c[1], dz = y[2] - S.nodes[s].c[2];
+        if (R.cfg.flags & ANALYTIC_CURL) {
+            regular_grad_cart(p, dx, dy, dz, ga.data());
+            for (int i = 0; i < L; i++)
+                for (int a = 0; a < 3; a++) g[a * L + i] = ga[3 * i + a];
+        } else {
+            const double off[3][3] = {{e, 0, 0}, {0, e, 0}, {0, 0, e}};
+            for (int a = 0; a < 3; a++) {
+                solid_harmonics(p, dx + off[a][0], dy + off[a][1], dz + off[a][2], true, R.cfg.flags, h1.data());
+                solid_harmonics(p, dx - off[a][0], dy - off[a][1], dz - off[a][2], true, R.cfg.flags, h2.data());
+                for (int i = 0; i < L; i++) g[a * L + i] = (h1[i] - h2[i]) / (2 * e);
+            }
+        }
+        const double d = R.bem_d[qi], *v = R.bem_v.data() + 3 * (size_t)qi;
+        for (int i = 0; i < L; i++) {
+            const double gx = g[i], gy = g[L + i], gz = g[2 * L + i];
+            out[i] -= gx * d + (gy * v[2] - gz * v[1]);
+            out[L + i] -= gy * d + (gz * v[0] - gx * v[2]);
+            out[2 * L + i] -= gz * d + (gx * v[1] - gy * v[0]);
+        }
+    }
+}
+// the BEM integrand of one source at x (integration.cpp:191-202), times its weight:
+// w ((y - x)(B.n) + (y - x) x (B x n)) / |y - x|^3
+inline void bem_kernel(const double* x, const double* y, double d, const double* v, double acc[3]) {
+    const double q[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+    const double r2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    if (r2 == 0) return;
+    const double r3 = r2 * std::sqrt(r2);
+    acc[0] += (q[0] * d + (q[1] * v[2] - q[2] * v[1])) / r3;
+    acc[1] += (q[1] * d + (q[2] * v[0] - q[0] * v[2])) / r3;
+    acc[2] += (q[2] * d + (q[0] * v[1] - q[1] * v[0])) / r3;
+}
+
 // P2M: integration.cpp:137-161 (one component-major expansion per leaf)
 static void p2m_leaf(const Run& R, const Tree& S, int s, std::vector<double>& out) {
+    if (R.cfg.problem == 1) { p2m_leaf_bem(R, S, s, out); return; }
     const int p = R.cfg.p, L = nlen(p);
     out.assign(3 * L, 0.0);
     std::vector<double> Rh(L);
@@ -643,7 +689,7 @@ static void collect_points(const Tree& S, int s, std::vector<int>& out) {
 }
 
 static Run* run_fmm(const Config& cfg, size_t ns, const double* src, const double* w, const double* J,
-                    size_t nt, const double* tgt) {
+                    size_t nt, const double* tgt, const double* bemB = nullptr, const double* bemN = nullptr) {
     Run* R = new Run();
     R->cfg = cfg;
     const int p = cfg.p, L = nlen(p);
@@ -653,6 +699,18 @@ static Run* run_fmm(const Config& cfg, size_t ns, const double* src, const doubl
         if (J) { j[0] = J[3 * i]; j[1] = J[3 * i + 1]; j[2] = J[3 * i + 2]; }
         else phi_hat(src + 3 * i, j);
         for (int c = 0; c < 3; c++) R->q[3 * i + c] = w[i] * j[c];
+    }
+    if (cfg.problem == 1) {   // BEMQuadrature: B and normal per source (bem_quadrature.hpp)
+        R->bem_d.resize(ns);
+        R->bem_v.resize(3 * ns);
+        for (size_t i = 0; i < ns; i++) {
+            const double* b = bemB + 3 * i;
+            const double* n = bemN + 3 * i;
+            R->bem_d[i] = w[i] * (b[0] * n[0] + b[1] * n[1] + b[2] * n[2]);
+            R->bem_v[3 * i] = w[i] * (b[1] * n[2] - b[2] * n[1]);
+            R->bem_v[3 * i + 1] = w[i] * (b[2] * n[0] - b[0] * n[2]);
+            R->bem_v[3 * i + 2] = w[i] * (b[0] * n[1] - b[1] * n[0]);
+        }
     }
     auto t0 = clk::now();
     R->S = build_tree(ns, src, cfg.source_root_c, cfg.source_root_h, cfg.cap_s, cfg.max_depth);
@@ -858,18 +916,23 @@ static Run* run_fmm(const Config& cfg, size_t ns, const double* src, const doubl
                 double af[3] = {0, 0, 0}, an[3] = {0, 0, 0};
                 for (int c = 0; c < 3; c++)
                     for (int i = 0; i < L; i++) af[c] += R->Lc[b][c * L + i] * Rh[i];
-                if (cfg.flags & P2P) {
+                if ((cfg.flags & P2P) && cfg.problem == 1) {
+                    for (int j : near_pts)
+                        bem_kernel(x, S.P + 3 * (size_t)j, R->bem_d[j], R->bem_v.data() + 3 * (size_t)j, an);
+                } else if (cfg.flags & P2P) {
                     for (int j : near_pts) a_kernel(x, S.P + 3 * (size_t)j, R->q.data() + 3 * (size_t)j, an);
                 } else {
                     auto it = close_of.find(b);
                     if (it != close_of.end())
                         for (int s : it->second) m2p_A(*R, S, s, x, an);
                 }
-                const double sa = cfg.current / (4.0 * PI);
+                // calcBEM (fast_multipole_solver.cpp:233-250) divides by 4 pi only
+                const double cur = cfg.problem == 1 ? 1.0 : cfg.current;
+                const double sa = cur / (4.0 * PI);
                 for (int c = 0; c < 3; c++) {
                     R->Afar[3 * k + c] = af[c] * sa;
                     R->Anear[3 * k + c] = an[c] * sa;
-                    R->A[3 * k + c] = (af[c] + an[c]) / (4.0 * PI) * cfg.current;
+                    R->A[3 * k + c] = (af[c] + an[c]) / (4.0 * PI) * cur;
                 }
             }
         }
@@ -1006,6 +1069,8 @@ struct orc_config {
     int p, cap_s, cap_t, flags, target_root_mode, max_depth;
     double target_root_c[3], target_root_h[3];
     double mac_scale, current, fd_eps;
+    int problem;
+    double source_root_c[3], source_root_h[3];
 };
 void* orc_fmm_run(const orc_config* c, size_t ns, const double* src, const double* w, const double* J, size_t nt,
                   const double* tgt) {
@@ -1020,7 +1085,46 @@ void* orc_fmm_run(const orc_config* c, size_t ns, const double* src, const doubl
     cfg.mac_scale = c->mac_scale;
     cfg.current = c->current;
     cfg.fd_eps = c->fd_eps > 0 ? c->fd_eps : FD_EPS;
+    cfg.problem = c->problem;
+    for (int a = 0; a < 3; a++) { cfg.source_root_c[a] = c->source_root_c[a]; cfg.source_root_h[a] = c->source_root_h[a]; }
     return run_fmm(cfg, ns, src, w, J, nt, tgt);
+}
+// Problem::BEM: sources with B and normal vectors (BEMQuadrature)
+void* orc_fmm_run_bem(const orc_config* c, size_t ns, const double* src, const double* w, const double* Bv,
+                      const double* nrm, size_t nt, const double* tgt) {
+    Config cfg;
+    cfg.p = c->p;
+    cfg.cap_s = c->cap_s;
+    cfg.cap_t = c->cap_t;
+    cfg.flags = c->flags;
+    cfg.target_root_mode = c->target_root_mode;
+    cfg.max_depth = c->max_depth;
+    for (int a = 0; a < 3; a++) { cfg.target_root_c[a] = c->target_root_c[a]; cfg.target_root_h[a] = c->target_root_h[a]; }
+    cfg.mac_scale = c->mac_scale;
+    cfg.current = 1.0;
+    cfg.fd_eps = c->fd_eps > 0 ? c->fd_eps : FD_EPS;
+    cfg.problem = 1;
+    for (int a = 0; a < 3; a++) { cfg.source_root_c[a] = c->source_root_c[a]; cfg.source_root_h[a] = c->source_root_h[a]; }
+    return run_fmm(cfg, ns, src, w, nullptr, nt, tgt, Bv, nrm);
+}
+// math::calcBEMIntegral (integration.cpp:204-215): sum w BEMFunction / (4 pi)
+void orc_direct_bem(size_t ns, const double* src, const double* w, const double* Bv, const double* nrm, size_t nt,
+                    const double* tgt, double* out) {
+    std::vector<double> d(ns), v(3 * ns);
+    for (size_t i = 0; i < ns; i++) {
+        const double* b = Bv + 3 * i;
+        const double* n = nrm + 3 * i;
+        d[i] = w[i] * (b[0] * n[0] + b[1] * n[1] + b[2] * n[2]);
+        v[3 * i] = w[i] * (b[1] * n[2] - b[2] * n[1]);
+        v[3 * i + 1] = w[i] * (b[2] * n[0] - b[0] * n[2]);
+        v[3 * i + 2] = w[i] * (b[0] * n[1] - b[1] * n[0]);
+    }
+#pragma omp parallel for schedule(static)
+    for (long long t = 0; t < (long long)nt; t++) {
+        double acc[3] = {0, 0, 0};
+        for (size_t i = 0; i < ns; i++) bem_kernel(tgt + 3 * t, src + 3 * i, d[i], v.data() + 3 * i, acc);
+        for (int c = 0; c < 3; c++) out[3 * t + c] = acc[c] / (4 * PI);
+    }
 }
 void orc_fmm_free(void* h) { delete (Run*)h; }
 int orc_fmm_num_slots(void* h) { return (int)((Run*)h)->T->slots.size(); }

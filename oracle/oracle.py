@@ -45,8 +45,10 @@ def lib():
         L.orc_tree_build.restype = ctypes.c_void_p
         L.orc_lists_build.restype = ctypes.c_void_p
         L.orc_fmm_run.restype = ctypes.c_void_p
+        L.orc_fmm_run_bem.restype = ctypes.c_void_p
         for name in ("orc_tree_free", "orc_tree_num_nodes", "orc_tree_num_slots", "orc_tree_get",
                      "orc_lists_free", "orc_lists_counts", "orc_lists_get", "orc_fmm_free", "orc_fmm_get_A",
+                     "orc_direct_bem",
                      "orc_treecode",
                      "orc_fmm_num_slots", "orc_fmm_get", "orc_fmm_times", "orc_fmm_counts",
                      "orc_fmm_local", "orc_fmm_multipole"):
@@ -206,14 +208,18 @@ class _Cfg(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("cap_s", ctypes.c_int), ("cap_t", ctypes.c_int), ("flags", ctypes.c_int),
                 ("target_root_mode", ctypes.c_int), ("max_depth", ctypes.c_int),
                 ("target_root_c", ctypes.c_double * 3), ("target_root_h", ctypes.c_double * 3),
-                ("mac_scale", ctypes.c_double), ("current", ctypes.c_double), ("fd_eps", ctypes.c_double)]
+                ("mac_scale", ctypes.c_double), ("current", ctypes.c_double), ("fd_eps", ctypes.c_double),
+                ("problem", ctypes.c_int), ("source_root_c", ctypes.c_double * 3),
+                ("source_root_h", ctypes.c_double * 3)]
 
 
 class FMMRun:
     """One reference FMM evaluation (FastMultipoleSolver restated)."""
 
     def __init__(self, src, w, tgt, p=10, cap_s=32, cap_t=32, flags=FAITHFUL, J=None, target_root=None,
-                 mac_scale=1.0, current=1.0, max_depth=21, fd_eps=1e-3):
+                 mac_scale=1.0, current=1.0, max_depth=21, fd_eps=1e-3, bem=None, source_root=None):
+        """bem = (B, normal), each (N, 3): Problem::BEM with BEMQuadrature sources; the source root
+        is then Box((0,0,0), (1.1, 1.1, 1.6)) (multipole_solver.cpp:668-669) unless given."""
         src = np.ascontiguousarray(src, dtype=np.float64)
         tgt = np.ascontiguousarray(tgt, dtype=np.float64)
         w = np.ascontiguousarray(w, dtype=np.float64)
@@ -230,14 +236,27 @@ class FMMRun:
         c.mac_scale = mac_scale
         c.current = current
         c.fd_eps = fd_eps
+        c.problem = 0 if bem is None else 1
+        if source_root is None:
+            source_root = (np.zeros(3), np.array([1.1, 1.1, 1.6]) if bem is not None else np.full(3, 3.0))
+        for a in range(3):
+            c.source_root_c[a] = source_root[0][a]
+            c.source_root_h[a] = source_root[1][a]
         Jp = None
         if J is not None:
             J = np.ascontiguousarray(J, dtype=np.float64)
             Jp = _d(J)
         L = lib()
         self.p = p
-        self.h = ctypes.c_void_p(L.orc_fmm_run(ctypes.byref(c), ctypes.c_size_t(len(src)), _d(src), _d(w), Jp,
-                                               ctypes.c_size_t(len(tgt)), _d(tgt)))
+        if bem is None:
+            self.h = ctypes.c_void_p(L.orc_fmm_run(ctypes.byref(c), ctypes.c_size_t(len(src)), _d(src), _d(w), Jp,
+                                                   ctypes.c_size_t(len(tgt)), _d(tgt)))
+        else:
+            Bv = np.ascontiguousarray(bem[0], dtype=np.float64)
+            nv = np.ascontiguousarray(bem[1], dtype=np.float64)
+            self._bem = (Bv, nv)
+            self.h = ctypes.c_void_p(L.orc_fmm_run_bem(ctypes.byref(c), ctypes.c_size_t(len(src)), _d(src), _d(w),
+                                                       _d(Bv), _d(nv), ctypes.c_size_t(len(tgt)), _d(tgt)))
         n = L.orc_fmm_num_slots(self.h)
         self.slots = np.zeros(n, np.int32)
         self.B = np.zeros((n, 3))
@@ -317,6 +336,19 @@ def direct_A(src, w, tgt, J=None, current=1.0):
     lib().orc_direct_A(ctypes.c_size_t(len(src)), _d(src), _d(w), Jp, ctypes.c_size_t(len(tgt)), _d(tgt),
                        ctypes.c_double(current), _d(A))
     return A
+
+
+def direct_bem(src, w, Bv, nrm, tgt):
+    """math::calcBEMIntegral at each target (integration.cpp:191-215)."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    tgt = np.ascontiguousarray(tgt, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    Bv = np.ascontiguousarray(Bv, dtype=np.float64)
+    nrm = np.ascontiguousarray(nrm, dtype=np.float64)
+    out = np.zeros((len(tgt), 3))
+    lib().orc_direct_bem(ctypes.c_size_t(len(src)), _d(src), _d(w), _d(Bv), _d(nrm), ctypes.c_size_t(len(tgt)),
+                         _d(tgt), _d(out))
+    return out
 
 
 def direct_grad_w(src, w, tgt, Bstar, J=None, current=1.0):
