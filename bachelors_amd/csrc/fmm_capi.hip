@@ -139,7 +139,7 @@ struct fmm_ctx {
     PairIndex m2l_ix;                            // far pairs by target node
     // F16X3: per-coefficient scales dm, the M2L tables split in FP16 (their FP32 staging), the
     // multipole column scales; down_l2l = L2L operators ahead of the M2L ones in the down tables
-    DevBuf dm, m2l_hi, m2l_lo, m2l_rinv, m2l_work, cscale;
+    DevBuf dm, m2l_hi, m2l_lo, m2l_rinv, m2l_work, cscale, Mh, Ml;   // Mh / Ml: split multipoles
     int down_l2l = 0;
     const OpSpec* down_specs = nullptr;          // device specs of the down tables
     DevBuf m2l_slab;                             // M2L result per (pair, component)
@@ -469,16 +469,12 @@ void stage_upward(fmm_ctx* c, int alg = FMM_M2M_MATRICES) {
     int nlev = S.num_levels;
     OpTables& O = c->up_ops;
     O.nops = 8 * std::max(nlev - 1, 0);
-    std::vector<OpSpec> specs(O.nops);
+    // operator per (level, orientation) from the first node's own offset (the reference's
+    // choice); the nominal offsets (child - parent = +-half) for orientations a level lacks
+    std::vector<double> nominal(3 * (size_t)std::max(O.nops, 1));
     for (int k = 1; k < nlev; k++)
-        for (int o = 0; o < 8; o++) {
-            OpSpec& s = specs[(k - 1) * 8 + o];
-            for (int a = 0; a < 3; a++) s.t[a] = SIGN[o][a] * S.half[k][a];   // child - parent
-            s.sig_in = S.scale[k];
-            s.sig_out = S.scale[k - 1];
-            s.kind = OPK_M2M;
-            s.m2l_full = 0;
-        }
+        for (int o = 0; o < 8; o++)
+            for (int a = 0; a < 3; a++) nominal[((k - 1) * 8 + o) * 3 + a] = SIGN[o][a] * S.half[k][a];
     // per level: many parents -> parent-major tiles, each parent column summing its 8 children
     // (one K segment per orientation operator; empty children hold zero multipoles), written
     // once without atomics; few parents -> the 8 orientations as parallel classes of
@@ -486,8 +482,11 @@ void stage_upward(fmm_ctx* c, int alg = FMM_M2M_MATRICES) {
     if (O.nops) {   // built for every algorithm: the adjoint applies them transposed
         Staging& sp = c->up_specs;
         sp.begin();
-        const size_t o = sp.put(specs);
-        build_tables(c, O, (const OpSpec*)(sp.send(c->stream) + o), c->stream);
+        const size_t o = sp.put(nominal);
+        const double* d_nominal = (const double*)(sp.send(c->stream) + o);
+        OpSpec* d_specs = O.specs.get<OpSpec>(O.nops);
+        make_m2m_specs(S, c->sigS, d_nominal, d_specs, c->stream);
+        build_tables(c, O, d_specs, c->stream);
     }
     if (alg != FMM_M2M_MATRICES) {
         upward_without_matrices(c, alg);
@@ -699,9 +698,10 @@ void translate_gemm(fmm_ctx* c) {
     const OpTables& O = c->down_ops;
     const Pass& P = c->m2l_pass;
     const bool f16 = c->prec == Precision::F16X3;
-    if (f16 && P.ntiles) {   // multipole column scales (the B side of the FP16 split)
-        const int ncols = 3 * c->S.num_nodes;
-        column_scales(c->M.as<float>(), ncols, c->L, c->Lp, c->dm.as<float>(), c->cscale.get<float>(ncols), st);
+    if (f16 && P.ntiles) {   // the B side of the FP16 split: column scales and hi / lo parts
+        const int ncols = 3 * c->S.num_nodes, kp = m2l_f16_kpad(c->L);
+        split_columns(c->M.as<float>(), ncols, c->L, c->Lp, kp, c->dm.as<float>(), c->cscale.get<float>(ncols),
+                      c->Mh.get<_Float16>((size_t)ncols * kp), c->Ml.get<_Float16>((size_t)ncols * kp), st);
     }
     c->rec(E_GEMM0);
     if (P.ntiles) {
@@ -709,8 +709,8 @@ void translate_gemm(fmm_ctx* c) {
         void* slab = c->m2l_slab.reserve((size_t)3 * P.npairs * R * c->esize());
         if (f16)
             m2l_gemm_f16x3(c->m2l_hi.as<_Float16>(), c->m2l_lo.as<_Float16>(), c->m2l_rinv.as<float>(), c->down_l2l,
-                           P.tiles, P.ntiles, m2l_tile_cols(c), c->lists.far_s.as<int>(), c->M.as<float>(), c->dm.as<float>(),
-                           c->cscale.as<float>(), (float*)slab, c->L, c->Lp, st);
+                           P.tiles, P.ntiles, m2l_tile_cols(c), c->lists.far_s.as<int>(), c->Mh.as<_Float16>(),
+                           c->Ml.as<_Float16>(), c->cscale.as<float>(), (float*)slab, c->L, st);
         else
             m2l_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, c->lists.far_s.as<int>(), c->M.ptr, slab,
                      c->L, c->Lp, st);

@@ -53,97 +53,86 @@ __global__ void k_gather_xyz(const int* __restrict__ slots, int n, const double*
 }
 
 // ---------------------------------------------------------------- P2M
-// One wave per useful source leaf.  Lane (m, g) owns column m of the harmonic table for
-// the points g, g+G, ... (G = 64 / (P+1)); its 3 x 2 x (P+1) partial sums stay in registers
-// (the order is a template parameter so they index statically) and the G groups are added
-// through LDS at the end.  M_l^{+-m} = sum_j q_j R_l^{+-m}(u_j), u = (y - c) / sigma, with
-// R(l, 0) = (-1)^l Phi_l^0, R(l, +-m) = (-1)^l sqrt2 {Re, Im} Phi_l^m (harmonics.cpp:281-320).
-template <class T, int P>
-__global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, int nleaves,
-                                            const double* __restrict__ center, const int* __restrict__ begin,
-                                            const int* __restrict__ end, const int* __restrict__ level, ScaleTable sig,
-                                            const double* __restrict__ sxyz, const double* __restrict__ sq, int Lp,
-                                            T* __restrict__ M) {
-    constexpr int P1 = P + 1, G = 64 / P1, L = P1 * P1;
-    __shared__ double part[G][3][L];
-    const int lane = threadIdx.x, m = lane % P1, g = lane / P1;
-  // a grid smaller than the leaf count: the wave strides over leaves
-  for (int il = blockIdx.x; il < nleaves; il += gridDim.x) {
-    const int leaf = leaves[il];
-    double ar[3][P1], ai[3][P1];
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int l = 0; l < P1; l++) ar[c][l] = ai[c][l] = 0.0;
+// M_l^{+-m} = sum_j q_j R_l^{+-m}(u_j), u = (y - c) / sigma, with R(l, 0) = (-1)^l Phi_l^0 and
+// R(l, +-m) = (-1)^l sqrt2 {Re, Im} Phi_l^m (harmonics.cpp:281-320).  One workgroup per useful
+// source leaf; the leaf's points go through LDS in chunks of P2M_CHUNK.  Phase 1: thread task
+// (point, m) runs column m of the recurrences -- the sectoral chain Phi_m^m = (x + iy)^m /
+// (2^m m!) then Phi_l^m = ((2l - 1) z Phi_{l-1}^m - r^2 Phi_{l-2}^m) / (l^2 - m^2) up to l = p, p + 1
+// steps for every m -- and writes the real harmonics of the point to its LDS row.  Phase 2: thread
+// k owns coefficients k, k + 128, ... and adds q_j R_k(u_j) over the chunk's points for the three
+// components (FP64 sums in point order).  Both phases give every thread the same amount of work.
+constexpr int P2M_CHUNK = 16;
+template <class T>
+__global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, const double* __restrict__ center,
+                                             const int* __restrict__ begin, const int* __restrict__ end,
+                                             const int* __restrict__ level, ScaleTable sig,
+                                             const double* __restrict__ sxyz, const double* __restrict__ sq, int p,
+                                             int Lp, T* __restrict__ M) {
+    extern __shared__ double p2m_lds[];   // [P2M_CHUNK][L] real harmonics, then [P2M_CHUNK][3] charges
+    constexpr int RK = (MAX_ORDER + 1) * (MAX_ORDER + 1) / 128 + 1;
+    const int L = (p + 1) * (p + 1), P1 = p + 1, tid = threadIdx.x;
+    double* tab = p2m_lds;
+    double* qs = p2m_lds + P2M_CHUNK * L;
+    const int leaf = leaves[blockIdx.x];
     const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
     const double inv = 1.0 / sig.s[level[leaf]];
-    if (g < G) {
-        // two points per pass (j and j + G; a missing second point has zero charge): their
-        // recurrence chains are independent, so the two hide each other's latency; the sums
-        // still add the points in the order j, j + G, j + 2G, ...
-        const int jb = begin[leaf], je = end[leaf];
-        for (int j = jb + g; j < je; j += 2 * G) {
-            const bool two = j + G < je;
-            const int j1 = two ? j + G : j;
-            double ux = (sxyz[3 * j] - cx) * inv, uy = (sxyz[3 * j + 1] - cy) * inv, uz = (sxyz[3 * j + 2] - cz) * inv;
-            double wx = (sxyz[3 * j1] - cx) * inv, wy = (sxyz[3 * j1 + 1] - cy) * inv, wz = (sxyz[3 * j1 + 2] - cz) * inv;
-            double r2 = ux * ux + uy * uy + uz * uz, s2 = wx * wx + wy * wy + wz * wz;
-            double q0 = sq[3 * j], q1 = sq[3 * j + 1], q2 = sq[3 * j + 2];
-            double t0 = two ? sq[3 * j1] : 0.0, t1 = two ? sq[3 * j1 + 1] : 0.0, t2 = two ? sq[3 * j1 + 2] : 0.0;
-            double dr = 1.0, di = 0.0, er = 1.0, ei = 0.0;   // Phi_m^m = (x + iy)^m / (2^m m!)
-            for (int i = 1; i <= m; i++) {
-                double s = 0.5 / i;
-                double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
-                double mr = (er * wx - ei * wy) * s, mi = (er * wy + ei * wx) * s;
-                dr = nr; di = ni; er = mr; ei = mi;
-            }
-            double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
-            double o1r = er, o1i = ei, o2r = 0.0, o2i = 0.0;
+    double acc[RK][3];
 #pragma unroll
-            for (int l = 0; l < P1; l++) {
-                if (l < m) continue;
-                double vr = dr, vi = di, yr = er, yi = ei;
-                if (l > m) {   // Phi_l^m = ((2l-1) z Phi_{l-1}^m - r^2 Phi_{l-2}^m) / (l^2 - m^2)
-                    double f = 1.0 / (double)(l * l - m * m);
+    for (int r = 0; r < RK; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
+    const int jb = begin[leaf], je = end[leaf];
+    for (int j0 = jb; j0 < je; j0 += P2M_CHUNK) {
+        const int nj = min(P2M_CHUNK, je - j0);
+        __syncthreads();   // the previous chunk's rows are no longer read
+        for (int t = tid; t < nj * P1; t += blockDim.x) {
+            const int pt = t / P1, m = t - pt * P1, j = j0 + pt;
+            const double ux = (sxyz[3 * j] - cx) * inv, uy = (sxyz[3 * j + 1] - cy) * inv,
+                         uz = (sxyz[3 * j + 2] - cz) * inv;
+            const double r2 = ux * ux + uy * uy + uz * uz;
+            double dr = 1.0, di = 0.0;
+            for (int i = 1; i <= m; i++) {
+                const double s = recip(2 * i);
+                const double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
+                dr = nr;
+                di = ni;
+            }
+            double* row = tab + pt * L;
+            double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
+            for (int l = m; l <= p; l++) {
+                double vr = dr, vi = di;
+                if (l > m) {
+                    const double f = recip(l * l - m * m);
                     vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
                     vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
-                    yr = ((2.0 * l - 1) * wz * o1r - s2 * o2r) * f;
-                    yi = ((2.0 * l - 1) * wz * o1i - s2 * o2i) * f;
                     p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
-                    o2r = o1r; o2i = o1i; o1r = yr; o1i = yi;
                 }
-                ar[0][l] += q0 * vr; ar[1][l] += q1 * vr; ar[2][l] += q2 * vr;
-                ai[0][l] += q0 * vi; ai[1][l] += q1 * vi; ai[2][l] += q2 * vi;
-                if (two) {
-                    ar[0][l] += t0 * yr; ar[1][l] += t1 * yr; ar[2][l] += t2 * yr;
-                    ai[0][l] += t0 * yi; ai[1][l] += t1 * yi; ai[2][l] += t2 * yi;
-                }
+                const double s = sgnl(l) * (m == 0 ? 1.0 : SQRT2);
+                row[lmi(l, m)] = s * vr;
+                if (m > 0) row[lmi(l, -m)] = s * vi;
             }
         }
-        // real basis: (-1)^l for m = 0, (-1)^l sqrt2 (Re, Im) for +-m
+        for (int t = tid; t < 3 * nj; t += blockDim.x) qs[t] = sq[3 * j0 + t];
+        __syncthreads();
 #pragma unroll
-        for (int l = 0; l < P1; l++) {
-            if (l < m) continue;
-            const double s = sgnl(l) * (m == 0 ? 1.0 : SQRT2);
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                part[g][c][lmi(l, m)] = s * ar[c][l];
-                if (m > 0) part[g][c][lmi(l, -m)] = s * ai[c][l];
+        for (int r = 0; r < RK; r++) {
+            const int k = tid + 128 * r;
+            if (k >= L) break;
+            for (int pt = 0; pt < nj; pt++) {
+                const double v = tab[pt * L + k];
+                acc[r][0] += qs[3 * pt] * v;
+                acc[r][1] += qs[3 * pt + 1] * v;
+                acc[r][2] += qs[3 * pt + 2] * v;
             }
         }
     }
-    __syncthreads();
     T* out = M + (size_t)leaf * 3 * Lp;
-    for (int e = lane; e < 3 * Lp; e += 64) {
-        int c = e / Lp, k = e - c * Lp;
-        double s = 0.0;
-        if (k < L)
 #pragma unroll
-            for (int gg = 0; gg < G; gg++) s += part[gg][c][k];
-        out[e] = (T)s;
+    for (int r = 0; r < RK; r++) {
+        const int k = tid + 128 * r;
+        if (k >= Lp) break;
+        for (int c = 0; c < 3; c++) out[c * Lp + k] = (T)(k < L ? acc[r][c] : 0.0);
     }
-    __syncthreads();   // part is rewritten for the next leaf
-  }
+    for (int k = tid + 128 * RK; k < Lp; k += 128)   // padding beyond what the loop above covers
+        for (int c = 0; c < 3; c++) out[c * Lp + k] = (T)0;
 }
 
 // ---------------------------------------------------------------- L2P
@@ -744,34 +733,18 @@ void launch_charges(const int* slots, int n, const double* xyz, const double* w,
 void launch_gather_xyz(const int* slots, int n, const double* xyz, double* out, hipStream_t st) {
     if (n) k_gather_xyz<<<blocks(n), 256, 0, st>>>(slots, n, xyz, out);
 }
-template <class T, int P>
-static void p2m_order(const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
-                      const double* sq, int Lp, T* M, hipStream_t st) {
-    // at most 8 waves per CU, each striding over leaves: the P2M waves hold most of a SIMD's
-    // registers, and a capped grid leaves the rest of the GPU to the interaction lists that run
-    // beside it (measured: the lists' far part 1.59 -> 1.49 ms, the P2M 0.60 -> 0.73 ms)
-    const int grid = std::min(nleaves, 8 * num_cus());
-    k_p2m<T, P><<<grid, 64, 0, st>>>(leaves, nleaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
-                                     S.level.as<int>(), sig, sxyz, sq, Lp, M);
-}
-template <class T>
-static void p2m_dispatch(int p, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig,
-                         const double* sxyz, const double* sq, int Lp, T* M, hipStream_t st) {
-#define FMM_P2M(P) \
-    case P: p2m_order<T, P>(leaves, nleaves, S, sig, sxyz, sq, Lp, M, st); break;
-    switch (p) {
-        FMM_P2M(0) FMM_P2M(1) FMM_P2M(2) FMM_P2M(3) FMM_P2M(4) FMM_P2M(5) FMM_P2M(6) FMM_P2M(7) FMM_P2M(8) FMM_P2M(9)
-        FMM_P2M(10) FMM_P2M(11) FMM_P2M(12) FMM_P2M(13) FMM_P2M(14) FMM_P2M(15) FMM_P2M(16) FMM_P2M(17) FMM_P2M(18)
-        FMM_P2M(19) FMM_P2M(20)
-        default: throw Error(4, "expansion order above 20");
-    }
-#undef FMM_P2M
-}
 void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
                 const double* sq, int p, int Lp, void* M, hipStream_t st) {
     if (!nleaves) return;
-    if (f64) p2m_dispatch<double>(p, leaves, nleaves, S, sig, sxyz, sq, Lp, (double*)M, st);
-    else p2m_dispatch<float>(p, leaves, nleaves, S, sig, sxyz, sq, Lp, (float*)M, st);
+    if (p < 0 || p > MAX_ORDER) throw Error(4, "expansion order above 20");
+    const int L = (p + 1) * (p + 1);
+    const size_t lds = (size_t)P2M_CHUNK * (L + 3) * sizeof(double);
+    if (f64)
+        k_p2m<double><<<nleaves, 128, lds, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
+                                                 S.level.as<int>(), sig, sxyz, sq, p, Lp, (double*)M);
+    else
+        k_p2m<float><<<nleaves, 128, lds, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
+                                                S.level.as<int>(), sig, sxyz, sq, p, Lp, (float*)M);
     FMM_HIP(hipGetLastError());
 }
 template <bool POT>

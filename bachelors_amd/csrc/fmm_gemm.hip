@@ -416,8 +416,8 @@ __global__ __launch_bounds__(256) void k_gemm_m2l(const T* __restrict__ ops, con
 // narrow range, so both sides are equilibrated by powers of 2 (exact):
 //   A_eq = diag(rs) T diag(1/dm)   (operator tables: row scale rs per operator, dm per input
 //                                   coefficient; the tables hold A_eq split, rinv = 1 / rs)
-//   B_eq = diag(dm) M diag(s)      (multipoles: dm per coefficient, s per column, split here
-//                                   while staging B from the FP32 multipoles)
+//   B_eq = diag(dm) M diag(s)      (multipoles: dm per coefficient, s per column, split once per
+//                                   evaluation by k_split_columns into FP16 hi and lo parts)
 // so T M = diag(rinv) (A_eq B_eq) diag(1/s), applied in the epilogue.  Rows R = 16 NB >= L,
 // K padded to Kp = 32 NKS >= L (zero operator columns and multipole rows beyond L).
 // Layout and schedule follow k_gemm_m2l: wave w owns the tile's columns [16w, 16w + 16) and all
@@ -430,9 +430,10 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
                                                         const float* __restrict__ ops_rinv,
                                                         const GemmTile* __restrict__ tiles,
                                                         const int* __restrict__ pair_src,
-                                                        const float* __restrict__ src_exp, const float* __restrict__ dm,
+                                                        const _Float16* __restrict__ src_hi,
+                                                        const _Float16* __restrict__ src_lo,
                                                         const float* __restrict__ cscale, float* __restrict__ slab,
-                                                        int Lp, int op_base) {
+                                                        int op_base) {
     constexpr int R = 16 * NB, Kp = 32 * NKS, BK = 32, S = BK + 8;   // LDS row stride: 80 bytes
     constexpr int NQ = (4 * R + 255) / 256;                          // A: 8-half vectors per thread and part
     constexpr int W = 64 * NC;                                       // tile columns (NC blocks of 16 per wave)
@@ -448,24 +449,23 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
     const int op = tile.op - op_base;   // tile.op numbers the operators of all down passes
     const _Float16* Ahg = ops_hi + (size_t)op * R * Kp;
     const _Float16* Alg = ops_lo + (size_t)op * R * Kp;
-    // B staging: columns bc + 64 c, k = bq .. bq + 7 of each K step
+    // B staging: columns bc + 64 c, k = bq .. bq + 7 of each K step, from the multipoles split
+    // beforehand (split_columns): equilibrated FP16 hi and lo parts, Kp per column
     const int bc = tid >> 2, bq = (tid & 3) * 8;
-    const float* bptr[NC];
-    float bs[NC];
+    const _Float16* bph[NC];
+    const _Float16* bpl[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-        bptr[c] = nullptr;
-        bs[c] = 0.0f;
+        bph[c] = bpl[c] = nullptr;
         if (bc + 64 * c < tile.ncol) {
             const int col = tile.col0 + bc + 64 * c, pair = col / 3, comp = col - 3 * pair;
-            const int sc = pair_src[pair] * 3 + comp;
-            bptr[c] = src_exp + (size_t)sc * Lp;
-            bs[c] = cscale[sc];
+            const size_t sc = (size_t)pair_src[pair] * 3 + comp;
+            bph[c] = src_hi + sc * Kp;
+            bpl[c] = src_lo + sc * Kp;
         }
     }
     half8 rah[NQ], ral[NQ];
-    float4 rb0[NC], rb1[NC];
-    float4 d0, d1;
+    half8 rbh[NC], rbl[NC];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int j = 0; j < NQ; j++) {
@@ -476,14 +476,12 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
                 ral[j] = *(const half8*)(Alg + o);
             }
         }
-        d0 = *(const float4*)(dm + k0 + bq);
-        d1 = *(const float4*)(dm + k0 + bq + 4);
 #pragma unroll
-        for (int c = 0; c < NC; c++)
-            if (bptr[c]) {
-                rb0[c] = *(const float4*)(bptr[c] + k0 + bq);
-                rb1[c] = *(const float4*)(bptr[c] + k0 + bq + 4);
-            }
+        for (int c = 0; c < NC; c++) {
+            const half8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+            rbh[c] = bph[c] ? *(const half8*)(bph[c] + k0 + bq) : z;
+            rbl[c] = bpl[c] ? *(const half8*)(bpl[c] + k0 + bq) : z;
+        }
     };
     auto lstore = [&]() {
 #pragma unroll
@@ -496,18 +494,8 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
         }
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const float v[8] = {rb0[c].x * d0.x, rb0[c].y * d0.y, rb0[c].z * d0.z, rb0[c].w * d0.w,
-                                rb1[c].x * d1.x, rb1[c].y * d1.y, rb1[c].z * d1.z, rb1[c].w * d1.w};
-            half8 h, l;
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const float x = bptr[c] ? v[i] * bs[c] : 0.0f;
-                const _Float16 hh = (_Float16)x;
-                h[i] = hh;
-                l[i] = (_Float16)(x - (float)hh);
-            }
-            *(half8*)&Bh[bc + 64 * c][bq] = h;
-            *(half8*)&Bl[bc + 64 * c][bq] = l;
+            *(half8*)&Bh[bc + 64 * c][bq] = rbh[c];
+            *(half8*)&Bl[bc + 64 * c][bq] = rbl[c];
         }
     };
     f32x4 acc[NB][NC];
@@ -560,24 +548,35 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
 }
 
 // per multipole column c (node * 3 + component): the power of 2 s_c that brings
-// max_i |M[c][i] dm[i]| into [0.5, 1) (1 for a zero column); a wave per column, the waves of a
-// capped grid striding over the columns
-__global__ __launch_bounds__(256) void k_column_scales(const float* __restrict__ M, int ncols, int L, int Lp,
-                                                       const float* __restrict__ dm, float* __restrict__ scale) {
+// max_i |M[c][i] dm[i]| into [0.5, 1) (1 for a zero column), and the column's equilibrated values
+// x_i = M[c][i] dm[i] s_c split into FP16 hi = fp16(x) and lo = fp16(x - hi), Kp per column (zero
+// beyond L): the B operand of the F16X3 M2L, split once per evaluation instead of once per tile.
+// A wave per column, the waves of a capped grid striding over the columns.
+__global__ __launch_bounds__(256) void k_split_columns(const float* __restrict__ M, int ncols, int L, int Lp, int Kp,
+                                                       const float* __restrict__ dm, float* __restrict__ scale,
+                                                       _Float16* __restrict__ hi, _Float16* __restrict__ lo) {
     const int lane = threadIdx.x & 63;
     for (int c = blockIdx.x * 4 + (threadIdx.x >> 6); c < ncols; c += gridDim.x * 4) {
         const float* col = M + (size_t)c * Lp;
         float mx = 0.0f;
         for (int i = lane; i < L; i += 64) mx = fmaxf(mx, fabsf(col[i] * dm[i]));
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-        if (lane == 0) scale[c] = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
+        const float s = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
+        if (lane == 0) scale[c] = s;
+        for (int i = lane; i < Kp; i += 64) {
+            const float x = i < L ? col[i] * dm[i] * s : 0.0f;
+            const _Float16 h = (_Float16)x;
+            hi[(size_t)c * Kp + i] = h;
+            lo[(size_t)c * Kp + i] = (_Float16)(x - (float)h);
+        }
     }
 }
 
-void column_scales(const float* M, int ncols, int L, int Lp, const float* dm, float* scale, hipStream_t st) {
+void split_columns(const float* M, int ncols, int L, int Lp, int Kp, const float* dm, float* scale, _Float16* hi,
+                   _Float16* lo, hipStream_t st) {
     if (ncols <= 0) return;
     const int grid = std::min((ncols + 3) / 4, 8 * num_cus());
-    k_column_scales<<<grid, 256, 0, st>>>(M, ncols, L, Lp, dm, scale);
+    k_split_columns<<<grid, 256, 0, st>>>(M, ncols, L, Lp, Kp, dm, scale, hi, lo);
     FMM_HIP(hipGetLastError());
 }
 
@@ -585,20 +584,19 @@ int m2l_f16_rows(int L) { return 16 * ((L + 15) / 16); }
 int m2l_f16_kpad(int L) { return 32 * ((L + 31) / 32); }
 
 void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv, int op_base,
-                    const GemmTile* d_tiles, int ntiles, int tile_cols, const int* pair_src, const float* src_exp,
-                    const float* dm, const float* cscale, float* slab, int L, int Lp, hipStream_t st) {
+                    const GemmTile* d_tiles, int ntiles, int tile_cols, const int* pair_src, const _Float16* src_hi,
+                    const _Float16* src_lo, const float* cscale, float* slab, int L, hipStream_t st) {
     if (ntiles == 0) return;
     const int nb = m2l_f16_rows(L) / 16, nks = m2l_f16_kpad(L) / 32;
-    if (m2l_f16_kpad(L) > Lp) throw Error(4, "FP16 M2L: K padding beyond the expansion length");
     if (tile_cols != 64 && tile_cols != 128) throw Error(4, "FP16 M2L: tiles of 64 or 128 columns");
 #define FMM_F16(NBV, NKV)                                                                                            \
     if (nb == NBV && nks == NKV) {                                                                                   \
         if (tile_cols == 128)                                                                                        \
             k_gemm_m2l_f16x3<NBV, NKV, 2><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src,       \
-                                                                  src_exp, dm, cscale, slab, Lp, op_base);           \
+                                                                  src_hi, src_lo, cscale, slab, op_base);            \
         else                                                                                                         \
             k_gemm_m2l_f16x3<NBV, NKV, 1><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src,       \
-                                                                  src_exp, dm, cscale, slab, Lp, op_base);           \
+                                                                  src_hi, src_lo, cscale, slab, op_base);            \
         FMM_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                      \
     }

@@ -157,15 +157,25 @@ __global__ __launch_bounds__(256) void k_lists_step(const int2* __restrict__ fro
 // Far-pair sort key: class code (level difference and centre offset, each axis in units of
 // the finer box's half size along that axis) above the target index (target_bits wide);
 // value = source.  hs / ht: per-level half sizes [level][3] of the source / target boxes.
-// Offsets that are not whole multiples of the half size (to 1e-9, relative), or beyond the key's
-// range, set the error flag instead of sharing an operator with another offset.
-__device__ __forceinline__ int lattice_offset(double d, double h, unsigned* err) {
+// An offset that is not a whole multiple of the half size (to 1e-9, relative) sets error bit 2
+// (such pairs cannot share an operator); offsets or level differences beyond the compact code
+// (|n| <= 127, |d| <= 31) set bit 1, and the classes are then keyed by the wide code below.
+constexpr unsigned ERR_RANGE = 1u, ERR_LATTICE = 2u;
+constexpr int WIDE_BITS = 19;   // per axis in the wide code: |n| < 2^18
+__device__ __forceinline__ long long lattice_offset(double d, double h, unsigned* err) {
     const double q = d / h, r = rint(q);
-    if (!(fabs(q - r) <= 1e-9 * fmax(1.0, fabs(q))) || r < -127.0 || r > 127.0) {
-        atomicOr(err, 1u);
+    if (!(fabs(q - r) <= 1e-9 * fmax(1.0, fabs(q))) || !(fabs(r) < (double)(1 << (WIDE_BITS - 1)))) {
+        atomicOr(err, ERR_LATTICE);
         return 0;
     }
-    return (int)r;
+    if (r < -127.0 || r > 127.0) atomicOr(err, ERR_RANGE);
+    return (long long)r;
+}
+// the wide class code: (d + 32) above three WIDE_BITS-bit offsets (63 bits)
+__device__ __forceinline__ uint64_t wide_code(int d, long long nx, long long ny, long long nz) {
+    const long long o = 1ll << (WIDE_BITS - 1);
+    return ((uint64_t)(d + 32) << (3 * WIDE_BITS)) | ((uint64_t)(nx + o) << (2 * WIDE_BITS)) |
+           ((uint64_t)(ny + o) << WIDE_BITS) | (uint64_t)(nz + o);
 }
 __global__ void k_far_keys(const int2* __restrict__ far, int n, const NodeRec* __restrict__ S,
                            const NodeRec* __restrict__ T, const double* __restrict__ hs, const double* __restrict__ ht,
@@ -179,16 +189,52 @@ __global__ void k_far_keys(const int2* __restrict__ far, int n, const NodeRec* _
         const NodeRec rs = S[s], rt = T[t];
         int ls = lvl(rs), lt = lvl(rt);
         // finer of the two boxes (physical size, not tree level), per axis
-        int nx = lattice_offset(rs.c[0] - rt.c[0], fmin(hs[3 * ls], ht[3 * lt]), err);
-        int ny = lattice_offset(rs.c[1] - rt.c[1], fmin(hs[3 * ls + 1], ht[3 * lt + 1]), err);
-        int nz = lattice_offset(rs.c[2] - rt.c[2], fmin(hs[3 * ls + 2], ht[3 * lt + 2]), err);
+        long long nx = lattice_offset(rs.c[0] - rt.c[0], fmin(hs[3 * ls], ht[3 * lt]), err);
+        long long ny = lattice_offset(rs.c[1] - rt.c[1], fmin(hs[3 * ls + 1], ht[3 * lt + 1]), err);
+        long long nz = lattice_offset(rs.c[2] - rt.c[2], fmin(hs[3 * ls + 2], ht[3 * lt + 2]), err);
         int d = ls - lt;
-        if (d < -31 || d > 31) atomicOr(err, 1u);
-        code = ((uint64_t)(d + 32) << 24) | ((uint64_t)(nx + 128) << 16) | ((uint64_t)(ny + 128) << 8) |
-               (uint64_t)(nz + 128);
+        if (d < -31 || d > 31) atomicOr(err, ERR_RANGE);
+        code = ((uint64_t)(d + 32) << 24) | ((uint64_t)((nx + 128) & 255) << 16) | ((uint64_t)((ny + 128) & 255) << 8) |
+               (uint64_t)((nz + 128) & 255);
     }
     key[i] = (code << target_bits) | (uint32_t)t;
     val[i] = s;
+}
+
+// the wide path: per far pair (unsorted, (t, s)) its wide class code and its target as a key
+__global__ void k_far_keys_wide(const int2* __restrict__ far, int n, const NodeRec* __restrict__ S,
+                                const NodeRec* __restrict__ T, const double* __restrict__ hs,
+                                const double* __restrict__ ht, uint64_t* __restrict__ code, uint32_t* __restrict__ tkey,
+                                int* __restrict__ idx, unsigned* __restrict__ err) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int t = far[i].x, s = far[i].y;
+    const NodeRec rs = S[s], rt = T[t];
+    const int ls = lvl(rs), lt = lvl(rt);
+    const long long nx = lattice_offset(rs.c[0] - rt.c[0], fmin(hs[3 * ls], ht[3 * lt]), err);
+    const long long ny = lattice_offset(rs.c[1] - rt.c[1], fmin(hs[3 * ls + 1], ht[3 * lt + 1]), err);
+    const long long nz = lattice_offset(rs.c[2] - rt.c[2], fmin(hs[3 * ls + 2], ht[3 * lt + 2]), err);
+    code[i] = wide_code(ls - lt, nx, ny, nz);
+    tkey[i] = (uint32_t)t;
+    idx[i] = i;
+}
+__global__ void k_gather_u64(const uint64_t* __restrict__ in, const int* __restrict__ idx, int n,
+                             uint64_t* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[idx[i]];
+}
+__global__ void k_gather_pairs(const int2* __restrict__ far, const int* __restrict__ idx, int n, int* __restrict__ ft,
+                               int* __restrict__ fs) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int2 p = far[idx[i]];
+    ft[i] = p.x;
+    fs[i] = p.y;
+}
+__global__ void k_seg_flags_u64(const uint64_t* __restrict__ code, int n, uint32_t* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || code[i] != code[i - 1]) ? 1u : 0u;
 }
 
 __global__ void k_close_keys(const int2* __restrict__ close, int n, uint64_t* __restrict__ key, int* __restrict__ val) {
@@ -374,7 +420,33 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
                 unsigned hc[2];
                 FMM_HIP(hipMemcpyAsync(hc, cc, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
                 FMM_HIP(hipStreamSynchronize(st));
-                if (hc[1]) throw Error(4, "interaction offset off the box lattice or out of the class-key range");
+                if (hc[1] & ERR_LATTICE)
+                    throw Error(4, "interaction offset off the box lattice (the shared frame needs lattice-aligned boxes)");
+                if (hc[1] & ERR_RANGE) {
+                    // offsets or level differences beyond the compact code (deep, strongly adaptive
+                    // trees): classes by the wide code, sorted in two stable passes (by target, then
+                    // by code), so the order is still (class, target)
+                    int* perm = L.perm.get<int>(nfar);
+                    int* perm2 = L.perm2.get<int>(nfar);
+                    uint64_t* wc = L.wkey.get<uint64_t>(nfar);
+                    uint64_t* wc2 = L.wkey2.get<uint64_t>(nfar);
+                    uint32_t* tk = L.tkey.get<uint32_t>(nfar);
+                    uint32_t* tk2 = L.tkey2.get<uint32_t>(nfar);
+                    int* idx = L.idx.get<int>(nfar);
+                    FMM_HIP(hipMemsetAsync(err, 0, sizeof(unsigned), st));
+                    k_far_keys_wide<<<blocks(nfar), 256, 0, st>>>(far, nfar, srec, trec, dh, dh + 3 * MAX_LEVELS, wc, tk,
+                                                                  idx, err);
+                    sort_pairs_u32(tk, tk2, idx, perm, nfar, tbits, L.tmp, st);
+                    k_gather_u64<<<blocks(nfar), 256, 0, st>>>(wc, perm, nfar, wc2);
+                    sort_pairs_u64(wc2, wc, perm, perm2, nfar, 3 * WIDE_BITS + 6, L.tmp, st);
+                    k_gather_pairs<<<blocks(nfar), 256, 0, st>>>(far, perm2, nfar, ft, fs);
+                    k_seg_flags_u64<<<blocks(nfar), 256, 0, st>>>(wc, nfar, flag);
+                    exclusive_scan_u32(flag, pos, nfar, L.tmp, st);
+                    k_class_count<<<1, 1, 0, st>>>(pos, flag, nfar, err, cc);
+                    FMM_HIP(hipMemcpyAsync(hc, cc, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+                    FMM_HIP(hipStreamSynchronize(st));
+                    if (hc[1] & ERR_LATTICE) throw Error(4, "interaction offset beyond the wide class code");
+                }
                 int nc = (int)hc[0];
                 int* starts = out.class_begin.get<int>(nc + 1);
                 k_seg_starts<<<blocks(nfar), 256, 0, st>>>(flag, pos, nfar, starts);

@@ -7,6 +7,7 @@ namespace fmm {
 
 struct ListScratch {
     DevBuf far, close, frontA, frontB, counters, key, val, key2, val2, flag, pos, tmp, hs, srec, trec;
+    DevBuf perm, perm2, wkey, wkey2, tkey, tkey2, idx;   // wide class codes (deep adaptive trees)
     unsigned far_cap = 0, close_cap = 0, front_cap = 0;
     unsigned h_seed[3] = {0, 0, 0};        // host staging of the root pair and its count
     double h_half[6 * MAX_LEVELS] = {0};   // host staging of the per-level half sizes [tree][level][axis]

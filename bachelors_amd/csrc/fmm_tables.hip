@@ -9,6 +9,7 @@
 // multiplied by sigma^(degree+1)), which makes every operator of the shared frame
 // independent of the level and keeps FP32/BF16 magnitudes uniform across degrees.
 #include <algorithm>
+#include <climits>
 
 #include "fmm_tables.hpp"
 #include "fmm_recip.hpp"
@@ -536,6 +537,61 @@ __global__ void k_pair_specs(const int* __restrict__ src, const int* __restrict_
     o.kind = kind;
     o.m2l_full = m2l_full;
     out[i] = o;
+}
+
+// M2M operator specs per (child level k, orientation o), as the reference builds them
+// (multipole_solver.cpp:472-534): from the first non-empty child of orientation o at level k (the
+// lowest node id, BFS order) and its parent, so the translation is the float-rounded offset of
+// that pair; an orientation with no such child keeps the nominal offset (no node applies it).
+// One workgroup per child level.
+struct LevelRanges {
+    int begin[MAX_LEVELS + 2];
+};
+__global__ __launch_bounds__(64) void k_m2m_specs(LevelRanges lv, int nlev, const int* __restrict__ first_child,
+                                                  const int* __restrict__ npts, const int* __restrict__ parent,
+                                                  const double* __restrict__ center, ScaleTable sig,
+                                                  const double* __restrict__ nominal, OpSpec* __restrict__ out) {
+    // One wave per child level.  Children are numbered breadth first, parent by parent, so the
+    // first parent (in level order) holding a non-empty child of orientation o holds the
+    // smallest such child: the wave walks the parents 64 at a time and stops once all eight
+    // orientations are found (usually in the first chunk).
+    const int k = blockIdx.x + 1, lane = threadIdx.x;
+    int best = INT_MAX;   // lane o < 8: the child of orientation o
+    unsigned found = 0;   // bit o: orientation o found (uniform over the wave)
+    for (int n0 = lv.begin[k - 1]; n0 < lv.begin[k] && found != 0xffu; n0 += 64) {
+        const int n = n0 + lane;
+        const int fc = n < lv.begin[k] ? first_child[n] : -1;
+        for (int o = 0; o < 8; o++) {
+            if ((found >> o) & 1u) continue;
+            const bool ok = fc >= 0 && (first_child[fc + o] >= 0 || npts[fc + o] > 0);
+            const unsigned long long b = __ballot(ok);
+            if (b == 0) continue;
+            const int ch = __shfl(fc + o, __ffsll(b) - 1);
+            if (lane == o) best = ch;
+            found |= 1u << o;
+        }
+    }
+    if (lane < 8) {
+        const int o = lane, ch = best;
+        OpSpec sp;
+        for (int a = 0; a < 3; a++)
+            sp.t[a] = ch == INT_MAX ? nominal[((k - 1) * 8 + o) * 3 + a]
+                                    : center[3 * ch + a] - center[3 * parent[ch] + a];   // child - parent
+        sp.sig_in = sig.s[k];
+        sp.sig_out = sig.s[k - 1];
+        sp.kind = OPK_M2M;
+        sp.m2l_full = 0;
+        out[(k - 1) * 8 + o] = sp;
+    }
+}
+
+void make_m2m_specs(const TreeDev& S, const ScaleTable& sig, const double* d_nominal, OpSpec* out, hipStream_t st) {
+    if (S.num_levels < 2) return;
+    LevelRanges lv;
+    for (int k = 0; k < MAX_LEVELS + 2; k++) lv.begin[k] = S.level_begin[k];
+    k_m2m_specs<<<S.num_levels - 1, 64, 0, st>>>(lv, S.num_levels, S.first_child.as<int>(), S.npts.as<int>(),
+                                                   S.parent.as<int>(), S.center.as<double>(), sig, d_nominal, out);
+    FMM_HIP(hipGetLastError());
 }
 
 void make_pair_specs(const int* src, const int* dst, int n, const double* csrc, const double* cdst, const int* lsrc,

@@ -44,6 +44,10 @@ void make_m2l_specs(const int* class_begin, int nc, const int* far_t, const int*
 // expansions, double if f64 else float)
 void m2m_per_node(bool f64, const int* parents, int nparents, const TreeDev& S, const ScaleTable& sig, int p, int Lp,
                   void* M, hipStream_t st);
+// the M2M operator specs per (child level, orientation), [(k - 1) * 8 + o], from the first non-empty
+// node of each (the reference's choice, multipole_solver.cpp:472-534); d_nominal [(k - 1) * 8 + o][3]
+// holds the offsets used where a level has no node of an orientation
+void make_m2m_specs(const TreeDev& S, const ScaleTable& sig, const double* d_nominal, OpSpec* out, hipStream_t st);
 void make_pair_specs(const int* src, const int* dst, int n, const double* csrc, const double* cdst, const int* lsrc,
                      const int* ldst, const ScaleTable& sig_src, const ScaleTable& sig_dst, int kind, int m2l_full,
                      OpSpec* out, hipStream_t st);

@@ -42,7 +42,8 @@ class _Apply(torch.autograd.Function):
 
 
 class FMMBiotSavart:
-    def __init__(self, src, tgt, J=None, order=12, cap=64, precision="fp64", current=1.0, w0=None):
+    def __init__(self, src, tgt, J=None, order=12, cap=64, precision="fp64", current=1.0, w0=None, **config):
+        """config: further fmm_config fields (max_depth, mac_scale, m2l_full, ...)."""
         dev = torch.device("cuda:0")
         self.src = torch.as_tensor(src, dtype=torch.float64, device=dev).contiguous()
         self.tgt = torch.as_tensor(tgt, dtype=torch.float64, device=dev).contiguous()
@@ -52,7 +53,7 @@ class FMMBiotSavart:
         w = torch.ones(self.ns, dtype=torch.float64, device=dev) if w0 is None else \
             torch.as_tensor(w0, dtype=torch.float64, device=dev).contiguous()
         self._fmm = fm.FMM(order=order, source_leaf_capacity=cap, target_leaf_capacity=cap, precision=precision,
-                           frame=fm.FRAME_SHARED)
+                           frame=fm.FRAME_SHARED, **config)
         self._fmm.set_sources_dev(self.ns, _vp(self.src), _vp(w), None if self.J is None else _vp(self.J))
         self._fmm.set_targets_dev(self.nt, _vp(self.tgt))
         self._fmm.build_trees()

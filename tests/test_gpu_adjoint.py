@@ -58,7 +58,19 @@ def test_adjoint_gradient_vs_finite_differences_and_direct():
             fd.append((lp - lm) / 2e-3)
     fd = np.array(fd)
     assert rel_l2(grad[idx], fd) < 1e-5
-    # against the FP64 direct-sum gradient (same loss with direct-sum B)
+    # the adjoint operator alone: the FMM residual r = B(w) - B* pulled back through the exact
+    # transposed kernel, dL/dw_j = 2 sum_i r_i . (MU0/4pi) (J_j x (x_i - y_j)) / |x_i - y_j|^3
+    with torch.no_grad():
+        r = (F(torch.tensor(w, device="cuda")) - Bstar).cpu().numpy()
+    J = np.stack([-src[:, 1], src[:, 0], np.zeros(len(src))], 1) / np.hypot(src[:, 0], src[:, 1])[:, None]
+    gk = np.zeros(len(src))
+    for j0 in range(0, len(src), 500):
+        d = tgt[None, :, :] - src[j0:j0 + 500, None, :]                        # (j, i, 3)
+        k = np.cross(np.broadcast_to(J[j0:j0 + 500, None, :], d.shape), d) / (np.linalg.norm(d, axis=2) ** 3)[..., None]
+        gk[j0:j0 + 500] = 2 * np.einsum("jic,ic->j", k, r) * (1.2566370614e-6 / (4 * 3.14159265359))
+    assert rel_l2(grad, gk) < 1e-5      # p = 10 truncation of the transposed FMM
+    # against the FP64 direct-sum gradient of the same loss (direct-sum B and B*): the FMM error of
+    # the residual (a tenth of B) enters here too
     Bs = O.direct_B(src, wstar, tgt)
     gd = O.direct_grad_w(src, w, tgt, Bs)
-    assert rel_l2(grad, gd) < 1e-3
+    assert rel_l2(grad, gd) < 1e-4
