@@ -30,6 +30,30 @@ using namespace fmm;
 
 namespace {
 
+// Device scratch of the allocation-free direct sum (fmm_direct_B_noalloc_dev): the partial sums
+// of its source ranges, FMM_NOALLOC_SCRATCH_DOUBLES doubles (32 MiB) per GPU, held until the
+// process ends.  Host code never passes this name to a kernel: the device address is looked up
+// once per device (scratch_on_current_device) and that pointer is used.
+__device__ __align__(256) double scratch[FMM_NOALLOC_SCRATCH_DOUBLES];
+
+std::mutex g_scratch_mutex;
+std::vector<double*> g_scratch_ptr;   // per device id
+
+// the device address of `scratch` on the current device (looked up on the first call per device)
+double* scratch_on_current_device() {
+    int dev = 0;
+    FMM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        FMM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw Error(FMM_ERR_HIP, "no device address for the scratch array");
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
 enum Stage { ST_EMPTY = 0, ST_TREES = 1, ST_LEAVES = 2, ST_UP = 3, ST_DOWN = 4, ST_EVAL = 5 };
 // M2M levels with at least this many parent columns use parent-major tiles
 constexpr int M2M_PARENT_MAJOR_COLUMNS = 64 * 256;
@@ -105,6 +129,7 @@ struct OpTables {           // FP64 master + the precision actually applied
 struct fmm_ctx {
     fmm_config cfg;
     std::string err;
+    double* scratch = nullptr;                   // this device's address of the no-alloc scratch array
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int p = 10, L = 121, Lp = 128;
@@ -1077,6 +1102,7 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
         if (ndev < 1) throw Error(FMM_ERR_HIP, "no HIP device");
         FMM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->own_stream = true;
+        c->scratch = scratch_on_current_device();
         for (int i = 0; i < E_COUNT; i++) FMM_HIP(hipEventCreate(&c->ev[i]));
         c->ev_ok = true;
         FMM_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));   // default priority
@@ -1330,6 +1356,24 @@ int fmm_direct_B_dev(fmm_ctx* c, double current, size_t m, const double* xyz, do
 int fmm_direct_A_dev(fmm_ctx* c, double current, size_t m, const double* xyz, double* A) {
     return direct_entry(c, true, current, m, xyz, A);
 }
+// fmm_direct_B_dev without allocating: the partial sums go to the device scratch array (as many
+// source ranges as fit), nothing is allocated or freed
+int fmm_direct_B_noalloc_dev(fmm_ctx* c, double current, size_t m, const double* xyz, double* B) {
+    return guard(c, [&] {
+        require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built (sources are gathered with them)");
+        require(m <= FMM_NOALLOC_MAX_TARGETS, FMM_ERR_LIMIT,
+                "fmm_direct_B_noalloc_dev: more targets than FMM_NOALLOC_MAX_TARGETS (65536)");
+        require(m == 0 || (xyz && B), FMM_ERR_ARG, "null target or output array");
+        require(c->scratch != nullptr, FMM_ERR_STATE, "scratch array not set up");
+        if (!m) return;
+        join_default_stream(c);
+        const int cap = (int)(FMM_NOALLOC_SCRATCH_DOUBLES / (3 * m));
+        const int nsplit = std::max(1, std::min(direct_splits(c->S.num_slots, (int)m), cap));
+        direct_sum(c, false, current, m, xyz, B, c->scratch, nsplit);
+        FMM_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
 int fmm_evaluate_A_dev(fmm_ctx* c, double current, double* A) {
     return guard(c, [&] {
         require(A != nullptr || c->nt == 0, FMM_ERR_ARG, "null output");

@@ -175,6 +175,15 @@ int fmm_adjoint_weights_dev(fmm_ctx* ctx, double current, const double* dLdB, do
 int fmm_direct_B_dev(fmm_ctx* ctx, double current, size_t m, const double* xyz, double* B);
 int fmm_direct_A_dev(fmm_ctx* ctx, double current, size_t m, const double* xyz, double* A);
 
+/* fmm_direct_B_dev that allocates nothing (no reference counterpart: for callers that cannot
+ * allocate device memory, e.g. code under HIP graph capture or tight serving loops).  The partial
+ * sums use a fixed device scratch array of FMM_NOALLOC_SCRATCH_DOUBLES doubles (32 MiB) per GPU,
+ * held until the process ends; m <= FMM_NOALLOC_MAX_TARGETS, larger m returns FMM_ERR_LIMIT.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (other threads or
+ * contexts, other streams) race and must be serialised by the caller. */
+#define FMM_NOALLOC_MAX_TARGETS 65536
+#define FMM_NOALLOC_SCRATCH_DOUBLES ((size_t)4 << 20)
+int fmm_direct_B_noalloc_dev(fmm_ctx* ctx, double current, size_t m, const double* xyz, double* B);
 
 int fmm_get_stats(fmm_ctx* ctx, fmm_stats* stats);
 
