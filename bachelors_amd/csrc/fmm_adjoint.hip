@@ -252,7 +252,7 @@ void run_adjoint(AdjointState& S, const AdjointArgs& a, const double* dLdB, doub
         const PassView& v = a.l2l_levels[k];
         if (v.ntiles)
             class_gemm(a.prec, a.down_ops, a.down_ops_lo, v.tiles, v.ntiles, v.dst, v.src, lam, lam, a.Lp, EPI_ATOMIC,
-                       st, true);
+                       st, true, 1, TRI_LOWER);
     }
     // M2L^T (forward far_s -> far_t; here far_t -> far_s): one slab per (pair, component), then
     // each source node sums its pairs' slabs in pair order
@@ -268,7 +268,8 @@ void run_adjoint(AdjointState& S, const AdjointArgs& a, const double* dLdB, doub
     for (int k = 1; k < Ts.num_levels; k++) {
         const PassView& v = a.m2m_levels[k];
         if (v.ntiles)
-            class_gemm(a.prec, a.up_ops, a.up_ops_lo, v.tiles, v.ntiles, v.dst, v.src, mu, mu, a.Lp, EPI_ADD, st, true);
+            class_gemm(a.prec, a.up_ops, a.up_ops_lo, v.tiles, v.ntiles, v.dst, v.src, mu, mu, a.Lp, EPI_ADD, st, true, 1,
+                       TRI_UPPER);
     }
     // P2M^T
     double* gq = S.gq.get<double>(3 * (size_t)std::max(nss, 1));

@@ -316,10 +316,11 @@ const void* ops_ptr(const fmm_ctx* c, const OpTables& O) {
     return O.hi.ptr;
 }
 
-void run_pass(fmm_ctx* c, const OpTables& O, const Pass& P, const void* src_exp, void* dst_exp, int epilogue) {
+void run_pass(fmm_ctx* c, const OpTables& O, const Pass& P, const void* src_exp, void* dst_exp, int epilogue,
+              int tri) {
     if (!P.ntiles) return;
     class_gemm(c->pass_prec(), ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, P.src, P.dst, src_exp, dst_exp, c->Lp,
-               epilogue, c->stream);
+               epilogue, c->stream, false, 1, tri);
 }
 
 // Roots of both trees (tc, th: target root) and the start mark of an evaluation.
@@ -555,9 +556,9 @@ void stage_upward(fmm_ctx* c, int alg = FMM_M2M_MATRICES) {
         if (!P.ntiles) continue;
         if (parent_major[k])
             class_gemm(c->pass_prec(), ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, P.src, P.dst, c->M.ptr, c->M.ptr, c->Lp,
-                       EPI_ADD, c->stream, false, 8);
+                       EPI_ADD, c->stream, false, 8, TRI_LOWER);
         else
-            run_pass(c, O, P, c->M.ptr, c->M.ptr, EPI_ATOMIC);
+            run_pass(c, O, P, c->M.ptr, c->M.ptr, EPI_ATOMIC, TRI_LOWER);
     }
     c->rec(E_M2M);
     c->stage = ST_UP;
@@ -587,7 +588,7 @@ void run_per_pair(fmm_ctx* c, int kind, const int* d_src, const int* d_dst, int 
         GemmTile* dt = c->pp_tiles.get<GemmTile>(n);
         FMM_HIP(hipMemcpyAsync(dt, tiles.data(), n * sizeof(GemmTile), hipMemcpyHostToDevice, st));
         class_gemm(c->pass_prec(), ops_ptr(c, O), O.lo.ptr, dt, n, d_src, d_dst, src_exp, dst_exp, c->Lp,
-                   atomic ? EPI_ATOMIC : EPI_ADD, st);
+                   atomic ? EPI_ATOMIC : EPI_ADD, st, false, 1, kind == OPK_L2L ? TRI_UPPER : TRI_NONE);
         FMM_HIP(hipStreamSynchronize(st));   // the chunk buffers are reused
     }
 }
@@ -781,7 +782,7 @@ void translate_finish(fmm_ctx* c) {
     const char* dev = sg.send(st);
     for (int k = 1; k < nlev; k++) {
         resolve_pass(c->l2l_pass[k], dev);
-        run_pass(c, O, c->l2l_pass[k], Lx, Lx, EPI_ADD);
+        run_pass(c, O, c->l2l_pass[k], Lx, Lx, EPI_ADD, TRI_UPPER);
     }
     c->rec(E_L2L);
     c->stage = ST_DOWN;

@@ -54,85 +54,106 @@ __global__ void k_gather_xyz(const int* __restrict__ slots, int n, const double*
 
 // ---------------------------------------------------------------- P2M
 // M_l^{+-m} = sum_j q_j R_l^{+-m}(u_j), u = (y - c) / sigma, with R(l, 0) = (-1)^l Phi_l^0 and
-// R(l, +-m) = (-1)^l sqrt2 {Re, Im} Phi_l^m (harmonics.cpp:281-320).  One workgroup per useful
-// source leaf; the leaf's points go through LDS in chunks of P2M_CHUNK.  Phase 1: thread task
-// (point, m) runs column m of the recurrences -- the sectoral chain Phi_m^m = (x + iy)^m /
-// (2^m m!) then Phi_l^m = ((2l - 1) z Phi_{l-1}^m - r^2 Phi_{l-2}^m) / (l^2 - m^2) up to l = p, p + 1
-// steps for every m -- and writes the real harmonics of the point to its LDS row.  Phase 2: thread
-// k owns coefficients k, k + 128, ... and adds q_j R_k(u_j) over the chunk's points for the three
-// components (FP64 sums in point order).  Both phases give every thread the same amount of work.
-constexpr int P2M_CHUNK = 16;
+// R(l, +-m) = (-1)^l sqrt2 {Re, Im} Phi_l^m (harmonics.cpp:281-320).  One wave (a one-wave
+// workgroup) per useful source leaf.  The leaf's tasks (point, m) -- point-major, p + 1 per point
+// -- run 64 at a time; task (point, m) runs column m of the recurrences, the sectoral chain
+// Phi_m^m = (x + iy)^m / (2^m m!) and then Phi_l^m = ((2l - 1) z Phi_{l-1}^m - r^2 Phi_{l-2}^m) /
+// (l^2 - m^2) up to l = p (p + 1 steps for every m, so the lanes stay in step), and writes Phi to
+// the point's row of a small LDS ring.  After each round of 64 tasks the points it completed are
+// added, lane k owning coefficients k, k + 64, ..., in point order with FP64 sums; the degree
+// signs and sqrt2 are applied once when the leaf is written.  The ring holds the points of one
+// round and its two boundary points (64 / (p + 1) + 2 rows).  T is the type of the recurrences
+// and the ring: FP64 in the FP64 mode, FP32 otherwise (|u| <= sqrt3 keeps every Phi_l^m in range
+// and the recurrences forward-stable).
+template <class T> __device__ __forceinline__ T recip_t(int n);
+template <> __device__ __forceinline__ double recip_t<double>(int n) { return recip(n); }
+template <> __device__ __forceinline__ float recip_t<float>(int n) { return recipf(n); }
+
 template <class T>
-__global__ __launch_bounds__(128) void k_p2m(const int* __restrict__ leaves, const double* __restrict__ center,
-                                             const int* __restrict__ begin, const int* __restrict__ end,
-                                             const int* __restrict__ level, ScaleTable sig,
-                                             const double* __restrict__ sxyz, const double* __restrict__ sq, int p,
-                                             int Lp, T* __restrict__ M) {
-    extern __shared__ double p2m_lds[];   // [P2M_CHUNK][L] real harmonics, then [P2M_CHUNK][3] charges
-    constexpr int RK = (MAX_ORDER + 1) * (MAX_ORDER + 1) / 128 + 1;
-    const int L = (p + 1) * (p + 1), P1 = p + 1, tid = threadIdx.x;
-    double* tab = p2m_lds;
-    double* qs = p2m_lds + P2M_CHUNK * L;
+__global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, const double* __restrict__ center,
+                                            const int* __restrict__ begin, const int* __restrict__ end,
+                                            const int* __restrict__ level, ScaleTable sig,
+                                            const double* __restrict__ sxyz, const double* __restrict__ sq, int p,
+                                            int Lp, T* __restrict__ M) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char p2m_lds[];
+    T* ring = (T*)p2m_lds;   // [RING][L] complex Phi of the points in flight (Re at (l, m), Im at (l, -m))
+    constexpr int KK = (MAX_ORDER + 1) * (MAX_ORDER + 1) / 64 + 1;   // coefficients per lane
+    const int P1 = p + 1, L = P1 * P1, RING = 64 / P1 + 2, lane = threadIdx.x;
     const int leaf = leaves[blockIdx.x];
     const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
     const double inv = 1.0 / sig.s[level[leaf]];
-    double acc[RK][3];
+    const int jb = begin[leaf], n = end[leaf] - jb, ntask = n * P1;
+    double acc[KK][3];
 #pragma unroll
-    for (int r = 0; r < RK; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
-    const int jb = begin[leaf], je = end[leaf];
-    for (int j0 = jb; j0 < je; j0 += P2M_CHUNK) {
-        const int nj = min(P2M_CHUNK, je - j0);
-        __syncthreads();   // the previous chunk's rows are no longer read
-        for (int t = tid; t < nj * P1; t += blockDim.x) {
-            const int pt = t / P1, m = t - pt * P1, j = j0 + pt;
-            const double ux = (sxyz[3 * j] - cx) * inv, uy = (sxyz[3 * j + 1] - cy) * inv,
-                         uz = (sxyz[3 * j + 2] - cz) * inv;
-            const double r2 = ux * ux + uy * uy + uz * uz;
-            double dr = 1.0, di = 0.0;
+    for (int r = 0; r < KK; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
+    // this lane's task: point pt, column m; a round moves it 64 tasks on
+    int pt = lane / P1, m = lane - pt * P1;
+    const int dpt = 64 / P1, dmv = 64 - dpt * P1;
+    int done = 0;
+    for (int t0 = 0; t0 < ntask; t0 += 64) {
+        if (t0 + lane < ntask) {
+            const int j = jb + pt;
+            const T ux = (T)((sxyz[3 * j] - cx) * inv), uy = (T)((sxyz[3 * j + 1] - cy) * inv),
+                    uz = (T)((sxyz[3 * j + 2] - cz) * inv);
+            const T r2 = ux * ux + uy * uy + uz * uz;
+            T dr = 1, di = 0;
             for (int i = 1; i <= m; i++) {
-                const double s = recip(2 * i);
-                const double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
+                const T s = recip_t<T>(2 * i);
+                const T nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
                 dr = nr;
                 di = ni;
             }
-            double* row = tab + pt * L;
-            double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
-            for (int l = m; l <= p; l++) {
-                double vr = dr, vi = di;
-                if (l > m) {
-                    const double f = recip(l * l - m * m);
-                    vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
-                    vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
-                    p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
-                }
-                const double s = sgnl(l) * (m == 0 ? 1.0 : SQRT2);
-                row[lmi(l, m)] = s * vr;
-                if (m > 0) row[lmi(l, -m)] = s * vi;
+            T* row = ring + (pt % RING) * L;
+            int idx = m * m + 2 * m;   // (l, m) at l = m; (l, -m) is idx - 2m
+            row[idx] = dr;
+            if (m > 0) row[idx - 2 * m] = di;
+            T p1r = dr, p1i = di, p2r = 0, p2i = 0, c = (T)(2 * m + 1);
+            for (int l = m + 1; l <= p; l++) {
+                const T f = recip_t<T>(l * l - m * m), a = c * uz;
+                const T vr = (a * p1r - r2 * p2r) * f, vi = (a * p1i - r2 * p2i) * f;
+                p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
+                idx += 2 * l;
+                row[idx] = vr;
+                if (m > 0) row[idx - 2 * m] = vi;
+                c += (T)2;
             }
         }
-        for (int t = tid; t < 3 * nj; t += blockDim.x) qs[t] = sq[3 * j0 + t];
-        __syncthreads();
+        pt += dpt;
+        m += dmv;
+        if (m >= P1) { m -= P1; pt++; }
+        __syncthreads();   // one wave: the ring rows are written before they are read
+        const int complete = min(n, (t0 + 64) / P1);
+        for (int q = done; q < complete; q++) {
+            const T* row = ring + (q % RING) * L;
+            const double q0 = sq[3 * (jb + q)], q1 = sq[3 * (jb + q) + 1], q2 = sq[3 * (jb + q) + 2];
 #pragma unroll
-        for (int r = 0; r < RK; r++) {
-            const int k = tid + 128 * r;
-            if (k >= L) break;
-            for (int pt = 0; pt < nj; pt++) {
-                const double v = tab[pt * L + k];
-                acc[r][0] += qs[3 * pt] * v;
-                acc[r][1] += qs[3 * pt + 1] * v;
-                acc[r][2] += qs[3 * pt + 2] * v;
+            for (int r = 0; r < KK; r++) {
+                const int k = lane + 64 * r;
+                if (k < L) {
+                    const double v = (double)row[k];
+                    acc[r][0] += q0 * v;
+                    acc[r][1] += q1 * v;
+                    acc[r][2] += q2 * v;
+                }
             }
         }
+        done = complete;
+        __syncthreads();   // the rows are read before the next round overwrites them
     }
     T* out = M + (size_t)leaf * 3 * Lp;
 #pragma unroll
-    for (int r = 0; r < RK; r++) {
-        const int k = tid + 128 * r;
+    for (int r = 0; r < KK; r++) {
+        const int k = lane + 64 * r;
         if (k >= Lp) break;
-        for (int c = 0; c < 3; c++) out[c * Lp + k] = (T)(k < L ? acc[r][c] : 0.0);
+        double s = 0.0;
+        if (k < L) {
+            const int l = (int)sqrt((double)k), mm = k - l * l - l;   // k = l^2 + l + mm
+            s = ((l & 1) ? -1.0 : 1.0) * (mm == 0 ? 1.0 : SQRT2);
+        }
+        for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = (T)(acc[r][cc] * s);
     }
-    for (int k = tid + 128 * RK; k < Lp; k += 128)   // padding beyond what the loop above covers
-        for (int c = 0; c < 3; c++) out[c * Lp + k] = (T)0;
+    for (int k = lane + 64 * KK; k < Lp; k += 64)   // padding beyond what the loop above covers
+        for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = (T)0;
 }
 
 // ---------------------------------------------------------------- L2P
@@ -171,125 +192,110 @@ __device__ __forceinline__ void deriv_coef(const double* Ls, int L, int c, int a
 
 // POT: the vector potential A = sigma^-1 Re sum alpha_c(l,m) Phi_l^m(u) over degree <= p (the
 // reference's calcA, calculation_point_octree.cpp:192-212) instead of B = curl A (degree <= p-1).
-template <class T, bool POT>
+//
+// One wave (a one-wave workgroup) per useful target leaf.  The leaf's local expansion is mapped
+// once to the coefficients beta (LDS, type T); then each lane evaluates one target: the sectoral
+// chain Phi_m^m = Phi_{m-1}^{m-1} (x + iy) / (2m) and, per m, Phi_l^m = ((2l - 1) z Phi_{l-1}^m -
+// r^2 Phi_{l-2}^m) / (l^2 - m^2), every (l, m) term added to the three components as it is made.
+// D (the highest degree evaluated) is a template parameter, so the loops unroll with the
+// recurrence factors as constants and the term index and the beta offsets fixed; the beta reads
+// are the same address in every lane (LDS broadcast); explicit fused multiply-adds.  T is FP64
+// in the FP64 mode, FP32 otherwise (|u| <= sqrt3 inside the leaf box keeps the recurrences in
+// range); the result is scaled and written in FP64.
+__device__ __forceinline__ float fma_l(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_l(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <class T, int D, bool POT>
 __global__ __launch_bounds__(64) void k_l2p(const int* __restrict__ leaves, const double* __restrict__ center,
                                             const int* __restrict__ begin, const int* __restrict__ end,
                                             const int* __restrict__ level, ScaleTable sig,
-                                            const double* __restrict__ txyz, const T* __restrict__ Lexp, int p, int Lp,
-                                            double* __restrict__ Bfar) {
-    extern __shared__ double sh[];
+                                            const double* __restrict__ txyz, const void* __restrict__ Lexp_v, int p,
+                                            int Lp, bool lexp_f64, double* __restrict__ Bfar) {
+    constexpr int NB = (D + 1) * (D + 2) / 2;   // (l', m') pairs with l' <= D
+    extern __shared__ __attribute__((aligned(16))) double sh[];
     const int L = (p + 1) * (p + 1);
-    const int D = POT ? p : p - 1;                // highest degree evaluated
-    const int NB = (D + 1) * (D + 2) / 2;         // (l', m') pairs with l' <= D
-    // LDS: beta, then one region holding first the local expansion and, once beta is built,
-    // the lanes' partial sums and the reciprocal table (fewer bytes per wave: more waves per CU)
-    double* beta = sh;                             // [3][NB][2]
-    double* Ls = sh + 6 * NB;                      // [3][L]
-    double* part = Ls;                             // [64][2][3] partial sums of the lanes
-    double* rcp = Ls + 6 * 64;                     // 1 / n for n <= max(D^2, 2D) (the recurrences' divisors)
+    double* Ls = sh;                             // [3][L] the local expansion (FP64)
+    T* beta = (T*)(sh + 3 * L);                  // [NB][3][2]
     const int leaf = leaves[blockIdx.x];
     const int lane = threadIdx.x;
-    const T* Lsrc = Lexp + (size_t)leaf * 3 * Lp;
-    for (int e = lane; e < 3 * L; e += 64) {
-        int c = e / L, k = e - c * L;
-        Ls[e] = (double)Lsrc[c * Lp + k];
+    if (lexp_f64) {
+        const double* Lsrc = (const double*)Lexp_v + (size_t)leaf * 3 * Lp;
+        for (int e = lane; e < 3 * L; e += 64) {
+            const int c = e / L, k = e - c * L;
+            Ls[e] = Lsrc[c * Lp + k];
+        }
+    } else {
+        const float* Lsrc = (const float*)Lexp_v + (size_t)leaf * 3 * Lp;
+        for (int e = lane; e < 3 * L; e += 64) {
+            const int c = e / L, k = e - c * L;
+            Ls[e] = (double)Lsrc[c * Lp + k];
+        }
     }
     __syncthreads();
     for (int e = lane; e < NB; e += 64) {
         int lp = 0;
         while ((lp + 1) * (lp + 2) / 2 <= e) lp++;
-        int mp = e - lp * (lp + 1) / 2;
+        const int mp = e - lp * (lp + 1) / 2;
+        double bv[3][2];
         if (POT) {
-            for (int c = 0; c < 3; c++) alpha_of(Ls, L, c, lp, mp, beta[(c * NB + e) * 2], beta[(c * NB + e) * 2 + 1]);
-            continue;
+            for (int c = 0; c < 3; c++) alpha_of(Ls, L, c, lp, mp, bv[c][0], bv[c][1]);
+        } else {
+            double d[3][3][2];   // d[comp][axis]
+            for (int c = 0; c < 3; c++)
+                for (int a = 0; a < 3; a++) deriv_coef(Ls, L, c, a, lp, mp, d[c][a][0], d[c][a][1]);
+            for (int q = 0; q < 2; q++) {   // B = curl A
+                bv[0][q] = d[2][1][q] - d[1][2][q];
+                bv[1][q] = d[0][2][q] - d[2][0][q];
+                bv[2][q] = d[1][0][q] - d[0][1][q];
+            }
         }
-        double d[3][3][2];   // d[comp][axis]
-        for (int c = 0; c < 3; c++)
-            for (int a = 0; a < 3; a++) deriv_coef(Ls, L, c, a, lp, mp, d[c][a][0], d[c][a][1]);
-        // B = curl A
-        for (int q = 0; q < 2; q++) {
-            beta[(0 * NB + e) * 2 + q] = d[2][1][q] - d[1][2][q];
-            beta[(1 * NB + e) * 2 + q] = d[0][2][q] - d[2][0][q];
-            beta[(2 * NB + e) * 2 + q] = d[1][0][q] - d[0][1][q];
+        for (int c = 0; c < 3; c++) {
+            beta[(e * 3 + c) * 2] = (T)bv[c][0];
+            beta[(e * 3 + c) * 2 + 1] = (T)bv[c][1];
         }
     }
-    __syncthreads();   // the local expansion is no longer read: its space takes the reciprocals
-    for (int e = lane; e <= max(D * D, 2 * D); e += 64) rcp[e] = recip(e);
     __syncthreads();
     const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
     const double inv = 1.0 / sig.s[level[leaf]];
     const double s2 = POT ? inv : inv * inv;
-    // lane (m, g): column m (<= D) of the degree <= D table for the targets 2 G it + g and
-    // 2 G it + G + g (two independent recurrence chains sharing the beta loads)
-    const int NM = D + 1, G = 64 / NM, m = lane % NM, g = lane / NM;
     const int kb = begin[leaf], n = end[leaf] - kb;
-    for (int it = 0; 2 * it * G < n; it++) {
-        const int t0 = 2 * it * G + g, t1 = t0 + G;
-        const bool v0 = g < G && t0 < n, v1 = g < G && t1 < n;
-        double b0 = 0, b1 = 0, b2 = 0, c0 = 0, c1 = 0, c2 = 0;
-        if (v0) {
-            const int k0 = kb + t0, k1 = kb + (v1 ? t1 : t0);
-            const double ux = (txyz[3 * k0] - cx) * inv, uy = (txyz[3 * k0 + 1] - cy) * inv,
-                         uz = (txyz[3 * k0 + 2] - cz) * inv;
-            const double wx = (txyz[3 * k1] - cx) * inv, wy = (txyz[3 * k1 + 1] - cy) * inv,
-                         wz = (txyz[3 * k1 + 2] - cz) * inv;
-            const double r2 = ux * ux + uy * uy + uz * uz, q2 = wx * wx + wy * wy + wz * wz;
-            double dr = 1.0, di = 0.0, er = 1.0, ei = 0.0;
-            for (int i = 1; i <= m; i++) {
-                const double s = rcp[2 * i];
-                double nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
-                double mr = (er * wx - ei * wy) * s, mi = (er * wy + ei * wx) * s;
-                dr = nr; di = ni; er = mr; ei = mi;
+    for (int t0 = 0; t0 < n; t0 += 64) {
+        const int t = t0 + lane;
+        if (t >= n) break;
+        const int k = kb + t;
+        const T ux = (T)((txyz[3 * k] - cx) * inv), uy = (T)((txyz[3 * k + 1] - cy) * inv),
+                uz = (T)((txyz[3 * k + 2] - cz) * inv);
+        const T r2 = ux * ux + uy * uy + uz * uz;
+        T b0 = 0, b1 = 0, b2 = 0;
+        T sr = 1, si = 0;   // Phi_m^m
+#pragma unroll
+        for (int m = 0; m <= D; m++) {
+            if (m > 0) {
+                constexpr T one = 1;
+                const T h = one / (T)(2 * m);
+                const T nr = (sr * ux - si * uy) * h, ni = (sr * uy + si * ux) * h;
+                sr = nr;
+                si = ni;
             }
-            double p1r = dr, p1i = di, p2r = 0.0, p2i = 0.0;
-            double o1r = er, o1i = ei, o2r = 0.0, o2i = 0.0;
+            T p1r = sr, p1i = si, p2r = 0, p2i = 0;
+#pragma unroll
             for (int l = m; l <= D; l++) {
-                double vr = dr, vi = di, yr = er, yi = ei;
+                T vr = p1r, vi = p1i;
                 if (l > m) {
-                    const double f = rcp[l * l - m * m];
-                    vr = ((2.0 * l - 1) * uz * p1r - r2 * p2r) * f;
-                    vi = ((2.0 * l - 1) * uz * p1i - r2 * p2i) * f;
-                    yr = ((2.0 * l - 1) * wz * o1r - q2 * o2r) * f;
-                    yi = ((2.0 * l - 1) * wz * o1i - q2 * o2i) * f;
+                    const T a = (T)(2 * l - 1) * uz, f = (T)1 / (T)(l * l - m * m);
+                    vr = fma_l(a, p1r, -r2 * p2r) * f;
+                    vi = fma_l(a, p1i, -r2 * p2i) * f;
                     p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
-                    o2r = o1r; o2i = o1i; o1r = yr; o1i = yi;
                 }
-                const int e = l * (l + 1) / 2 + m;
-                const double x0r = beta[(0 * NB + e) * 2], x0i = beta[(0 * NB + e) * 2 + 1];
-                const double x1r = beta[(1 * NB + e) * 2], x1i = beta[(1 * NB + e) * 2 + 1];
-                const double x2r = beta[(2 * NB + e) * 2], x2i = beta[(2 * NB + e) * 2 + 1];
-                b0 += x0r * vr - x0i * vi;
-                b1 += x1r * vr - x1i * vi;
-                b2 += x2r * vr - x2i * vi;
-                c0 += x0r * yr - x0i * yi;
-                c1 += x1r * yr - x1i * yi;
-                c2 += x2r * yr - x2i * yi;
+                const T* bt = beta + (l * (l + 1) / 2 + m) * 6;
+                b0 = fma_l(bt[0], vr, fma_l(-bt[1], vi, b0));
+                b1 = fma_l(bt[2], vr, fma_l(-bt[3], vi, b1));
+                b2 = fma_l(bt[4], vr, fma_l(-bt[5], vi, b2));
             }
         }
-        part[6 * lane] = b0;
-        part[6 * lane + 1] = b1;
-        part[6 * lane + 2] = b2;
-        part[6 * lane + 3] = c0;
-        part[6 * lane + 4] = c1;
-        part[6 * lane + 5] = c2;
-        __syncthreads();
-        if (m == 0 && (v0 || v1)) {
-            const int h = v0 ? 0 : 1;   // lanes m = 0 of group g: target t0 (and t1 below)
-            for (int u = h; u < 2; u++) {
-                if (u == 1 && !v1) break;
-                double s0 = 0, s1 = 0, sz = 0;
-                for (int mm = 0; mm < NM; mm++) {
-                    s0 += part[6 * (lane + mm) + 3 * u];
-                    s1 += part[6 * (lane + mm) + 3 * u + 1];
-                    sz += part[6 * (lane + mm) + 3 * u + 2];
-                }
-                const int k = kb + (u ? t1 : t0);
-                Bfar[3 * k] = s0 * s2;
-                Bfar[3 * k + 1] = s1 * s2;
-                Bfar[3 * k + 2] = sz * s2;
-            }
-        }
-        __syncthreads();
+        Bfar[3 * k] = (double)b0 * s2;
+        Bfar[3 * k + 1] = (double)b1 * s2;
+        Bfar[3 * k + 2] = (double)b2 * s2;
     }
 }
 
@@ -737,29 +743,48 @@ void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, cons
                 const double* sq, int p, int Lp, void* M, hipStream_t st) {
     if (!nleaves) return;
     if (p < 0 || p > MAX_ORDER) throw Error(4, "expansion order above 20");
-    const int L = (p + 1) * (p + 1);
-    const size_t lds = (size_t)P2M_CHUNK * (L + 3) * sizeof(double);
+    const int L = (p + 1) * (p + 1), ring = 64 / (p + 1) + 2;
     if (f64)
-        k_p2m<double><<<nleaves, 128, lds, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
-                                                 S.level.as<int>(), sig, sxyz, sq, p, Lp, (double*)M);
+        k_p2m<double><<<nleaves, 64, (size_t)ring * L * sizeof(double), st>>>(
+            leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(), S.level.as<int>(), sig, sxyz, sq, p, Lp,
+            (double*)M);
     else
-        k_p2m<float><<<nleaves, 128, lds, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
-                                                S.level.as<int>(), sig, sxyz, sq, p, Lp, (float*)M);
+        k_p2m<float><<<nleaves, 64, (size_t)ring * L * sizeof(float), st>>>(
+            leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(), S.level.as<int>(), sig, sxyz, sq, p, Lp,
+            (float*)M);
     FMM_HIP(hipGetLastError());
+}
+template <class T, int D, bool POT>
+static void l2p_go(const int* leaves, int nleaves, const TreeDev& Tr, const ScaleTable& sig, const double* txyz,
+                   const void* Lexp, bool lexp_f64, int p, int Lp, double* out, hipStream_t st) {
+    constexpr int NB = (D + 1) * (D + 2) / 2;
+    const size_t lds = (size_t)3 * (p + 1) * (p + 1) * sizeof(double) + (size_t)NB * 6 * sizeof(T);
+    k_l2p<T, D, POT><<<nleaves, 64, lds, st>>>(leaves, Tr.center.as<double>(), Tr.begin.as<int>(), Tr.end.as<int>(),
+                                               Tr.level.as<int>(), sig, txyz, Lexp, p, Lp, lexp_f64, out);
+}
+template <int D, bool POT>
+static void l2p_d(bool f64, const int* leaves, int nleaves, const TreeDev& Tr, const ScaleTable& sig,
+                  const double* txyz, const void* Lexp, int p, int Lp, double* out, hipStream_t st) {
+    if (f64)
+        l2p_go<double, D, POT>(leaves, nleaves, Tr, sig, txyz, Lexp, true, p, Lp, out, st);
+    else
+        l2p_go<float, D, POT>(leaves, nleaves, Tr, sig, txyz, Lexp, false, p, Lp, out, st);
 }
 template <bool POT>
 static void l2p_launch(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig,
                        const double* txyz, const void* Lexp, int p, int Lp, double* out, hipStream_t st) {
-    const int L = (p + 1) * (p + 1), D = POT ? p : p - 1, NB = (D + 1) * (D + 2) / 2;
-    // degree <= 63 columns: 64 lanes hold at least one target group
-    if (p < 1 || D + 1 > 64) throw Error(1, "L2P: order out of range");
-    size_t lds = (size_t)(6 * NB + std::max(3 * L, 6 * 64 + std::max(D * D, 2 * D) + 1)) * sizeof(double);
-    if (f64)
-        k_l2p<double, POT><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
-                                                     T.level.as<int>(), sig, txyz, (const double*)Lexp, p, Lp, out);
-    else
-        k_l2p<float, POT><<<nleaves, 64, lds, st>>>(leaves, T.center.as<double>(), T.begin.as<int>(), T.end.as<int>(),
-                                                    T.level.as<int>(), sig, txyz, (const float*)Lexp, p, Lp, out);
+    const int D = POT ? p : p - 1;
+    if (p < 1 || p > MAX_ORDER) throw Error(1, "L2P: order out of range");
+    if (!nleaves) return;
+    switch (D) {
+#define FMM_L2P_D(DV) \
+    case DV: l2p_d<DV, POT>(f64, leaves, nleaves, T, sig, txyz, Lexp, p, Lp, out, st); break;
+        FMM_L2P_D(0) FMM_L2P_D(1) FMM_L2P_D(2) FMM_L2P_D(3) FMM_L2P_D(4) FMM_L2P_D(5) FMM_L2P_D(6) FMM_L2P_D(7)
+        FMM_L2P_D(8) FMM_L2P_D(9) FMM_L2P_D(10) FMM_L2P_D(11) FMM_L2P_D(12) FMM_L2P_D(13) FMM_L2P_D(14)
+        FMM_L2P_D(15) FMM_L2P_D(16) FMM_L2P_D(17) FMM_L2P_D(18) FMM_L2P_D(19) FMM_L2P_D(20)
+#undef FMM_L2P_D
+        default: throw Error(1, "L2P: order out of range");
+    }
     FMM_HIP(hipGetLastError());
 }
 void launch_l2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig, const double* txyz,

@@ -8,8 +8,8 @@
 // Expansions are node-major [node][3][Lp] so a column is one contiguous vector.
 //
 // Each workgroup (4 waves) computes the whole Lp-row panel of its 64 columns, so every
-// gathered column is read from HBM once; wave w owns rows [w Lp/4, (w+1) Lp/4) as RB x 4
-// MFMA blocks of 16 x 16 (Lp = 64 RB).  K is staged through double-buffered LDS with the
+// gathered column is read from HBM once; wave w owns the row blocks w, w + 4, ... (RB of them,
+// Lp = 64 RB) times the 4 column blocks, as MFMA blocks of 16 x 16.  K is staged through double-buffered LDS with the
 // next step's global loads issued before the current step's MFMAs (one barrier per step).
 //
 // Precisions:
@@ -58,7 +58,7 @@ template <class T, int RB, bool TRANS, bool MULTI>
 __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const GemmTile* __restrict__ tiles,
                                               const int* __restrict__ pair_src, const int* __restrict__ pair_dst,
                                               const T* __restrict__ src_exp, T* __restrict__ dst_exp, int epi,
-                                              int nseg) {
+                                              int nseg, int tri) {
     // LDS row strides: four K-rows are read by the 16-lane groups of a wave at once; strides of
     // 16 (mod 64) words put the four rows on disjoint bank ranges (FP64: 8 doubles = 16 words)
     constexpr int Lp = 64 * RB, BK = 16, PAD = sizeof(T) == 8 ? 8 : 16, SA = Lp + PAD, SB = BN + PAD, NK = Lp / BK;
@@ -120,20 +120,33 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
     gload(0);
     lstore(0);
     __syncthreads();
-    const int wrow = wave * 16 * RB;
+    // wave w owns the row blocks w, w + 4, ..., so under a degree-triangular operator every wave
+    // skips about as many blocks as the others; per block its lowest and highest degree
+    int dlo[RB], dhi[RB];
+#pragma unroll
+    for (int a = 0; a < RB; a++) {
+        const int r0 = 16 * (wave + 4 * a);
+        dlo[a] = (int)sqrtf((float)r0);
+        dhi[a] = (int)sqrtf((float)(r0 + 15));
+    }
     for (int kt = 0; kt < NKT; kt++) {
         const int cur = kt & 1;
         if (kt + 1 < NKT) gload(kt + 1);
+        const int k0 = (kt % NK) * BK, klo = (int)sqrtf((float)k0), khi = (int)sqrtf((float)(k0 + BK - 1));
+        bool live[RB];
+#pragma unroll
+        for (int a = 0; a < RB; a++)
+            live[a] = tri == TRI_NONE || (tri == TRI_LOWER ? klo <= dhi[a] : khi >= dlo[a]);
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
             T af[RB], bf[4];
             const int kk = ks + (lane >> 4);
 #pragma unroll
-            for (int a = 0; a < RB; a++) af[a] = As[cur][kk][wrow + a * 16 + (lane & 15)];
-#pragma unroll
             for (int b = 0; b < 4; b++) bf[b] = Bs[cur][kk][b * 16 + (lane & 15)];
 #pragma unroll
-            for (int a = 0; a < RB; a++)
+            for (int a = 0; a < RB; a++) {
+                if (!live[a]) continue;   // uniform over the wave
+                af[a] = As[cur][kk][16 * (wave + 4 * a) + (lane & 15)];
 #pragma unroll
                 for (int b = 0; b < 4; b++) {
                     if constexpr (sizeof(T) == 8)
@@ -141,6 +154,7 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
                     else
                         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
                 }
+            }
         }
         if (kt + 1 < NKT) lstore(cur ^ 1);
         __syncthreads();
@@ -151,27 +165,28 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
         if (cl >= tile.ncol) continue;
         int col = tile.col0 + cl;
         if (epi == EPI_STORE) {
-            T* out = dst_exp + (size_t)col * Lp + wrow;
+            T* out = dst_exp + (size_t)col * Lp;
 #pragma unroll
             for (int a = 0; a < RB; a++) {
+                const int r0 = 16 * (wave + 4 * a);
                 if constexpr (sizeof(T) == 8) {
 #pragma unroll
-                    for (int r = 0; r < 4; r++) out[a * 16 + 4 * r + (lane >> 4)] = acc[a][b][r];
+                    for (int r = 0; r < 4; r++) out[r0 + 4 * r + (lane >> 4)] = acc[a][b][r];
                 } else {
-                    *(float4*)(out + a * 16 + 4 * (lane >> 4)) =
+                    *(float4*)(out + r0 + 4 * (lane >> 4)) =
                         make_float4(acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]);
                 }
             }
             continue;
         }
         int pair = col / 3, comp = col - 3 * pair;
-        T* dst = dst_exp + ((size_t)pair_dst[pair] * 3 + comp) * Lp + wrow;
+        T* dst = dst_exp + ((size_t)pair_dst[pair] * 3 + comp) * Lp;
 #pragma unroll
         for (int a = 0; a < RB; a++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 int rr = (sizeof(T) == 8) ? (4 * r + (lane >> 4)) : (4 * (lane >> 4) + r);
-                add_out(dst + a * 16 + rr, (T)acc[a][b][r], epi);
+                add_out(dst + 16 * (wave + 4 * a) + rr, (T)acc[a][b][r], epi);
             }
     }
 }
@@ -547,6 +562,160 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
     }
 }
 
+// The 128-column F16X3 M2L with the waves on a 2 x 2 grid: wave (wr, wc) owns the row blocks
+// [wr NB0, wr NB0 + NB0) below NB (NB0 = ceil(NB / 2)) and the tile columns [64 wc, 64 wc + 64),
+// so it reads half the operator rows and half the columns from LDS per K step (a wave tile of
+// 16 NB0 x 64 instead of 16 NB x 32: about a third less LDS traffic per MFMA).  Staging, numerics
+// and epilogue as k_gemm_m2l_f16x3.
+template <int NB, int NKS, int BD>
+__global__ __launch_bounds__(256) void k_gemm_m2l_f16x3_q(const _Float16* __restrict__ ops_hi,
+                                                          const _Float16* __restrict__ ops_lo,
+                                                          const float* __restrict__ ops_rinv,
+                                                          const GemmTile* __restrict__ tiles,
+                                                          const int* __restrict__ pair_src,
+                                                          const _Float16* __restrict__ src_hi,
+                                                          const _Float16* __restrict__ src_lo,
+                                                          const float* __restrict__ cscale, float* __restrict__ slab,
+                                                          int op_base, int ntiles) {
+    constexpr int R = 16 * NB, Kp = 32 * NKS, BK = 32, S = BK + 8;   // LDS row stride: 80 bytes
+    constexpr int NQ = (4 * R + 255) / 256;                          // A: 8-half vectors per thread and part
+    constexpr int W = 128, NB0 = (NB + 1) / 2;
+    __shared__ __attribute__((aligned(16))) _Float16 Ah[2][R][S];
+    __shared__ __attribute__((aligned(16))) _Float16 Al[2][R][S];
+    __shared__ __attribute__((aligned(16))) _Float16 Bh[2][W][S];
+    __shared__ __attribute__((aligned(16))) _Float16 Bl[2][W][S];
+    // persistent: the grid (a multiple of 8, workgroup b on XCD b mod 8) walks the tiles, each
+    // XCD a contiguous eighth of them (one class's operator shared in one L2); a workgroup
+    // starts its next tile while its last stores drain
+    const int xcd = blockIdx.x & 7, q = ntiles >> 3, rem = ntiles & 7;
+    const int xbeg = xcd * q + min(xcd, rem), xcnt = q + (xcd < rem ? 1 : 0), per_xcd = gridDim.x >> 3;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, li = lane & 15;
+    const int wr = wave & 1, wc = wave >> 1, rb0 = wr * NB0;
+    for (int it = blockIdx.x >> 3; it < xcnt; it += per_xcd) {
+        const GemmTile tile = tiles[xbeg + it];
+        const int op = tile.op - op_base;
+        const _Float16* Ahg = ops_hi + (size_t)op * R * Kp;
+        const _Float16* Alg = ops_lo + (size_t)op * R * Kp;
+        const int bc = tid >> 2, bq = (tid & 3) * 8;
+        const _Float16* bph[2];
+        const _Float16* bpl[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            bph[c] = bpl[c] = nullptr;
+            if (bc + 64 * c < tile.ncol) {
+                const int col = tile.col0 + bc + 64 * c, pair = col / 3, comp = col - 3 * pair;
+                const size_t sc = (size_t)pair_src[pair] * 3 + comp;
+                bph[c] = src_hi + sc * Kp;
+                bpl[c] = src_lo + sc * Kp;
+            }
+        }
+        half8 rah[NQ], ral[NQ];
+        half8 rbh[BD][2], rbl[BD][2];   // [set][c]: the multipole columns of BD K steps in flight
+        auto aload = [&](int k0) {
+#pragma unroll
+            for (int j = 0; j < NQ; j++) {
+                const int qi = tid + 256 * j;
+                if (qi < 4 * R) {
+                    const size_t o = (size_t)(qi >> 2) * Kp + k0 + (qi & 3) * 8;
+                    rah[j] = *(const half8*)(Ahg + o);
+                    ral[j] = *(const half8*)(Alg + o);
+                }
+            }
+        };
+        auto bload = [&](int set, int k0) {
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const half8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+                rbh[set][c] = bph[c] ? *(const half8*)(bph[c] + k0 + bq) : z;
+                rbl[set][c] = bpl[c] ? *(const half8*)(bpl[c] + k0 + bq) : z;
+            }
+        };
+        auto lstore = [&](int buf, int set) {
+#pragma unroll
+            for (int j = 0; j < NQ; j++) {
+                const int qi = tid + 256 * j;
+                if (qi < 4 * R) {
+                    *(half8*)&Ah[buf][qi >> 2][(qi & 3) * 8] = rah[j];
+                    *(half8*)&Al[buf][qi >> 2][(qi & 3) * 8] = ral[j];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                *(half8*)&Bh[buf][bc + 64 * c][bq] = rbh[set][c];
+                *(half8*)&Bl[buf][bc + 64 * c][bq] = rbl[set][c];
+            }
+        };
+        f32x4 acc[NB0][4];
+#pragma unroll
+        for (int a = 0; a < NB0; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[a][c] = f32x4{0, 0, 0, 0};
+        // the multipole columns (gathered from HBM) are loaded BD K steps ahead, the operator
+        // rows (shared by the tiles of one class, mostly L2 hits) one step ahead; the K loop is
+        // unrolled so the register sets are indexed statically
+        aload(0);
+#pragma unroll
+        for (int d = 0; d < BD; d++)
+            if (d < NKS) bload(d, d * BK);
+        lstore(0, 0);
+        __syncthreads();
+#pragma unroll
+        for (int kt = 0; kt < NKS; kt++) {
+            const int cur = kt & 1;
+            if (kt + 1 < NKS) aload((kt + 1) * BK);
+            if (kt + BD < NKS) bload(kt % BD, (kt + BD) * BK);   // this set held step kt, stored already
+            const int kk = 8 * g;
+            half8 bh[4], bl[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                bh[c] = *(const half8*)&Bh[cur][wc * 64 + 16 * c + li][kk];
+                bl[c] = *(const half8*)&Bl[cur][wc * 64 + 16 * c + li][kk];
+            }
+            // all of this K step's operator fragments are read up front: the MFMAs of a row block
+            // wait only for its own reads, which land while the earlier blocks' MFMAs run
+            half8 ah[NB0], al[NB0];
+#pragma unroll
+            for (int a = 0; a < NB0; a++) {
+                // the second row half's last block is past NB when NB is odd: it reads the block
+                // NB - 1 instead (in bounds) and its MFMAs are skipped
+                const int row = min(rb0 + a, NB - 1) * 16 + li;
+                ah[a] = *(const half8*)&Ah[cur][row][kk];
+                al[a] = *(const half8*)&Al[cur][row][kk];
+            }
+#pragma unroll
+            for (int a = 0; a < NB0; a++) {
+                if (rb0 + a >= NB) continue;   // uniform over the wave
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh[c], acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl[c], acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[c], acc[a][c], 0, 0, 0);
+                }
+            }
+            // the other LDS buffer was last read before the previous barrier
+            if (kt + 1 < NKS) lstore(cur ^ 1, (kt + 1) % BD);
+            __syncthreads();
+        }
+        const float* rv = ops_rinv + (size_t)op * R;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int cl = wc * 64 + 16 * c + li;
+            if (cl >= tile.ncol) continue;
+            const int col = tile.col0 + cl, pair = col / 3, comp = col - 3 * pair;
+            const float sinv = 1.0f / cscale[pair_src[pair] * 3 + comp];
+            float* out = slab + (size_t)col * R;
+#pragma unroll
+            for (int a = 0; a < NB0; a++) {
+                if (rb0 + a >= NB) break;
+                const float4 r = *(const float4*)(rv + (rb0 + a) * 16 + 4 * g);
+                const f32x4 v = f32x4{acc[a][c][0] * r.x * sinv, acc[a][c][1] * r.y * sinv, acc[a][c][2] * r.z * sinv,
+                                      acc[a][c][3] * r.w * sinv};
+                __builtin_nontemporal_store(v, (f32x4*)(out + (rb0 + a) * 16 + 4 * g));
+            }
+        }
+    }
+}
+
 // per multipole column c (node * 3 + component): the power of 2 s_c that brings
 // max_i |M[c][i] dm[i]| into [0.5, 1) (1 for a zero column), and the column's equilibrated values
 // x_i = M[c][i] dm[i] s_c split into FP16 hi = fp16(x) and lo = fp16(x - hi), Kp per column (zero
@@ -583,6 +752,24 @@ void split_columns(const float* M, int ncols, int L, int Lp, int Kp, const float
 int m2l_f16_rows(int L) { return 16 * ((L + 15) / 16); }
 int m2l_f16_kpad(int L) { return 32 * ((L + 31) / 32); }
 
+// 128-column tiles: the double-buffered 2 x 2 kernel where the single-buffered one's
+// accumulators leave one workgroup per CU anyway (R >= 224, p >= 13) and its two LDS buffers fit
+// (R <= 384, p <= 18); the single-buffered kernel (two or more workgroups per CU below) otherwise
+template <int NB, int NKS>
+static void launch_f16x3_128(int ntiles, const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv,
+                             const GemmTile* d_tiles, const int* pair_src, const _Float16* src_hi,
+                             const _Float16* src_lo, const float* cscale, float* slab, int op_base, hipStream_t st) {
+    if constexpr (NB >= 14 && NB <= 24) {
+        // one workgroup per CU (the two LDS buffers), 8 per XCD round
+        const int grid = std::min(8 * ((ntiles + 7) / 8), 8 * ((num_cus() + 7) / 8));
+        k_gemm_m2l_f16x3_q<NB, NKS, 2><<<grid, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hi, src_lo,
+                                                          cscale, slab, op_base, ntiles);
+    }
+    else
+        k_gemm_m2l_f16x3<NB, NKS, 2><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hi,
+                                                             src_lo, cscale, slab, op_base);
+}
+
 void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv, int op_base,
                     const GemmTile* d_tiles, int ntiles, int tile_cols, const int* pair_src, const _Float16* src_hi,
                     const _Float16* src_lo, const float* cscale, float* slab, int L, hipStream_t st) {
@@ -592,8 +779,8 @@ void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float*
 #define FMM_F16(NBV, NKV)                                                                                            \
     if (nb == NBV && nks == NKV) {                                                                                   \
         if (tile_cols == 128)                                                                                        \
-            k_gemm_m2l_f16x3<NBV, NKV, 2><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src,       \
-                                                                  src_hi, src_lo, cscale, slab, op_base);            \
+            launch_f16x3_128<NBV, NKV>(ntiles, ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hi, src_lo, cscale,  \
+                                       slab, op_base, st);                                                           \
         else                                                                                                         \
             k_gemm_m2l_f16x3<NBV, NKV, 1><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src,       \
                                                                   src_hi, src_lo, cscale, slab, op_base);            \
@@ -654,13 +841,13 @@ void m2l_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTil
 template <int RB, bool TR, bool MU>
 static void launch_rb(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
                       const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int epi,
-                      int nseg, hipStream_t st) {
+                      int nseg, int tri, hipStream_t st) {
     if (prec == Precision::F64)
         k_gemm<double, RB, TR, MU><<<ntiles, 256, 0, st>>>((const double*)ops, d_tiles, pair_src, pair_dst,
-                                                          (const double*)src_exp, (double*)dst_exp, epi, nseg);
+                                                          (const double*)src_exp, (double*)dst_exp, epi, nseg, tri);
     else if (prec == Precision::F32)
         k_gemm<float, RB, TR, MU><<<ntiles, 256, 0, st>>>((const float*)ops, d_tiles, pair_src, pair_dst,
-                                                         (const float*)src_exp, (float*)dst_exp, epi, nseg);
+                                                         (const float*)src_exp, (float*)dst_exp, epi, nseg, tri);
     else
         k_gemm_bf16x3<RB, TR, MU><<<ntiles, 256, 0, st>>>((const __bf16*)ops, (const __bf16*)ops_lo, d_tiles, pair_src,
                                                          pair_dst, (const float*)src_exp, (float*)dst_exp, epi, nseg);
@@ -670,15 +857,15 @@ static void launch_rb(Precision prec, const void* ops, const void* ops_lo, const
 template <bool TR>
 static void launch(int RB, Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
                    const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int epi, int nseg,
-                   hipStream_t st) {
+                   int tri, hipStream_t st) {
 #define FMM_RB(R)                                                                                             \
     case R:                                                                                                   \
         if (nseg > 1)                                                                                         \
             launch_rb<R, TR, true>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epi, \
-                                   nseg, st);                                                                 \
+                                   nseg, tri, st);                                                            \
         else                                                                                                  \
             launch_rb<R, TR, false>(prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epi, \
-                                    1, st);                                                                   \
+                                    1, tri, st);                                                              \
         break;
     switch (RB) {
         FMM_RB(1) FMM_RB(2) FMM_RB(3) FMM_RB(4) FMM_RB(5) FMM_RB(6) FMM_RB(7)
@@ -689,16 +876,19 @@ static void launch(int RB, Precision prec, const void* ops, const void* ops_lo, 
 
 void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
                 const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, int epilogue,
-                hipStream_t st, bool transpose, int nseg) {
+                hipStream_t st, bool transpose, int nseg, int tri) {
     if (ntiles == 0) return;
+    if (tri < TRI_NONE || tri > TRI_UPPER) throw Error(1, "unknown operator structure");
     if (Lp % 64 != 0) throw Error(1, "padded expansion length must be a multiple of 64");
     int RB = Lp / 64;
     if (epilogue < EPI_ADD || epilogue > EPI_STORE) throw Error(1, "unknown GEMM epilogue");
     if (nseg < 1) throw Error(1, "a GEMM tile needs at least one segment");
     if (transpose)
-        launch<true>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epilogue, nseg, st);
+        launch<true>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epilogue, nseg, tri,
+                     st);
     else
-        launch<false>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epilogue, nseg, st);
+        launch<false>(RB, prec, ops, ops_lo, d_tiles, ntiles, pair_src, pair_dst, src_exp, dst_exp, epilogue, nseg, tri,
+                      st);
 }
 
 // host-side tiling of class column ranges: class c owns pairs [begin[c], begin[c+1])

@@ -17,6 +17,11 @@ enum Epilogue {
     EPI_ATOMIC = 1,   // the same with hardware FP atomics (destinations repeat)
     EPI_STORE = 2     // written to dst_exp + j * Lp (a per-pair slab; pair_dst unused)
 };
+// Degree structure of the applied operator (row = output coefficient of degree l, column = input
+// coefficient of degree l'): TRI_LOWER has zeros wherever l' > l (M2M, and the transposed L2L),
+// TRI_UPPER wherever l' < l (L2L, and the transposed M2M).  The FP64/FP32 GEMM skips the 16 x 16
+// blocks that are zero by this structure.
+enum { TRI_NONE = 0, TRI_LOWER = 1, TRI_UPPER = 2 };
 
 // ops: operator tables (double / float / bf16 hi), ops_lo: bf16 lo parts (BF16X3 only).
 // transpose: apply T^T (adjoint passes).
@@ -25,7 +30,7 @@ enum Epilogue {
 // node pair_src + g (the 8 children of a parent, first child in pair_src).
 void class_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTile* d_tiles, int ntiles,
                 const int* pair_src, const int* pair_dst, const void* src_exp, void* dst_exp, int Lp, int epilogue,
-                hipStream_t st, bool transpose = false, int nseg = 1);
+                hipStream_t st, bool transpose = false, int nseg = 1, int tri = TRI_NONE);
 
 // Pairs grouped by the node that receives them: order[begin[v] .. end[v]) are the pair indices
 // (ascending) whose key is v; nodes without pairs have begin = end = 0.  Each node's range is

@@ -19,4 +19,15 @@ static __constant__ RecipTable RECIP = RecipTable();
 
 __device__ __forceinline__ double recip(int n) { return RECIP.v[n]; }
 
+// the same reciprocals rounded to FP32, for the FP32 recurrences
+struct RecipTableF {
+    float v[RECIP_MAX + 1];
+    constexpr RecipTableF() : v() {
+        for (int i = 1; i <= RECIP_MAX; i++) v[i] = (float)(1.0 / (double)i);
+    }
+};
+static __constant__ RecipTableF RECIPF = RecipTableF();
+
+__device__ __forceinline__ float recipf(int n) { return RECIPF.v[n]; }
+
 }  // namespace fmm
