@@ -164,7 +164,7 @@ struct fmm_ctx {
     PairIndex m2l_ix;                            // far pairs by target node
     // F16X3: per-coefficient scales dm, the M2L tables split in FP16 (their FP32 staging), the
     // multipole column scales; down_l2l = L2L operators ahead of the M2L ones in the down tables
-    DevBuf dm, m2l_hi, m2l_lo, m2l_rinv, m2l_work, cscale, Mh, Ml;   // Mh / Ml: split multipoles
+    DevBuf dm, m2l_hi, m2l_lo, m2l_rinv, m2l_work, cscale, Mhl;   // Mhl: split multipoles (hi | lo)
     int down_l2l = 0;
     const OpSpec* down_specs = nullptr;          // device specs of the down tables
     DevBuf m2l_slab;                             // M2L result per (pair, component)
@@ -727,7 +727,7 @@ void translate_gemm(fmm_ctx* c) {
     if (f16 && P.ntiles) {   // the B side of the FP16 split: column scales and hi / lo parts
         const int ncols = 3 * c->S.num_nodes, kp = m2l_f16_kpad(c->L);
         split_columns(c->M.as<float>(), ncols, c->L, c->Lp, kp, c->dm.as<float>(), c->cscale.get<float>(ncols),
-                      c->Mh.get<_Float16>((size_t)ncols * kp), c->Ml.get<_Float16>((size_t)ncols * kp), st);
+                      c->Mhl.get<_Float16>((size_t)2 * ncols * kp), st);
     }
     c->rec(E_GEMM0);
     if (P.ntiles) {
@@ -735,8 +735,8 @@ void translate_gemm(fmm_ctx* c) {
         void* slab = c->m2l_slab.reserve((size_t)3 * P.npairs * R * c->esize());
         if (f16)
             m2l_gemm_f16x3(c->m2l_hi.as<_Float16>(), c->m2l_lo.as<_Float16>(), c->m2l_rinv.as<float>(), c->down_l2l,
-                           P.tiles, P.ntiles, m2l_tile_cols(c), c->lists.far_s.as<int>(), c->Mh.as<_Float16>(),
-                           c->Ml.as<_Float16>(), c->cscale.as<float>(), (float*)slab, c->L, st);
+                           P.tiles, P.ntiles, m2l_tile_cols(c), c->lists.far_s.as<int>(), c->Mhl.as<_Float16>(),
+                           c->cscale.as<float>(), (float*)slab, c->L, st);
         else
             m2l_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, c->lists.far_s.as<int>(), c->M.ptr, slab,
                      c->L, c->Lp, st);

@@ -445,8 +445,7 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
                                                         const float* __restrict__ ops_rinv,
                                                         const GemmTile* __restrict__ tiles,
                                                         const int* __restrict__ pair_src,
-                                                        const _Float16* __restrict__ src_hi,
-                                                        const _Float16* __restrict__ src_lo,
+                                                        const _Float16* __restrict__ src_hl,
                                                         const float* __restrict__ cscale, float* __restrict__ slab,
                                                         int op_base) {
     constexpr int R = 16 * NB, Kp = 32 * NKS, BK = 32, S = BK + 8;   // LDS row stride: 80 bytes
@@ -465,7 +464,7 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
     const _Float16* Ahg = ops_hi + (size_t)op * R * Kp;
     const _Float16* Alg = ops_lo + (size_t)op * R * Kp;
     // B staging: columns bc + 64 c, k = bq .. bq + 7 of each K step, from the multipoles split
-    // beforehand (split_columns): equilibrated FP16 hi and lo parts, Kp per column
+    // beforehand (split_columns): equilibrated FP16 hi and lo parts, [column][K step][hi | lo]
     const int bc = tid >> 2, bq = (tid & 3) * 8;
     const _Float16* bph[NC];
     const _Float16* bpl[NC];
@@ -475,8 +474,8 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
         if (bc + 64 * c < tile.ncol) {
             const int col = tile.col0 + bc + 64 * c, pair = col / 3, comp = col - 3 * pair;
             const size_t sc = (size_t)pair_src[pair] * 3 + comp;
-            bph[c] = src_hi + sc * Kp;
-            bpl[c] = src_lo + sc * Kp;
+            bph[c] = src_hl + sc * 2 * Kp;
+            bpl[c] = bph[c] + 32;
         }
     }
     half8 rah[NQ], ral[NQ];
@@ -494,8 +493,8 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             const half8 z = {0, 0, 0, 0, 0, 0, 0, 0};
-            rbh[c] = bph[c] ? *(const half8*)(bph[c] + k0 + bq) : z;
-            rbl[c] = bpl[c] ? *(const half8*)(bpl[c] + k0 + bq) : z;
+            rbh[c] = bph[c] ? *(const half8*)(bph[c] + 2 * k0 + bq) : z;
+            rbl[c] = bpl[c] ? *(const half8*)(bpl[c] + 2 * k0 + bq) : z;
         }
     };
     auto lstore = [&]() {
@@ -573,8 +572,7 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3_q(const _Float16* __rest
                                                           const float* __restrict__ ops_rinv,
                                                           const GemmTile* __restrict__ tiles,
                                                           const int* __restrict__ pair_src,
-                                                          const _Float16* __restrict__ src_hi,
-                                                          const _Float16* __restrict__ src_lo,
+                                                          const _Float16* __restrict__ src_hl,
                                                           const float* __restrict__ cscale, float* __restrict__ slab,
                                                           int op_base, int ntiles) {
     constexpr int R = 16 * NB, Kp = 32 * NKS, BK = 32, S = BK + 8;   // LDS row stride: 80 bytes
@@ -605,8 +603,8 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3_q(const _Float16* __rest
             if (bc + 64 * c < tile.ncol) {
                 const int col = tile.col0 + bc + 64 * c, pair = col / 3, comp = col - 3 * pair;
                 const size_t sc = (size_t)pair_src[pair] * 3 + comp;
-                bph[c] = src_hi + sc * Kp;
-                bpl[c] = src_lo + sc * Kp;
+                bph[c] = src_hl + sc * 2 * Kp;
+                bpl[c] = bph[c] + 32;
             }
         }
         half8 rah[NQ], ral[NQ];
@@ -626,8 +624,8 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3_q(const _Float16* __rest
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 const half8 z = {0, 0, 0, 0, 0, 0, 0, 0};
-                rbh[set][c] = bph[c] ? *(const half8*)(bph[c] + k0 + bq) : z;
-                rbl[set][c] = bpl[c] ? *(const half8*)(bpl[c] + k0 + bq) : z;
+                rbh[set][c] = bph[c] ? *(const half8*)(bph[c] + 2 * k0 + bq) : z;
+                rbl[set][c] = bpl[c] ? *(const half8*)(bpl[c] + 2 * k0 + bq) : z;
             }
         };
         auto lstore = [&](int buf, int set) {
@@ -718,34 +716,56 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3_q(const _Float16* __rest
 
 // per multipole column c (node * 3 + component): the power of 2 s_c that brings
 // max_i |M[c][i] dm[i]| into [0.5, 1) (1 for a zero column), and the column's equilibrated values
-// x_i = M[c][i] dm[i] s_c split into FP16 hi = fp16(x) and lo = fp16(x - hi), Kp per column (zero
-// beyond L): the B operand of the F16X3 M2L, split once per evaluation instead of once per tile.
-// A wave per column, the waves of a capped grid striding over the columns.
+// x_i = M[c][i] dm[i] s_c split into FP16 hi = fp16(x) and lo = fp16(x - hi) (zero beyond L): the
+// B operand of the F16X3 M2L, split once per evaluation instead of once per tile.  Layout
+// [column][K step][hi 32 | lo 32]: the hi and lo halves of one K step share a 128-byte line.
+// A wave per column, lane j owning the 8 coefficients [8j, 8j + 8) (Kp <= 8 * 64): two 16-byte
+// loads, a maximum over the wave, two 16-byte stores; the waves of a capped grid stride over the
+// columns and keep their lanes' dm in registers.
 __global__ __launch_bounds__(256) void k_split_columns(const float* __restrict__ M, int ncols, int L, int Lp, int Kp,
                                                        const float* __restrict__ dm, float* __restrict__ scale,
-                                                       _Float16* __restrict__ hi, _Float16* __restrict__ lo) {
-    const int lane = threadIdx.x & 63;
+                                                       _Float16* __restrict__ hl) {
+    const int lane = threadIdx.x & 63, i0 = 8 * lane;
+    const bool act = i0 < Kp;   // Kp and Lp are multiples of 8, Kp <= Lp
+    const int off = (i0 >> 5) * 64 + (i0 & 31);   // K step i0 / 32, chunk within the step
+    float d[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) d[e] = act && i0 + e < L ? dm[i0 + e] : 0.0f;
     for (int c = blockIdx.x * 4 + (threadIdx.x >> 6); c < ncols; c += gridDim.x * 4) {
-        const float* col = M + (size_t)c * Lp;
+        float x[8];
         float mx = 0.0f;
-        for (int i = lane; i < L; i += 64) mx = fmaxf(mx, fabsf(col[i] * dm[i]));
-        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-        const float s = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
-        if (lane == 0) scale[c] = s;
-        for (int i = lane; i < Kp; i += 64) {
-            const float x = i < L ? col[i] * dm[i] * s : 0.0f;
-            const _Float16 h = (_Float16)x;
-            hi[(size_t)c * Kp + i] = h;
-            lo[(size_t)c * Kp + i] = (_Float16)(x - (float)h);
+        if (act) {
+            const float4 u = *(const float4*)(M + (size_t)c * Lp + i0);
+            const float4 v = *(const float4*)(M + (size_t)c * Lp + i0 + 4);
+            x[0] = u.x * d[0]; x[1] = u.y * d[1]; x[2] = u.z * d[2]; x[3] = u.w * d[3];
+            x[4] = v.x * d[4]; x[5] = v.y * d[5]; x[6] = v.z * d[6]; x[7] = v.w * d[7];
+#pragma unroll
+            for (int e = 0; e < 8; e++) mx = fmaxf(mx, fabsf(x[e]));
+        }
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        const float sc = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
+        if (lane == 0) scale[c] = sc;
+        if (act) {
+            half8 h, l;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const float y = x[e] * sc;
+                h[e] = (_Float16)y;
+                l[e] = (_Float16)(y - (float)h[e]);
+            }
+            _Float16* dst = hl + (size_t)c * 2 * Kp + off;
+            *(half8*)dst = h;
+            *(half8*)(dst + 32) = l;
         }
     }
 }
 
-void split_columns(const float* M, int ncols, int L, int Lp, int Kp, const float* dm, float* scale, _Float16* hi,
-                   _Float16* lo, hipStream_t st) {
+void split_columns(const float* M, int ncols, int L, int Lp, int Kp, const float* dm, float* scale, _Float16* hl,
+                   hipStream_t st) {
     if (ncols <= 0) return;
+    if (Kp > 8 * 64 || Kp % 32 || Lp % 8 || Kp > Lp) throw Error(4, "FP16 split: column length out of range");
     const int grid = std::min((ncols + 3) / 4, 8 * num_cus());
-    k_split_columns<<<grid, 256, 0, st>>>(M, ncols, L, Lp, Kp, dm, scale, hi, lo);
+    k_split_columns<<<grid, 256, 0, st>>>(M, ncols, L, Lp, Kp, dm, scale, hl);
     FMM_HIP(hipGetLastError());
 }
 
@@ -757,33 +777,33 @@ int m2l_f16_kpad(int L) { return 32 * ((L + 31) / 32); }
 // (R <= 384, p <= 18); the single-buffered kernel (two or more workgroups per CU below) otherwise
 template <int NB, int NKS>
 static void launch_f16x3_128(int ntiles, const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv,
-                             const GemmTile* d_tiles, const int* pair_src, const _Float16* src_hi,
-                             const _Float16* src_lo, const float* cscale, float* slab, int op_base, hipStream_t st) {
+                             const GemmTile* d_tiles, const int* pair_src, const _Float16* src_hl,
+                             const float* cscale, float* slab, int op_base, hipStream_t st) {
     if constexpr (NB >= 14 && NB <= 24) {
         // one workgroup per CU (the two LDS buffers), 8 per XCD round
         const int grid = std::min(8 * ((ntiles + 7) / 8), 8 * ((num_cus() + 7) / 8));
-        k_gemm_m2l_f16x3_q<NB, NKS, 2><<<grid, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hi, src_lo,
-                                                          cscale, slab, op_base, ntiles);
+        k_gemm_m2l_f16x3_q<NB, NKS, 2><<<grid, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hl,
+                                                             cscale, slab, op_base, ntiles);
     }
     else
-        k_gemm_m2l_f16x3<NB, NKS, 2><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hi,
-                                                             src_lo, cscale, slab, op_base);
+        k_gemm_m2l_f16x3<NB, NKS, 2><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hl,
+                                                             cscale, slab, op_base);
 }
 
 void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv, int op_base,
-                    const GemmTile* d_tiles, int ntiles, int tile_cols, const int* pair_src, const _Float16* src_hi,
-                    const _Float16* src_lo, const float* cscale, float* slab, int L, hipStream_t st) {
+                    const GemmTile* d_tiles, int ntiles, int tile_cols, const int* pair_src, const _Float16* src_hl,
+                    const float* cscale, float* slab, int L, hipStream_t st) {
     if (ntiles == 0) return;
     const int nb = m2l_f16_rows(L) / 16, nks = m2l_f16_kpad(L) / 32;
     if (tile_cols != 64 && tile_cols != 128) throw Error(4, "FP16 M2L: tiles of 64 or 128 columns");
 #define FMM_F16(NBV, NKV)                                                                                            \
     if (nb == NBV && nks == NKV) {                                                                                   \
         if (tile_cols == 128)                                                                                        \
-            launch_f16x3_128<NBV, NKV>(ntiles, ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hi, src_lo, cscale,  \
-                                       slab, op_base, st);                                                           \
+            launch_f16x3_128<NBV, NKV>(ntiles, ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hl, cscale, slab,    \
+                                       op_base, st);                                                                 \
         else                                                                                                         \
             k_gemm_m2l_f16x3<NBV, NKV, 1><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src,       \
-                                                                  src_hi, src_lo, cscale, slab, op_base);            \
+                                                                  src_hl, cscale, slab, op_base);                    \
         FMM_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                      \
     }
@@ -986,7 +1006,7 @@ __global__ __launch_bounds__(256) void k_reduce_segments(const uint32_t* __restr
                                                          const int* __restrict__ end, const int* __restrict__ order,
                                                          const V* __restrict__ slab, int nnodes, int nv,
                                                          V* __restrict__ partial) {
-    constexpr int U = NR <= 3 ? 4 : NR <= 6 ? 2 : 1;   // about 12 vectors in flight per lane
+    constexpr int U = NR <= 4 ? 4 : NR <= 8 ? 2 : 1;   // 8 to 16 vectors in flight per lane
     const int sg = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (sg >= (int)seg_off[nnodes]) return;
     const int v = seg_node[sg];
