@@ -165,6 +165,7 @@ struct fmm_ctx {
     // F16X3: per-coefficient scales dm, the M2L tables split in FP16 (their FP32 staging), the
     // multipole column scales; down_l2l = L2L operators ahead of the M2L ones in the down tables
     DevBuf dm, m2l_hi, m2l_lo, m2l_rinv, m2l_work, cscale, Mhl;   // Mhl: split multipoles (hi | lo)
+    DevBuf queues;   // work counters: [0, 8) the M2L GEMM's per-XCD tiles, [8] the near-field leaves
     int down_l2l = 0;
     const OpSpec* down_specs = nullptr;          // device specs of the down tables
     DevBuf m2l_slab;                             // M2L result per (pair, component)
@@ -736,7 +737,7 @@ void translate_gemm(fmm_ctx* c) {
         if (f16)
             m2l_gemm_f16x3(c->m2l_hi.as<_Float16>(), c->m2l_lo.as<_Float16>(), c->m2l_rinv.as<float>(), c->down_l2l,
                            P.tiles, P.ntiles, m2l_tile_cols(c), c->lists.far_s.as<int>(), c->Mhl.as<_Float16>(),
-                           c->cscale.as<float>(), (float*)slab, c->L, st);
+                           c->cscale.as<float>(), (float*)slab, c->L, c->queues.get<int>(16), st);
         else
             m2l_gemm(c->prec, ops_ptr(c, O), O.lo.ptr, P.tiles, P.ntiles, c->lists.far_s.as<int>(), c->M.ptr, slab,
                      c->L, c->Lp, st);
@@ -814,7 +815,7 @@ __global__ void k_fill_nan(double* p, size_t n) {
 }
 
 // near field on stream ps (needs the lists and the packed sources)
-void stage_p2p(fmm_ctx* c, hipStream_t ps) {
+void stage_p2p(fmm_ctx* c, hipStream_t ps, bool queue = false) {
     int nts = c->T.num_slots;
     double* Bnear = c->Bnear.get<double>(3 * (size_t)std::max(nts, 1));
     c->rec_on(E_P2P0, ps);
@@ -824,6 +825,12 @@ void stage_p2p(fmm_ctx* c, hipStream_t ps) {
         launch_m2p_close(c->f64(), c->S, c->sigS, c->M.ptr, c->p, c->Lp, c->tgt_leaves.as<int>(),
                          (int)c->h_tgt_leaves.size(), c->T, c->lists.close_begin.as<int>(), c->lists.close_end.as<int>(),
                          c->lists.close_s.as<int>(), c->txyz.as<double>(), Bnear, nullptr, ps);
+    } else if (c->cfg.near_field && queue) {
+        // one persistent workgroup per CU beside the M2L GEMM (see run_fused)
+        launch_p2p_queue(c->f64(), c->tgt_leaves.as<int>(), (int)c->h_tgt_leaves.size(), c->T, c->S,
+                         c->lists.close_begin.as<int>(), c->lists.close_end.as<int>(), c->lists.close_s.as<int>(),
+                         c->nP0.ptr, c->nP1.ptr, c->nP2.ptr, c->txyz.as<double>(), Bnear, c->queues.get<int>(16) + 8,
+                         num_cus(), ps);
     } else if (c->cfg.near_field) {
         launch_p2p(c->f64(), c->tgt_leaves.as<int>(), (int)c->h_tgt_leaves.size(), c->T, c->S, c->lists.close_begin.as<int>(),
                    c->lists.close_end.as<int>(), c->lists.close_s.as<int>(), c->nP0.ptr, c->nP1.ptr, c->nP2.ptr,
@@ -935,7 +942,14 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     trees_begin(c, tc, th);
     std::mutex mu;
     std::condition_variable cv;
-    int src_state = 0, far_state = 0;   // 0 pending, 1 ready, 2 failed
+    int src_state = 0, far_state = 0, pack_state = 0;   // 0 pending, 1 ready, 2 failed
+    // F16X3 with the direct near field and the persistent M2L GEMM (p = 14..18): the P2P runs as a
+    // persistent grid of one workgroup per CU from the moment the close lists and the packed
+    // sources exist, beside the M2M, the tables and the GEMM (whose register budget leaves it
+    // room); otherwise it follows the GEMM (beside the GEMMs of one workgroup per tile it only
+    // slows them)
+    const bool p2p_beside = c->prec == Precision::F16X3 && c->cfg.near_field == FMM_NEAR_P2P &&
+                            m2l_tile_cols(c) == 128 && m2l_f16x3_persistent(c->L);
     std::exception_ptr side_error;
     auto post = [&](int& state, int v) {
         {
@@ -955,6 +969,17 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
                 FMM_HIP(hipStreamWaitEvent(Bs, c->j_src, 0));   // the source tree on stream A
                 stage_lists(c, Bs, c->ev[E_FAR], [&] { post(far_state, 1); });   // E_FAR also joins
                 FMM_HIP(hipEventRecord(c->j_lists, Bs));
+                if (p2p_beside) {
+                    {
+                        std::unique_lock<std::mutex> lk(mu);
+                        cv.wait(lk, [&] { return pack_state != 0; });
+                    }
+                    if (pack_state == 1) {
+                        FMM_HIP(hipStreamWaitEvent(Bs, c->j_pack, 0));
+                        stage_p2p(c, Bs, true);
+                        FMM_HIP(hipEventRecord(c->j_p2p, Bs));
+                    }
+                }
             }
         } catch (...) {
             side_error = std::current_exception();
@@ -969,6 +994,7 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
         c->stage = ST_TREES;
         stage_leaves(c);
         FMM_HIP(hipEventRecord(c->j_pack, A));   // packed near-field sources, for the P2P
+        post(pack_state, 1);
         stage_upward(c);
         {
             std::unique_lock<std::mutex> lk(mu);
@@ -981,6 +1007,7 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
         }
     } catch (...) {
         if (src_state == 0) post(src_state, 2);
+        if (pack_state == 0) post(pack_state, 2);
         side_thread.join();
         throw;
     }
@@ -988,10 +1015,12 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     if (side_error) std::rethrow_exception(side_error);
     FMM_HIP(hipStreamWaitEvent(A, c->j_lists, 0));    // close lists, pair index
     translate_finish(c);
-    FMM_HIP(hipStreamWaitEvent(Bs, c->j_pack, 0));
-    FMM_HIP(hipStreamWaitEvent(Bs, c->j_gemm, 0));
-    stage_p2p(c, Bs);
-    FMM_HIP(hipEventRecord(c->j_p2p, Bs));
+    if (!p2p_beside) {
+        FMM_HIP(hipStreamWaitEvent(Bs, c->j_pack, 0));
+        FMM_HIP(hipStreamWaitEvent(Bs, c->j_gemm, 0));
+        stage_p2p(c, Bs);
+        FMM_HIP(hipEventRecord(c->j_p2p, Bs));
+    }
     stage_l2p(c);
     FMM_HIP(hipStreamWaitEvent(A, c->j_p2p, 0));
     stage_assemble(c, current, Bdev);
@@ -1104,6 +1133,7 @@ int fmm_create(fmm_ctx** out, const fmm_config* cfg) {
         FMM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->own_stream = true;
         c->scratch = scratch_on_current_device();
+        c->queues.get<int>(16);   // allocated once: both host threads of an evaluation use it
         for (int i = 0; i < E_COUNT; i++) FMM_HIP(hipEventCreate(&c->ev[i]));
         c->ev_ok = true;
         FMM_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));   // default priority

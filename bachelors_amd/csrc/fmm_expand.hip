@@ -348,21 +348,18 @@ __global__ void k_pack_sources(int n, const double* __restrict__ origin, const d
 // below 8; lane = target * G + g takes sources g, g + G, ...).  Lane partials are summed over
 // the G lanes, then over the 4 waves in a fixed order, and written once.
 // POT: the vector potential sum_j q_j / |x - y_j| instead of B.
-template <class R, bool POT>
-__global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, const int* __restrict__ tbegin, const int* __restrict__ tend,
-                                             const int* __restrict__ close_begin, const int* __restrict__ close_end,
-                                             const int* __restrict__ close_s, const double* __restrict__ origin,
-                                             const int* __restrict__ sbegin, const int* __restrict__ send,
-                                             const typename V4<R>::t* __restrict__ P0,
-                                             const typename V4<R>::t* __restrict__ P1, const R* __restrict__ P2,
-                                             const double* __restrict__ txyz, double* __restrict__ Bnear) {
+template <class R, bool POT, int UNROLL>
+__device__ __forceinline__ void p2p_leaf(int leaf, const int* __restrict__ tbegin, const int* __restrict__ tend,
+                                         const int* __restrict__ close_begin, const int* __restrict__ close_end,
+                                         const int* __restrict__ close_s, const double* __restrict__ origin,
+                                         const int* __restrict__ sbegin, const int* __restrict__ send,
+                                         const typename V4<R>::t* __restrict__ P0,
+                                         const typename V4<R>::t* __restrict__ P1, const R* __restrict__ P2,
+                                         const double* __restrict__ txyz, double* __restrict__ Bnear,
+                                         typename V4<R>::t (*s0)[64], typename V4<R>::t (*s1)[64], R (*s2)[64],
+                                         double (*red)[3][64]) {
     typedef typename V4<R>::t v4;
     constexpr bool split = sizeof(R) == 4;
-    __shared__ v4 s0[4][64];
-    __shared__ v4 s1[4][64];
-    __shared__ R s2[4][64];
-    __shared__ double red[4][3][64];
-    const int leaf = leaves[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tb = tbegin[leaf], te = tend[leaf];
     const int cb = close_begin[leaf], ce = close_end[leaf];
@@ -406,7 +403,7 @@ __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, con
                 }
                 __syncwarp();
                 R f0 = 0, f1 = 0, f2 = 0;
-#pragma unroll 4
+#pragma unroll UNROLL
                 for (int jj = g; jj < nj; jj += G) {
                     const v4 yh = s0[wave][jj];
                     const v4 yl = s1[wave][jj];
@@ -461,6 +458,46 @@ __global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, con
             }
         }
         __syncthreads();
+    }
+}
+
+template <class R, bool POT>
+__global__ __launch_bounds__(256) void k_p2p(const int* __restrict__ leaves, const int* __restrict__ tbegin,
+                                             const int* __restrict__ tend, const int* __restrict__ close_begin,
+                                             const int* __restrict__ close_end, const int* __restrict__ close_s,
+                                             const double* __restrict__ origin, const int* __restrict__ sbegin,
+                                             const int* __restrict__ send, const typename V4<R>::t* __restrict__ P0,
+                                             const typename V4<R>::t* __restrict__ P1, const R* __restrict__ P2,
+                                             const double* __restrict__ txyz, double* __restrict__ Bnear) {
+    __shared__ typename V4<R>::t s0[4][64];
+    __shared__ typename V4<R>::t s1[4][64];
+    __shared__ R s2[4][64];
+    __shared__ double red[4][3][64];
+    p2p_leaf<R, POT, 4>(leaves[blockIdx.x], tbegin, tend, close_begin, close_end, close_s, origin, sbegin, send, P0, P1, P2, txyz, Bnear, s0, s1, s2, red);
+}
+
+// The same near field as a small persistent grid taking leaves from a queue (an atomic counter
+// that starts at 0), with a register budget of 56 per lane: one workgroup per CU fits beside the
+// F16X3 M2L GEMM's, so the near field runs on the VALU slots the GEMM leaves idle.
+template <class R, bool POT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(10))) void k_p2p_queue(
+    const int* __restrict__ leaves, int nleaves, int* __restrict__ queue, const int* __restrict__ tbegin,
+    const int* __restrict__ tend, const int* __restrict__ close_begin, const int* __restrict__ close_end,
+    const int* __restrict__ close_s, const double* __restrict__ origin, const int* __restrict__ sbegin,
+    const int* __restrict__ send, const typename V4<R>::t* __restrict__ P0, const typename V4<R>::t* __restrict__ P1,
+    const R* __restrict__ P2, const double* __restrict__ txyz, double* __restrict__ Bnear) {
+    __shared__ typename V4<R>::t s0[4][64];
+    __shared__ typename V4<R>::t s1[4][64];
+    __shared__ R s2[4][64];
+    __shared__ double red[4][3][64];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(queue, 1);
+        __syncthreads();
+        const int i = next;
+        __syncthreads();   // every thread has read next before thread 0 takes the following leaf
+        if (i >= nleaves) break;
+        p2p_leaf<R, POT, 1>(leaves[i], tbegin, tend, close_begin, close_end, close_s, origin, sbegin, send, P0, P1, P2, txyz, Bnear, s0, s1, s2, red);
     }
 }
 
@@ -827,6 +864,24 @@ void launch_p2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, cons
                 const int* close_end, const int* close_s, const void* P0, const void* P1, const void* P2,
                 const double* txyz, double* Bnear, hipStream_t st) {
     p2p_launch<false>(f64, leaves, nleaves, T, S, close_begin, close_end, close_s, P0, P1, P2, txyz, Bnear, st);
+}
+void launch_p2p_queue(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S,
+                      const int* close_begin, const int* close_end, const int* close_s, const void* P0, const void* P1,
+                      const void* P2, const double* txyz, double* Bnear, int* queue, int grid, hipStream_t st) {
+    if (!nleaves) return;
+    if (grid < 1) throw Error(1, "near field: empty persistent grid");
+    FMM_HIP(hipMemsetAsync(queue, 0, sizeof(int), st));
+    if (f64)
+        k_p2p_queue<double, false><<<grid, 256, 0, st>>>(leaves, nleaves, queue, T.begin.as<int>(), T.end.as<int>(),
+                                                         close_begin, close_end, close_s, S.center.as<double>(),
+                                                         S.begin.as<int>(), S.end.as<int>(), (const double4*)P0,
+                                                         (const double4*)P1, (const double*)P2, txyz, Bnear);
+    else
+        k_p2p_queue<float, false><<<grid, 256, 0, st>>>(leaves, nleaves, queue, T.begin.as<int>(), T.end.as<int>(),
+                                                        close_begin, close_end, close_s, S.center.as<double>(),
+                                                        S.begin.as<int>(), S.end.as<int>(), (const float4*)P0,
+                                                        (const float4*)P1, (const float*)P2, txyz, Bnear);
+    FMM_HIP(hipGetLastError());
 }
 void launch_p2p_A(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
                   const int* close_end, const int* close_s, const void* P0, const void* P1, const void* P2,

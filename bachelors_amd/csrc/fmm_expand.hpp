@@ -17,6 +17,11 @@ void launch_pack_sources(bool f64, int n, const double* origin, const double* sx
 void launch_p2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S, const int* close_begin,
                 const int* close_end, const int* close_s, const void* P0, const void* P1, const void* P2,
                 const double* txyz, double* Bnear, hipStream_t st);
+// launch_p2p as a persistent grid of `grid` workgroups taking leaves from the counter *queue
+// (reset on the stream first); a small register budget lets it share CUs with the M2L GEMM
+void launch_p2p_queue(bool f64, const int* leaves, int nleaves, const TreeDev& T, const TreeDev& S,
+                      const int* close_begin, const int* close_end, const int* close_s, const void* P0, const void* P1,
+                      const void* P2, const double* txyz, double* Bnear, int* queue, int grid, hipStream_t st);
 void launch_assemble(const int* slots, int n, const double* Bfar, const double* Bnear, double scale, double* B,
                      hipStream_t st);
 // L2P / P2P of the vector potential A (the calcA counterparts of launch_l2p / launch_p2p)

@@ -567,14 +567,15 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3(const _Float16* __restri
 // 16 NB0 x 64 instead of 16 NB x 32: about a third less LDS traffic per MFMA).  Staging, numerics
 // and epilogue as k_gemm_m2l_f16x3.
 template <int NB, int NKS, int BD>
-__global__ __launch_bounds__(256) void k_gemm_m2l_f16x3_q(const _Float16* __restrict__ ops_hi,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_num_vgpr(216))) void k_gemm_m2l_f16x3_q(
+                                                          const _Float16* __restrict__ ops_hi,
                                                           const _Float16* __restrict__ ops_lo,
                                                           const float* __restrict__ ops_rinv,
                                                           const GemmTile* __restrict__ tiles,
                                                           const int* __restrict__ pair_src,
                                                           const _Float16* __restrict__ src_hl,
                                                           const float* __restrict__ cscale, float* __restrict__ slab,
-                                                          int op_base, int ntiles) {
+                                                          int op_base, int ntiles, int* __restrict__ xq) {
     constexpr int R = 16 * NB, Kp = 32 * NKS, BK = 32, S = BK + 8;   // LDS row stride: 80 bytes
     constexpr int NQ = (4 * R + 255) / 256;                          // A: 8-half vectors per thread and part
     constexpr int W = 128, NB0 = (NB + 1) / 2;
@@ -583,13 +584,20 @@ __global__ __launch_bounds__(256) void k_gemm_m2l_f16x3_q(const _Float16* __rest
     __shared__ __attribute__((aligned(16))) _Float16 Bh[2][W][S];
     __shared__ __attribute__((aligned(16))) _Float16 Bl[2][W][S];
     // persistent: the grid (a multiple of 8, workgroup b on XCD b mod 8) walks the tiles, each
-    // XCD a contiguous eighth of them (one class's operator shared in one L2); a workgroup
-    // starts its next tile while its last stores drain
+    // XCD a contiguous eighth of them (one class's operator shared in one L2) taken from that
+    // XCD's counter xq[xcd] (zero at launch), so a workgroup that starts late (its CU busy)
+    // takes fewer tiles; a workgroup starts its next tile while its last stores drain.  The
+    // register budget leaves room for one near-field workgroup (k_p2p_queue) per CU.
     const int xcd = blockIdx.x & 7, q = ntiles >> 3, rem = ntiles & 7;
-    const int xbeg = xcd * q + min(xcd, rem), xcnt = q + (xcd < rem ? 1 : 0), per_xcd = gridDim.x >> 3;
+    const int xbeg = xcd * q + min(xcd, rem), xcnt = q + (xcd < rem ? 1 : 0);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, li = lane & 15;
     const int wr = wave & 1, wc = wave >> 1, rb0 = wr * NB0;
-    for (int it = blockIdx.x >> 3; it < xcnt; it += per_xcd) {
+    __shared__ int next_tile;
+    for (;;) {
+        if (tid == 0) next_tile = atomicAdd(xq + xcd, 1);
+        __syncthreads();
+        const int it = next_tile;   // read before the K loop's barriers; rewritten only after them
+        if (it >= xcnt) break;
         const GemmTile tile = tiles[xbeg + it];
         const int op = tile.op - op_base;
         const _Float16* Ahg = ops_hi + (size_t)op * R * Kp;
@@ -770,20 +778,25 @@ void split_columns(const float* M, int ncols, int L, int Lp, int Kp, const float
 }
 
 int m2l_f16_rows(int L) { return 16 * ((L + 15) / 16); }
+bool m2l_f16x3_persistent(int L) {
+    const int nb = m2l_f16_rows(L) / 16;
+    return nb >= 14 && nb <= 23;   // the 128-column tiles of launch_f16x3_128 below
+}
 int m2l_f16_kpad(int L) { return 32 * ((L + 31) / 32); }
 
 // 128-column tiles: the double-buffered 2 x 2 kernel where the single-buffered one's
 // accumulators leave one workgroup per CU anyway (R >= 224, p >= 13) and its two LDS buffers fit
-// (R <= 384, p <= 18); the single-buffered kernel (two or more workgroups per CU below) otherwise
+// (R <= 368, p <= 18); the single-buffered kernel (two or more workgroups per CU below) otherwise
 template <int NB, int NKS>
 static void launch_f16x3_128(int ntiles, const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv,
                              const GemmTile* d_tiles, const int* pair_src, const _Float16* src_hl,
-                             const float* cscale, float* slab, int op_base, hipStream_t st) {
-    if constexpr (NB >= 14 && NB <= 24) {
+                             const float* cscale, float* slab, int op_base, int* queues, hipStream_t st) {
+    if constexpr (NB >= 14 && NB <= 23) {
         // one workgroup per CU (the two LDS buffers), 8 per XCD round
         const int grid = std::min(8 * ((ntiles + 7) / 8), 8 * ((num_cus() + 7) / 8));
+        FMM_HIP(hipMemsetAsync(queues, 0, 8 * sizeof(int), st));
         k_gemm_m2l_f16x3_q<NB, NKS, 2><<<grid, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hl,
-                                                             cscale, slab, op_base, ntiles);
+                                                             cscale, slab, op_base, ntiles, queues);
     }
     else
         k_gemm_m2l_f16x3<NB, NKS, 2><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hl,
@@ -792,7 +805,7 @@ static void launch_f16x3_128(int ntiles, const _Float16* ops_hi, const _Float16*
 
 void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv, int op_base,
                     const GemmTile* d_tiles, int ntiles, int tile_cols, const int* pair_src, const _Float16* src_hl,
-                    const float* cscale, float* slab, int L, hipStream_t st) {
+                    const float* cscale, float* slab, int L, int* queues, hipStream_t st) {
     if (ntiles == 0) return;
     const int nb = m2l_f16_rows(L) / 16, nks = m2l_f16_kpad(L) / 32;
     if (tile_cols != 64 && tile_cols != 128) throw Error(4, "FP16 M2L: tiles of 64 or 128 columns");
@@ -800,7 +813,7 @@ void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float*
     if (nb == NBV && nks == NKV) {                                                                                   \
         if (tile_cols == 128)                                                                                        \
             launch_f16x3_128<NBV, NKV>(ntiles, ops_hi, ops_lo, ops_rinv, d_tiles, pair_src, src_hl, cscale, slab,    \
-                                       op_base, st);                                                                 \
+                                       op_base, queues, st);                                                         \
         else                                                                                                         \
             k_gemm_m2l_f16x3<NBV, NKV, 1><<<ntiles, 256, 0, st>>>(ops_hi, ops_lo, ops_rinv, d_tiles, pair_src,       \
                                                                   src_hl, cscale, slab, op_base);                    \

@@ -60,14 +60,17 @@ void m2l_gemm(Precision prec, const void* ops, const void* ops_lo, const GemmTil
 // (src_hl [node * 3 + comp][Kp / 32][hi 32 | lo 32], equilibrated by dm[>= Kp], the per-coefficient
 // scales the tables were built with, and by cscale[node * 3 + comp]); the slab rows are
 // m2l_f16_rows(L) long.  Tile operator k is table entry k - op_base; tiles hold at most tile_cols
-// (64 or 128) columns.
+// (64 or 128) columns.  queues: 8 device ints, the per-XCD tile counters of the persistent kernel.
 int m2l_f16_rows(int L);
+// whether the F16X3 M2L of order (L = (p + 1)^2) runs as the persistent one-workgroup-per-CU kernel
+// (128-column tiles) that leaves room for a near-field workgroup per CU
+bool m2l_f16x3_persistent(int L);
 int m2l_f16_kpad(int L);
 void split_columns(const float* M, int ncols, int L, int Lp, int Kp, const float* dm, float* scale, _Float16* hl,
                    hipStream_t st);
 void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv, int op_base,
                     const GemmTile* d_tiles, int ntiles, int tile_cols, const int* pair_src, const _Float16* src_hl,
-                    const float* cscale, float* slab, int L, hipStream_t st);
+                    const float* cscale, float* slab, int L, int* queues, hipStream_t st);
 
 // tiles of at most width columns (default 64) over the classes' columns 3 * class_begin
 void plan_tiles(const std::vector<int>& class_begin, const std::vector<int>& class_op, std::vector<GemmTile>& tiles,
