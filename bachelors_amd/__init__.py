@@ -79,12 +79,31 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
+def _share_torch_hip_runtime():
+    """torch ships its own copy of the HIP runtime under the same soname (libamdhip64.so.7) and
+    loads it by file name.  Loaded first, that copy also serves this library (the dynamic loader
+    resolves the soname to it); loaded after the system copy, it is a second runtime in the
+    process, and torch then reports no HIP device.  So the torch copy, where torch is installed,
+    is loaded before this library, without importing torch."""
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    for d in (spec.submodule_search_locations or []) if spec else []:
+        path = os.path.join(d, "lib", "libamdhip64.so")
+        if os.path.exists(path):
+            ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+            return
+
+
 def lib():
     """Load the HIP library (raises if it has not been built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise FMMError("HIP library missing: %s (run python -m bachelors_amd.build)" % LIB_PATH)
+        _share_torch_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         L.fmm_last_error.restype = ctypes.c_char_p
         L.fmm_last_error.argtypes = [_vp]

@@ -73,3 +73,17 @@ def test_cpp_drop_in_header_compiles_and_links():
     d = os.path.join(ROOT, "tests", "cpp")
     subprocess.run(["make", "-s", "-B", "-C", d], check=True)
     assert os.path.exists(os.path.join(d, "drop_in_example"))
+
+
+def test_one_hip_runtime_with_torch():
+    """The library loaded before torch and torch imported after it map one HIP runtime between
+    them (torch's bundled copy), in a fresh process: two copies in one process leave torch
+    without a device."""
+    import subprocess
+    import sys
+    code = ("import sys, re; sys.path.insert(0, '.'); import bachelors_amd as fm; fm.lib(); import torch; "
+            "print(sorted(set(re.findall(r'(/\\S*libamdhip64\\S*)', open('/proc/self/maps').read()))))")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    maps = eval(r.stdout.strip().splitlines()[-1])
+    assert len(maps) == 1, maps

@@ -73,3 +73,19 @@ def test_config5_gaussian_clusters_p14(orc):
     assert rel_l2(B32, B64) < 3e-6
     assert rel_l2(Bbf, B64) < 2e-5
     assert rel_l2(B16, B64) < 3e-6
+
+
+@pytest.mark.parametrize("order", [13, 14, 15, 16, 17, 18])
+def test_f16x3_orders_13_to_18_match_fp64(order):
+    """The F16X3 M2L instances of the high orders against the FP64 FMM on the same trees: p = 13
+    (single-buffered kernel), 15 and 18 (persistent kernel, K padded to 32), 14 and 16 (persistent,
+    the last coefficient as an FP32 rank-1 update in the epilogue) and 17 (the last four)."""
+    import bachelors_amd as fm
+    rng = np.random.default_rng(20261202 + 60 + order)
+    src, w = coaxial_shells(rng, 4096)
+    tgt = rng.uniform(-2, 2, (4096, 3))
+    cfg = dict(order=order, source_leaf_capacity=32, target_leaf_capacity=32, frame=1)
+    _, B64 = _gpu(fm, src, w, tgt, precision="fp64", **cfg)
+    _, B16 = _gpu(fm, src, w, tgt, precision="f16x3", **cfg)
+    assert np.isfinite(B16).all()
+    assert rel_l2(B16, B64) < 3e-6
