@@ -942,12 +942,14 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     trees_begin(c, tc, th);
     std::mutex mu;
     std::condition_variable cv;
-    int src_state = 0, far_state = 0, pack_state = 0;   // 0 pending, 1 ready, 2 failed
+    int src_state = 0, far_state = 0, gemm_state = 0;   // 0 pending, 1 ready, 2 failed
     // F16X3 with the direct near field and the persistent M2L GEMM (p = 14..18): the P2P runs as a
-    // persistent grid of one workgroup per CU from the moment the close lists and the packed
-    // sources exist, beside the M2M, the tables and the GEMM (whose register budget leaves it
-    // room); otherwise it follows the GEMM (beside the GEMMs of one workgroup per tile it only
-    // slows them)
+    // persistent grid of one workgroup per CU beside the GEMM (whose register budget leaves it
+    // room), launched when the GEMM's inputs are complete (E_GEMM0, after the column split):
+    // launched earlier, beside the M2M and the tables, its workgroups can land two to a CU, and
+    // such a CU has no room for a GEMM workgroup until they finish (measured: the GEMM 1.4 ms
+    // slower).  Otherwise the P2P follows the GEMM (beside the GEMMs of one workgroup per tile it
+    // only slows them).
     const bool p2p_beside = c->prec == Precision::F16X3 && c->cfg.near_field == FMM_NEAR_P2P &&
                             m2l_tile_cols(c) == 128 && m2l_f16x3_persistent(c->L);
     std::exception_ptr side_error;
@@ -972,10 +974,10 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
                 if (p2p_beside) {
                     {
                         std::unique_lock<std::mutex> lk(mu);
-                        cv.wait(lk, [&] { return pack_state != 0; });
+                        cv.wait(lk, [&] { return gemm_state != 0; });
                     }
-                    if (pack_state == 1) {
-                        FMM_HIP(hipStreamWaitEvent(Bs, c->j_pack, 0));
+                    if (gemm_state == 1) {   // E_GEMM0 of this evaluation is recorded (it follows j_pack)
+                        FMM_HIP(hipStreamWaitEvent(Bs, c->ev[E_GEMM0], 0));
                         stage_p2p(c, Bs, true);
                         FMM_HIP(hipEventRecord(c->j_p2p, Bs));
                     }
@@ -994,7 +996,6 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
         c->stage = ST_TREES;
         stage_leaves(c);
         FMM_HIP(hipEventRecord(c->j_pack, A));   // packed near-field sources, for the P2P
-        post(pack_state, 1);
         stage_upward(c);
         {
             std::unique_lock<std::mutex> lk(mu);
@@ -1004,10 +1005,13 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
             FMM_HIP(hipStreamWaitEvent(A, c->ev[E_FAR], 0));   // far pairs and classes
             translate_tables(c, A);
             translate_gemm(c);
+            post(gemm_state, 1);
+        } else {
+            post(gemm_state, 2);
         }
     } catch (...) {
         if (src_state == 0) post(src_state, 2);
-        if (pack_state == 0) post(pack_state, 2);
+        if (gemm_state == 0) post(gemm_state, 2);
         side_thread.join();
         throw;
     }
