@@ -252,7 +252,8 @@ def test_expansions_fp64(fm, orc, small_case):
     assert checked > 20
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 3e-6), ("bf16x3", 1e-4), ("f16x3", 3e-6)])
+# measured: fp32 2.2e-7, bf16x3 9.6e-7, f16x3 2.1e-7
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-6), ("bf16x3", 1e-5), ("f16x3", 1e-6)])
 def test_B_matches_oracle(fm, orc, small_case, precision, tol):
     src, w, tgt, R = small_case
     F, B = _gpu_run(fm, src, w, tgt, 8, 32, 32, precision)
