@@ -710,9 +710,10 @@ void translate_tables(fmm_ctx* c, hipStream_t st) {
         P.width = m2l_tile_cols(c);
         P.tiles = (const GemmTile*)(sg.send(st) + o_tiles);
     }
+    // the pair reduction writes every local expansion whole; cleared here only when it will not run
     size_t n = (size_t)T.num_nodes * 3 * c->Lp;
     void* Lx = c->Lx.reserve(n * c->esize());
-    FMM_HIP(hipMemsetAsync(Lx, 0, n * c->esize(), st));
+    if (!P.ntiles) FMM_HIP(hipMemsetAsync(Lx, 0, n * c->esize(), st));
     c->rec_on(E_TABLES, st);
 }
 

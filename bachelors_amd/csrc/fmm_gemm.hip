@@ -995,9 +995,9 @@ void build_pair_index(const int* key, int npairs, int nnodes, PairIndex& ix, hip
 
 // Two passes, so that nodes with hundreds of pairs do not serialise the launch: (1) one wave
 // per segment (at most PAIR_SEGMENT consecutive pairs of one node) sums its pairs' slab blocks
-// in ascending pair order into a partial block; (2) one wave per node adds its segments'
-// partials, in segment order, to the node's expansion.  The order of every sum is fixed by the
-// pair numbering alone (deterministic).  A pair's slab block is nv = 3 vr 16-byte vectors (vr per
+// in ascending pair order into a partial block; (2) one wave per node sums its segments'
+// partials, in segment order, and writes the node's expansion.  The order of every sum is fixed
+// by the pair numbering alone (deterministic).  A pair's slab block is nv = 3 vr 16-byte vectors (vr per
 // component: the first R of each component's Lp coefficients); lane l owns the vectors
 // l + 64 r, r < NR = ceil(nv / 64); pass (1) keeps the loads of U pairs in flight.
 __device__ __forceinline__ float4 zero_vec(float4) { return make_float4(0.f, 0.f, 0.f, 0.f); }
@@ -1061,16 +1061,22 @@ __global__ __launch_bounds__(256) void k_reduce_nodes(const uint32_t* __restrict
     const int v = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (v >= nnodes) return;
     const int s0 = (int)seg_off[v], s1 = (int)seg_off[v + 1];
-    if (s0 == s1) return;
     const int nv = 3 * vr;
     V* o = out + (size_t)v * 3 * vl;
+    // the node's expansion is written whole (nothing clears it beforehand): the padding vectors
+    // vr .. vl - 1 of each component, and all of a node without far pairs, are zero
+    for (int i = lane; i < 3 * vl; i += 64) {
+        const int c = i / vl;
+        if (s0 == s1 || i - c * vl >= vr) o[i] = zero_vec(V());
+    }
+    if (s0 == s1) return;
     // the loads of four segments are issued together; the adds keep the segment order
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         const int i = lane + 64 * r;
         if (i >= nv) continue;
         const int c = i / vr, q = c * vl + (i - c * vr);
-        V acc = o[q];
+        V acc = zero_vec(V());
         const V* pp = partial + (size_t)s0 * nv + i;
         int sg = s0;
         for (; sg + 4 <= s1; sg += 4, pp += 4 * (size_t)nv) {

@@ -44,8 +44,10 @@ struct PairIndex {
 };
 void build_pair_index(const int* key, int npairs, int nnodes, PairIndex& ix, hipStream_t st);
 
-// out[v][c][0 .. R) += sum over the pairs of v, in ascending pair order, of slab[pair][c][0 .. R)
-// (slab = a per-pair store with row length R <= Lp, multiple of 4); double if f64 else float
+// out[v][c][0 .. R) = sum over the pairs of v, in ascending pair order, of slab[pair][c][0 .. R),
+// out[v][c][R .. Lp) = 0, for every node v < ix.nnodes (written, not added: nothing needs to clear
+// out first); slab = a per-pair store with row length R <= Lp, multiple of 4; double if f64 else
+// float
 void reduce_pairs(bool f64, PairIndex& ix, const void* slab, int R, int Lp, void* out, hipStream_t st);
 
 // Row length of the M2L slab: L padded to 16 (MFMA block) where a kernel instance exists for it
