@@ -93,7 +93,10 @@ def _share_torch_hip_runtime():
     for d in (spec.submodule_search_locations or []) if spec else []:
         path = os.path.join(d, "lib", "libamdhip64.so")
         if os.path.exists(path):
-            ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+            try:
+                ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+            except OSError:
+                pass   # unusable copy: the library then loads the system runtime
             return
 
 
