@@ -946,11 +946,11 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     int src_state = 0, far_state = 0, gemm_state = 0;   // 0 pending, 1 ready, 2 failed
     // F16X3 with the direct near field and the persistent M2L GEMM (p = 14..18): the P2P runs as a
     // persistent grid of one workgroup per CU beside the GEMM (whose register budget leaves it
-    // room), launched when the GEMM's inputs are complete (E_GEMM0, after the column split):
-    // launched earlier, beside the M2M and the tables, its workgroups can land two to a CU, and
-    // such a CU has no room for a GEMM workgroup until they finish (measured: the GEMM 1.4 ms
-    // slower).  Otherwise the P2P follows the GEMM (beside the GEMMs of one workgroup per tile it
-    // only slows them).
+    // room), launched when the GEMM's inputs are complete (E_GEMM0, after the column split).
+    // Launched earlier, beside the M2M and the tables, it left the GEMM up to 1.4 ms slower --
+    // most likely because workgroups placed on a busy chip can leave a CU without room for a
+    // GEMM workgroup.  Otherwise the P2P follows the GEMM (beside the GEMMs of one workgroup per
+    // tile it only slows them).
     const bool p2p_beside = c->prec == Precision::F16X3 && c->cfg.near_field == FMM_NEAR_P2P &&
                             m2l_tile_cols(c) == 128 && m2l_f16x3_persistent(c->L);
     std::exception_ptr side_error;
