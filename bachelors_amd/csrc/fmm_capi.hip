@@ -721,7 +721,7 @@ void translate_tables(fmm_ctx* c, hipStream_t st) {
 // (fast_multipole_solver.cpp:258-276) writing one slab per (pair, component); each target node
 // then sums its pairs' slabs in pair order (part 3: no atomics, deterministic).  E_GEMM0 ..
 // E_M2L_GEMM brackets the GEMM alone.
-void translate_gemm(fmm_ctx* c) {
+void translate_gemm(fmm_ctx* c, hipEvent_t gemm_after = nullptr) {
     hipStream_t st = c->stream;
     const OpTables& O = c->down_ops;
     const Pass& P = c->m2l_pass;
@@ -731,6 +731,7 @@ void translate_gemm(fmm_ctx* c) {
         split_columns(c->M.as<float>(), ncols, c->L, c->Lp, kp, c->dm.as<float>(), c->cscale.get<float>(ncols),
                       c->Mhl.get<_Float16>((size_t)2 * ncols * kp), st);
     }
+    if (gemm_after) FMM_HIP(hipStreamWaitEvent(st, gemm_after, 0));
     c->rec(E_GEMM0);
     if (P.ntiles) {
         const int R = m2l_slab_rows(c->prec, c->L, c->Lp);
@@ -943,7 +944,7 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     trees_begin(c, tc, th);
     std::mutex mu;
     std::condition_variable cv;
-    int src_state = 0, far_state = 0, gemm_state = 0;   // 0 pending, 1 ready, 2 failed
+    int src_state = 0, far_state = 0, lists_state = 0, gemm_state = 0;   // 0 pending, 1 ready, 2 failed
     // F16X3 with the direct near field and the persistent M2L GEMM (p = 14..18): the P2P runs as a
     // persistent grid of one workgroup per CU beside the GEMM (whose register budget leaves it
     // room), launched when the GEMM's inputs are complete (E_GEMM0, after the column split).
@@ -972,6 +973,7 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
                 FMM_HIP(hipStreamWaitEvent(Bs, c->j_src, 0));   // the source tree on stream A
                 stage_lists(c, Bs, c->ev[E_FAR], [&] { post(far_state, 1); });   // E_FAR also joins
                 FMM_HIP(hipEventRecord(c->j_lists, Bs));
+                post(lists_state, 1);
                 if (p2p_beside) {
                     {
                         std::unique_lock<std::mutex> lk(mu);
@@ -988,6 +990,7 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
             side_error = std::current_exception();
         }
         if (far_state == 0) post(far_state, 2);
+        if (lists_state == 0) post(lists_state, 2);
     });
     try {
         build_source_side(c);
@@ -1005,7 +1008,14 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
         if (far_state == 1) {
             FMM_HIP(hipStreamWaitEvent(A, c->ev[E_FAR], 0));   // far pairs and classes
             translate_tables(c, A);
-            translate_gemm(c);
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return lists_state != 0; });
+            }
+            // the GEMM starts after the lists' last kernels (the pair index sort): beside the
+            // persistent GEMM a sort whose blocks wait on each other crawled (17 ms instead of
+            // 0.1), holding back the near field queued after it; normally the lists end first
+            translate_gemm(c, lists_state == 1 ? c->j_lists : nullptr);
             post(gemm_state, 1);
         } else {
             post(gemm_state, 2);
