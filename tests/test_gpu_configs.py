@@ -89,3 +89,29 @@ def test_f16x3_orders_13_to_18_match_fp64(order):
     _, B16 = _gpu(fm, src, w, tgt, precision="f16x3", **cfg)
     assert np.isfinite(B16).all()
     assert rel_l2(B16, B64) < 1e-6      # measured 6.3e-8 .. 9.3e-8 over the six orders
+
+
+@pytest.mark.parametrize("ns,nt", [(40, 10), (300, 120), (3000, 500)])
+def test_f16x3_p16_fused_small_inputs(orc, ns, nt):
+    """The fused evaluation at p = 16 in F16X3 (persistent M2L GEMM, near field beside it) on inputs
+    so small that there are no far pairs at all (40 sources: one leaf, the near field only) or
+    only a few tiles, against the FP64 direct sum and against the staged evaluation."""
+    import ctypes
+    import torch
+    import bachelors_amd as fm
+    rng = np.random.default_rng(20261202 + 80 + ns)
+    src, w = coaxial_shells(rng, ns // 4)
+    tgt = rng.uniform(-2, 2, (nt, 3))
+    F = fm.FMM(order=16, source_leaf_capacity=64, target_leaf_capacity=64, precision="f16x3", frame=1)
+    F.set_sources(src, w)
+    F.set_targets(tgt)
+    B_staged = F.run(1.0)
+    out = torch.empty((nt, 3), dtype=torch.float64, device="cuda")
+    F.run_B_dev(1.0, ctypes.c_void_p(out.data_ptr()))
+    F.synchronize()
+    B = out.cpu().numpy()
+    assert np.isfinite(B).all()
+    assert rel_l2(B, B_staged) < 1e-6
+    assert rel_l2(B, orc.direct_B(src, w, tgt)) < 1e-6
+    if ns == 40:
+        assert F.stats()["far_pairs"] == 0
