@@ -225,6 +225,12 @@ def run_workload(fm, torch, name, args, with_fp64):
         "roofline": m2l_roofline(st, L, args.precision, committed_traffic(name, args.precision, p)),
         "other_stages": other_stage_rooflines(staged, L, ns, nt, args.precision),
     }
+    if staged.get("t_m2l_gemm", 0) > 0:
+        # the same GEMM launch in the stage-by-stage runs, with nothing beside it (in the timed
+        # fused evaluation the near field may share the chip with it)
+        alone = m2l_roofline(staged, L, args.precision, None)
+        out["roofline"]["alone"] = {"launch_ms": alone["launch_ms"], "achieved": alone["achieved"],
+                                    "frac": alone["frac"]}
     if with_fp64:
         print("bench: %s: timing fp64" % name, file=sys.stderr, flush=True)
         ms64, st64, _, err64, fin64 = time_fmm(fm, torch, "fp64", d_src, d_w, d_tgt, p, cap, args.steps,
