@@ -33,6 +33,24 @@ def test_adjoint_is_the_transpose(precision, tol):
     assert abs(lhs - rhs) <= tol * abs(lhs)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32", 2e-6), ("f16x3", 2e-6)])
+def test_adjoint_is_the_transpose_p16(precision, tol):
+    """The same identity at p = 16 (the order of the 4M workload; in F16X3 the forward runs the
+    persistent M2L GEMM with the near field beside it), with g = B(w) (1 + U[-1/2, 1/2]) so that
+    g.B(w) >= |B(w)|^2 / 2 and the quotient is not a random-sign cancellation."""
+    import torch
+    from bachelors_amd.torch_fmm import FMMBiotSavart
+    rng, src, w, tgt = _case(23, ns=8000, nt=2000)
+    F = FMMBiotSavart(src, tgt, order=16, cap=64, precision=precision)
+    wt = torch.tensor(w, dtype=torch.float64, device="cuda", requires_grad=True)
+    B = F(wt)
+    g = B.detach() * (1 + torch.tensor(rng.uniform(-0.5, 0.5, (len(tgt), 3)), device="cuda"))
+    (B * g).sum().backward()
+    lhs = float((B * g).sum())
+    rhs = float((wt.detach() * wt.grad).sum())
+    assert abs(lhs - rhs) <= tol * abs(lhs)
+
+
 def test_adjoint_gradient_vs_finite_differences_and_direct():
     import torch
     from bachelors_amd.torch_fmm import FMMBiotSavart
