@@ -922,9 +922,10 @@ void direct_sum(fmm_ctx* c, bool pot, double current, size_t m, const double* xy
 }
 
 // The whole evaluation with the side stream: the lists are built (on a host thread, since
-// their compaction reads counts back) while P2M and M2M run, and the near field runs after the
-// M2L GEMM, beside the memory-bound slab reduction, L2L and L2P; the streams join before the
-// assembly.  The reference frame's per-pair M2L has no GEMM event split and runs staged.
+// their compaction reads counts back) while P2M and M2M run, and the near field runs beside the
+// M2L GEMM (persistent F16X3 GEMM) or after it, beside the memory-bound slab reduction, L2L and
+// L2P; the streams join before the assembly.  The reference frame's per-pair M2L has no GEMM
+// event split and runs staged.
 void run_fused(fmm_ctx* c, double current, double* Bdev) {
     if (c->cfg.frame != FMM_FRAME_SHARED) {
         stage_trees(c);
@@ -938,7 +939,8 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     // Side thread (stream Bs): target tree, targets in slot order, then -- once the source tree
     // is built -- the far lists (E_FAR), the close lists and the pair index (j_lists).  This
     // thread (stream A): source tree, charges, P2M, M2M, then -- once the far classes are known
-    // on the host -- the operator tables and the M2L GEMM.  Neither thread blocks on the other's
+    // on the host -- the operator tables (or, below, the side stream's) and the M2L GEMM.
+    // Neither thread blocks on the other's
     // GPU work except through events; the side thread only waits for the counts it reads back.
     double tc[3], th[3];
     trees_begin(c, tc, th);
@@ -954,6 +956,11 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     // tile it only slows them).
     const bool p2p_beside = c->prec == Precision::F16X3 && c->cfg.near_field == FMM_NEAR_P2P &&
                             m2l_tile_cols(c) == 128 && m2l_f16x3_persistent(c->L);
+    // With the near field beside the GEMM the M2L tables are built on the side stream after the
+    // lists (beside the M2M; the near field queued there waits for the GEMM anyway), and the GEMM
+    // waits for them.  Otherwise they run on the main stream once the far classes are known.
+    const bool tables_beside = p2p_beside;
+    int tables_state = 0;
     std::exception_ptr side_error;
     auto post = [&](int& state, int v) {
         {
@@ -974,6 +981,11 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
                 stage_lists(c, Bs, c->ev[E_FAR], [&] { post(far_state, 1); });   // E_FAR also joins
                 FMM_HIP(hipEventRecord(c->j_lists, Bs));
                 post(lists_state, 1);
+                if (tables_beside) {
+                    translate_tables(c, Bs);
+                    FMM_HIP(hipEventRecord(c->j_tables, Bs));   // follows j_lists
+                    post(tables_state, 1);
+                }
                 if (p2p_beside) {
                     {
                         std::unique_lock<std::mutex> lk(mu);
@@ -991,6 +1003,7 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
         }
         if (far_state == 0) post(far_state, 2);
         if (lists_state == 0) post(lists_state, 2);
+        if (tables_state == 0) post(tables_state, 2);
     });
     try {
         build_source_side(c);
@@ -1006,17 +1019,26 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
             cv.wait(lk, [&] { return far_state != 0; });
         }
         if (far_state == 1) {
-            FMM_HIP(hipStreamWaitEvent(A, c->ev[E_FAR], 0));   // far pairs and classes
-            translate_tables(c, A);
+            if (!tables_beside) {
+                FMM_HIP(hipStreamWaitEvent(A, c->ev[E_FAR], 0));   // far pairs and classes
+                translate_tables(c, A);
+            }
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv.wait(lk, [&] { return lists_state != 0; });
+                cv.wait(lk, [&] { return lists_state != 0 && (!tables_beside || tables_state != 0); });
             }
             // the GEMM starts after the lists' last kernels (the pair index sort): beside the
             // persistent GEMM a sort whose blocks wait on each other crawled (17 ms instead of
             // 0.1), holding back the near field queued after it; normally the lists end first
-            translate_gemm(c, lists_state == 1 ? c->j_lists : nullptr);
-            post(gemm_state, 1);
+            if (!tables_beside) {
+                translate_gemm(c, lists_state == 1 ? c->j_lists : nullptr);
+                post(gemm_state, 1);
+            } else if (tables_state == 1) {
+                translate_gemm(c, c->j_tables);
+                post(gemm_state, 1);
+            } else {
+                post(gemm_state, 2);   // the side thread's error is rethrown after the join
+            }
         } else {
             post(gemm_state, 2);
         }
