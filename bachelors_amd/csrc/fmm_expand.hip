@@ -55,67 +55,80 @@ __global__ void k_gather_xyz(const int* __restrict__ slots, int n, const double*
 // ---------------------------------------------------------------- P2M
 // M_l^{+-m} = sum_j q_j R_l^{+-m}(u_j), u = (y - c) / sigma, with R(l, 0) = (-1)^l Phi_l^0 and
 // R(l, +-m) = (-1)^l sqrt2 {Re, Im} Phi_l^m (harmonics.cpp:281-320).  One wave (a one-wave
-// workgroup) per useful source leaf.  The leaf's tasks (point, m) -- point-major, p + 1 per point
-// -- run 64 at a time; task (point, m) runs column m of the recurrences, the sectoral chain
-// Phi_m^m = (x + iy)^m / (2^m m!) and then Phi_l^m = ((2l - 1) z Phi_{l-1}^m - r^2 Phi_{l-2}^m) /
-// (l^2 - m^2) up to l = p (p + 1 steps for every m, so the lanes stay in step), and writes Phi to
-// the point's row of a small LDS ring.  After each round of 64 tasks the points it completed are
-// added, lane k owning coefficients k, k + 64, ..., in point order with FP64 sums; the degree
-// signs and sqrt2 are applied once when the leaf is written.  The ring holds the points of one
-// round and its two boundary points (64 / (p + 1) + 2 rows).  T is the type of the recurrences
-// and the ring: FP64 in the FP64 mode, FP32 otherwise (|u| <= sqrt3 keeps every Phi_l^m in range
-// and the recurrences forward-stable).
+// workgroup) per useful source leaf.  The leaf's tasks (point, m) -- point-major, P + 1 per point
+// -- run 64 at a time; task (point, m) makes column m of the harmonics in one loop over the
+// degree l = 1 .. P (P, the order, is a template parameter, so the loop unrolls with l fixed):
+// for l <= m the sectoral step Phi_l^l = Phi_{l-1}^{l-1} (x + iy) / (2l), for l > m the degree
+// step Phi_l^m = ((2l - 1) z Phi_{l-1}^m - r^2 Phi_{l-2}^m) / (l^2 - m^2), both as
+// (A Phi_{l-1} - b Phi_{l-2}) f with a complex A = (x + iy or (2l - 1) z), b = (0 up to l = m + 1,
+// then r^2) and f from an LDS table F[m][l] (1/(2l) for l <= m, 1/(l^2 - m^2) above), so every
+// lane runs the same P steps whatever its m; from l = m on each value is written to the point's
+// row of a small LDS ring.  After each round the points it completed are added, lane k owning
+// coefficients k, k + 64, ... (ring rows padded to whole slots of 64, so the reads need no
+// masks), in point order with FP64 fused multiply-adds; the degree signs and sqrt2 are applied
+// once when the leaf is written.  The ring holds the points of one round and its two boundary
+// points (64 / (P + 1) + 2 rows).  T is the type of the recurrences and the ring: FP64 in the
+// FP64 mode, FP32 otherwise (|u| <= sqrt3 keeps every Phi_l^m in range and the recurrences
+// forward-stable).
 template <class T> __device__ __forceinline__ T recip_t(int n);
 template <> __device__ __forceinline__ double recip_t<double>(int n) { return recip(n); }
 template <> __device__ __forceinline__ float recip_t<float>(int n) { return recipf(n); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-template <class T>
+template <int P>
+constexpr int p2m_ring() { return 64 / (P + 1) + 2; }
+template <int P>
+constexpr int p2m_row() { return 64 * (((P + 1) * (P + 1) + 63) / 64); }
+
+template <class T, int P>
 __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, const double* __restrict__ center,
                                             const int* __restrict__ begin, const int* __restrict__ end,
                                             const int* __restrict__ level, ScaleTable sig,
-                                            const double* __restrict__ sxyz, const double* __restrict__ sq, int p,
-                                            int Lp, T* __restrict__ M) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char p2m_lds[];
-    T* ring = (T*)p2m_lds;   // [RING][L] complex Phi of the points in flight (Re at (l, m), Im at (l, -m))
-    constexpr int KK = (MAX_ORDER + 1) * (MAX_ORDER + 1) / 64 + 1;   // coefficients per lane
-    const int P1 = p + 1, L = P1 * P1, RING = 64 / P1 + 2, lane = threadIdx.x;
+                                            const double* __restrict__ sxyz, const double* __restrict__ sq, int Lp,
+                                            T* __restrict__ M) {
+    constexpr int P1 = P + 1, L = P1 * P1, NS = (L + 63) / 64, RS = p2m_row<P>(), RING = p2m_ring<P>();
+    __shared__ T ring[RING * RS];   // complex Phi of the points in flight (Re at (l, m), Im at (l, -m))
+    __shared__ T F[P1 * P1];        // F[m][l]: the step factor of column m at degree l
+    const int lane = threadIdx.x;
+    for (int i = lane; i < P1 * P1; i += 64) {
+        const int m = i / P1, l = i - m * P1;
+        F[i] = l == 0 ? (T)0 : l <= m ? recip_t<T>(2 * l) : recip_t<T>(l * l - m * m);
+    }
     const int leaf = leaves[blockIdx.x];
     const double cx = center[3 * leaf], cy = center[3 * leaf + 1], cz = center[3 * leaf + 2];
     const double inv = 1.0 / sig.s[level[leaf]];
     const int jb = begin[leaf], n = end[leaf] - jb, ntask = n * P1;
-    double acc[KK][3];
+    double acc[NS][3];
 #pragma unroll
-    for (int r = 0; r < KK; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
+    for (int r = 0; r < NS; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
     // this lane's task: point pt, column m; a round moves it 64 tasks on
     int pt = lane / P1, m = lane - pt * P1;
-    const int dpt = 64 / P1, dmv = 64 - dpt * P1;
+    constexpr int dpt = 64 / P1, dmv = 64 - dpt * P1;
     int done = 0;
+    __syncthreads();   // F is written
     for (int t0 = 0; t0 < ntask; t0 += 64) {
         if (t0 + lane < ntask) {
             const int j = jb + pt;
             const T ux = (T)((sxyz[3 * j] - cx) * inv), uy = (T)((sxyz[3 * j + 1] - cy) * inv),
                     uz = (T)((sxyz[3 * j + 2] - cz) * inv);
             const T r2 = ux * ux + uy * uy + uz * uz;
-            T dr = 1, di = 0;
-            for (int i = 1; i <= m; i++) {
-                const T s = recip_t<T>(2 * i);
-                const T nr = (dr * ux - di * uy) * s, ni = (dr * uy + di * ux) * s;
-                dr = nr;
-                di = ni;
-            }
-            T* row = ring + (pt % RING) * L;
-            int idx = m * m + 2 * m;   // (l, m) at l = m; (l, -m) is idx - 2m
-            row[idx] = dr;
-            if (m > 0) row[idx - 2 * m] = di;
-            T p1r = dr, p1i = di, p2r = 0, p2i = 0, c = (T)(2 * m + 1);
-            for (int l = m + 1; l <= p; l++) {
-                const T f = recip_t<T>(l * l - m * m), a = c * uz;
-                const T vr = (a * p1r - r2 * p2r) * f, vi = (a * p1i - r2 * p2i) * f;
+            T* row = ring + (pt % RING) * RS;
+            const T* Fm = F + m * P1;
+            T p1r = 1, p1i = 0, p2r = 0, p2i = 0;
+            if (m == 0) row[0] = (T)1;
+#pragma unroll
+            for (int l = 1; l <= P; l++) {
+                const bool sect = l <= m;
+                const T ar = sect ? ux : (T)(2 * l - 1) * uz, ai = sect ? uy : (T)0;
+                const T b = l <= m + 1 ? (T)0 : r2, f = Fm[l];
+                const T vr = fma_t(ar, p1r, fma_t(-ai, p1i, -(b * p2r))) * f;
+                const T vi = fma_t(ar, p1i, fma_t(ai, p1r, -(b * p2i))) * f;
                 p2r = p1r; p2i = p1i; p1r = vr; p1i = vi;
-                idx += 2 * l;
-                row[idx] = vr;
-                if (m > 0) row[idx - 2 * m] = vi;
-                c += (T)2;
+                if (l >= m) {
+                    row[l * l + l + m] = vr;
+                    if (m > 0) row[l * l + l - m] = vi;
+                }
             }
         }
         pt += dpt;
@@ -124,17 +137,17 @@ __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, cons
         __syncthreads();   // one wave: the ring rows are written before they are read
         const int complete = min(n, (t0 + 64) / P1);
         for (int q = done; q < complete; q++) {
-            const T* row = ring + (q % RING) * L;
+            const T* row = ring + (q % RING) * RS + lane;
             const double q0 = sq[3 * (jb + q)], q1 = sq[3 * (jb + q) + 1], q2 = sq[3 * (jb + q) + 2];
+            T v[NS];
 #pragma unroll
-            for (int r = 0; r < KK; r++) {
-                const int k = lane + 64 * r;
-                if (k < L) {
-                    const double v = (double)row[k];
-                    acc[r][0] += q0 * v;
-                    acc[r][1] += q1 * v;
-                    acc[r][2] += q2 * v;
-                }
+            for (int r = 0; r < NS; r++) v[r] = row[64 * r];   // beyond L: padding, never written out
+#pragma unroll
+            for (int r = 0; r < NS; r++) {
+                const double x = (double)v[r];
+                acc[r][0] = fma_t(q0, x, acc[r][0]);
+                acc[r][1] = fma_t(q1, x, acc[r][1]);
+                acc[r][2] = fma_t(q2, x, acc[r][2]);
             }
         }
         done = complete;
@@ -142,17 +155,18 @@ __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, cons
     }
     T* out = M + (size_t)leaf * 3 * Lp;
 #pragma unroll
-    for (int r = 0; r < KK; r++) {
+    for (int r = 0; r < NS; r++) {
         const int k = lane + 64 * r;
         if (k >= Lp) break;
-        double s = 0.0;
         if (k < L) {
-            const int l = (int)sqrt((double)k), mm = k - l * l - l;   // k = l^2 + l + mm
-            s = ((l & 1) ? -1.0 : 1.0) * (mm == 0 ? 1.0 : SQRT2);
+            const int l = (int)sqrtf((float)k + 0.5f), mm = k - l * l - l;   // k = l^2 + l + mm
+            const double s = ((l & 1) ? -1.0 : 1.0) * (mm == 0 ? 1.0 : SQRT2);
+            for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = (T)(acc[r][cc] * s);
+        } else {
+            for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = (T)0;
         }
-        for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = (T)(acc[r][cc] * s);
     }
-    for (int k = lane + 64 * KK; k < Lp; k += 64)   // padding beyond what the loop above covers
+    for (int k = lane + 64 * NS; k < Lp; k += 64)   // padding beyond what the loop above covers
         for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = (T)0;
 }
 
@@ -322,9 +336,6 @@ template <> struct V4<double> {
 // denormal-scaling wrapper of rsqrtf is not needed
 __device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
-__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
-
 // P0[j] = (hi(y_j - o), q_x), P1[j] = (lo(y_j - o), q_y), P2[j] = q_z for every source slot j
 template <class R>
 __global__ void k_pack_sources(int n, const double* __restrict__ origin, const double* __restrict__ sxyz,
@@ -776,19 +787,29 @@ void launch_charges(const int* slots, int n, const double* xyz, const double* w,
 void launch_gather_xyz(const int* slots, int n, const double* xyz, double* out, hipStream_t st) {
     if (n) k_gather_xyz<<<blocks(n), 256, 0, st>>>(slots, n, xyz, out);
 }
+template <int P>
+static void p2m_go(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig,
+                   const double* sxyz, const double* sq, int Lp, void* M, hipStream_t st) {
+    if (f64)
+        k_p2m<double, P><<<nleaves, 64, 0, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
+                                                 S.level.as<int>(), sig, sxyz, sq, Lp, (double*)M);
+    else
+        k_p2m<float, P><<<nleaves, 64, 0, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
+                                                S.level.as<int>(), sig, sxyz, sq, Lp, (float*)M);
+}
 void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
                 const double* sq, int p, int Lp, void* M, hipStream_t st) {
     if (!nleaves) return;
     if (p < 0 || p > MAX_ORDER) throw Error(4, "expansion order above 20");
-    const int L = (p + 1) * (p + 1), ring = 64 / (p + 1) + 2;
-    if (f64)
-        k_p2m<double><<<nleaves, 64, (size_t)ring * L * sizeof(double), st>>>(
-            leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(), S.level.as<int>(), sig, sxyz, sq, p, Lp,
-            (double*)M);
-    else
-        k_p2m<float><<<nleaves, 64, (size_t)ring * L * sizeof(float), st>>>(
-            leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(), S.level.as<int>(), sig, sxyz, sq, p, Lp,
-            (float*)M);
+    switch (p) {
+#define FMM_P2M_P(PV) \
+    case PV: p2m_go<PV>(f64, leaves, nleaves, S, sig, sxyz, sq, Lp, M, st); break;
+        FMM_P2M_P(0) FMM_P2M_P(1) FMM_P2M_P(2) FMM_P2M_P(3) FMM_P2M_P(4) FMM_P2M_P(5) FMM_P2M_P(6) FMM_P2M_P(7)
+        FMM_P2M_P(8) FMM_P2M_P(9) FMM_P2M_P(10) FMM_P2M_P(11) FMM_P2M_P(12) FMM_P2M_P(13) FMM_P2M_P(14)
+        FMM_P2M_P(15) FMM_P2M_P(16) FMM_P2M_P(17) FMM_P2M_P(18) FMM_P2M_P(19) FMM_P2M_P(20)
+#undef FMM_P2M_P
+        default: throw Error(4, "expansion order above 20");
+    }
     FMM_HIP(hipGetLastError());
 }
 template <class T, int D, bool POT>
