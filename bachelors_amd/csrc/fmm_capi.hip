@@ -649,8 +649,8 @@ void stage_lists(fmm_ctx* c, hipStream_t ls, hipEvent_t far_ready, const std::fu
 
 // Shared frame, downward pass part 1 on stream st: operator tables (L2L per level and
 // orientation, one M2L operator per class), the M2L tiles and the cleared local expansions.
-// Needs the far pairs and classes only (not the multipoles), so the fused evaluation runs it on
-// the lists' stream beside P2M and M2M.
+// Needs the far pairs and classes only (not the multipoles); the fused evaluation queues it on
+// the main stream after the M2M and the column split, once the far classes are known.
 void translate_tables(fmm_ctx* c, hipStream_t st) {
     const TreeDev& S = c->S;
     const TreeDev& T = c->T;
@@ -717,20 +717,24 @@ void translate_tables(fmm_ctx* c, hipStream_t st) {
     c->rec_on(E_TABLES, st);
 }
 
+// F16X3, stream A after the multipoles: the B side of the FP16 split of the M2L (column scales
+// and hi / lo parts of every source node's multipoles); needs neither the lists nor the tables.
+void split_for_gemm(fmm_ctx* c) {
+    const int ncols = 3 * c->S.num_nodes, kp = m2l_f16_kpad(c->L);
+    split_columns(c->M.as<float>(), ncols, c->L, c->Lp, kp, c->dm.as<float>(), c->cscale.get<float>(ncols),
+                  c->Mhl.get<_Float16>((size_t)2 * ncols * kp), c->stream);
+}
+
 // Part 2 (stream A, after the tables and the multipoles): the M2L of all classes in one launch
 // (fast_multipole_solver.cpp:258-276) writing one slab per (pair, component); each target node
 // then sums its pairs' slabs in pair order (part 3: no atomics, deterministic).  E_GEMM0 ..
-// E_M2L_GEMM brackets the GEMM alone.
-void translate_gemm(fmm_ctx* c, hipEvent_t gemm_after = nullptr) {
+// E_M2L_GEMM brackets the GEMM alone.  split_done: split_for_gemm has run on stream A.
+void translate_gemm(fmm_ctx* c, hipEvent_t gemm_after = nullptr, bool split_done = false) {
     hipStream_t st = c->stream;
     const OpTables& O = c->down_ops;
     const Pass& P = c->m2l_pass;
     const bool f16 = c->prec == Precision::F16X3;
-    if (f16 && P.ntiles) {   // the B side of the FP16 split: column scales and hi / lo parts
-        const int ncols = 3 * c->S.num_nodes, kp = m2l_f16_kpad(c->L);
-        split_columns(c->M.as<float>(), ncols, c->L, c->Lp, kp, c->dm.as<float>(), c->cscale.get<float>(ncols),
-                      c->Mhl.get<_Float16>((size_t)2 * ncols * kp), st);
-    }
+    if (f16 && P.ntiles && !split_done) split_for_gemm(c);
     if (gemm_after) FMM_HIP(hipStreamWaitEvent(st, gemm_after, 0));
     c->rec(E_GEMM0);
     if (P.ntiles) {
@@ -939,8 +943,7 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     // Side thread (stream Bs): target tree, targets in slot order, then -- once the source tree
     // is built -- the far lists (E_FAR), the close lists and the pair index (j_lists).  This
     // thread (stream A): source tree, charges, P2M, M2M, then -- once the far classes are known
-    // on the host -- the operator tables (or, below, the side stream's) and the M2L GEMM.
-    // Neither thread blocks on the other's
+    // on the host -- the operator tables and the M2L GEMM.  Neither thread blocks on the other's
     // GPU work except through events; the side thread only waits for the counts it reads back.
     double tc[3], th[3];
     trees_begin(c, tc, th);
@@ -956,11 +959,6 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
     // tile it only slows them).
     const bool p2p_beside = c->prec == Precision::F16X3 && c->cfg.near_field == FMM_NEAR_P2P &&
                             m2l_tile_cols(c) == 128 && m2l_f16x3_persistent(c->L);
-    // With the near field beside the GEMM the M2L tables are built on the side stream after the
-    // lists (beside the M2M; the near field queued there waits for the GEMM anyway), and the GEMM
-    // waits for them.  Otherwise they run on the main stream once the far classes are known.
-    const bool tables_beside = p2p_beside;
-    int tables_state = 0;
     std::exception_ptr side_error;
     auto post = [&](int& state, int v) {
         {
@@ -981,11 +979,6 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
                 stage_lists(c, Bs, c->ev[E_FAR], [&] { post(far_state, 1); });   // E_FAR also joins
                 FMM_HIP(hipEventRecord(c->j_lists, Bs));
                 post(lists_state, 1);
-                if (tables_beside) {
-                    translate_tables(c, Bs);
-                    FMM_HIP(hipEventRecord(c->j_tables, Bs));   // follows j_lists
-                    post(tables_state, 1);
-                }
                 if (p2p_beside) {
                     {
                         std::unique_lock<std::mutex> lk(mu);
@@ -1003,7 +996,6 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
         }
         if (far_state == 0) post(far_state, 2);
         if (lists_state == 0) post(lists_state, 2);
-        if (tables_state == 0) post(tables_state, 2);
     });
     try {
         build_source_side(c);
@@ -1014,31 +1006,26 @@ void run_fused(fmm_ctx* c, double current, double* Bdev) {
         stage_leaves(c);
         FMM_HIP(hipEventRecord(c->j_pack, A));   // packed near-field sources, for the P2P
         stage_upward(c);
+        // the column split needs only the multipoles: queued before any wait for the side thread,
+        // so that it follows the M2M at once (the tables, which need the far classes, follow it)
+        const bool split_early = c->prec == Precision::F16X3 && c->S.num_nodes > 0;
+        if (split_early) split_for_gemm(c);
         {
             std::unique_lock<std::mutex> lk(mu);
             cv.wait(lk, [&] { return far_state != 0; });
         }
         if (far_state == 1) {
-            if (!tables_beside) {
-                FMM_HIP(hipStreamWaitEvent(A, c->ev[E_FAR], 0));   // far pairs and classes
-                translate_tables(c, A);
-            }
+            FMM_HIP(hipStreamWaitEvent(A, c->ev[E_FAR], 0));   // far pairs and classes
+            translate_tables(c, A);
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv.wait(lk, [&] { return lists_state != 0 && (!tables_beside || tables_state != 0); });
+                cv.wait(lk, [&] { return lists_state != 0; });
             }
             // the GEMM starts after the lists' last kernels (the pair index sort): beside the
             // persistent GEMM a sort whose blocks wait on each other crawled (17 ms instead of
             // 0.1), holding back the near field queued after it; normally the lists end first
-            if (!tables_beside) {
-                translate_gemm(c, lists_state == 1 ? c->j_lists : nullptr);
-                post(gemm_state, 1);
-            } else if (tables_state == 1) {
-                translate_gemm(c, c->j_tables);
-                post(gemm_state, 1);
-            } else {
-                post(gemm_state, 2);   // the side thread's error is rethrown after the join
-            }
+            translate_gemm(c, lists_state == 1 ? c->j_lists : nullptr, split_early);
+            post(gemm_state, 1);
         } else {
             post(gemm_state, 2);
         }
