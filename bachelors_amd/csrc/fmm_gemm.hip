@@ -181,6 +181,21 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ ops, const G
         }
         int pair = col / 3, comp = col - 3 * pair;
         T* dst = dst_exp + ((size_t)pair_dst[pair] * 3 + comp) * Lp;
+        if constexpr (sizeof(T) == 4) {
+            if (epi == EPI_ADD) {   // a lane's four rows are consecutive: one 16-byte read-modify-write
+#pragma unroll
+                for (int a = 0; a < RB; a++) {
+                    float4* q = (float4*)(dst + 16 * (wave + 4 * a) + 4 * (lane >> 4));
+                    float4 v = *q;
+                    v.x += acc[a][b][0];
+                    v.y += acc[a][b][1];
+                    v.z += acc[a][b][2];
+                    v.w += acc[a][b][3];
+                    *q = v;
+                }
+                continue;
+            }
+        }
 #pragma unroll
         for (int a = 0; a < RB; a++)
 #pragma unroll
@@ -317,6 +332,19 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
         }
         int pair = col / 3, comp = col - 3 * pair;
         float* dst = dst_exp + ((size_t)pair_dst[pair] * 3 + comp) * Lp + wrow;
+        if (epi == EPI_ADD) {   // a lane's four rows are consecutive: one 16-byte read-modify-write
+#pragma unroll
+            for (int a = 0; a < RB; a++) {
+                float4* q = (float4*)(dst + a * 16 + 4 * (lane >> 4));
+                float4 v = *q;
+                v.x += acc[a][b][0];
+                v.y += acc[a][b][1];
+                v.z += acc[a][b][2];
+                v.w += acc[a][b][3];
+                *q = v;
+            }
+            continue;
+        }
 #pragma unroll
         for (int a = 0; a < RB; a++)
 #pragma unroll
