@@ -443,7 +443,9 @@ void stage_leaves(fmm_ctx* c) {
     require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built");
     size_t n = (size_t)c->S.num_nodes * 3 * c->Lp;
     void* M = c->M.reserve(n * c->esize());
-    FMM_HIP(hipMemsetAsync(M, 0, n * c->esize(), c->stream));
+    // k_p2m writes its leaves whole, so only the other nodes are cleared (all of them for BEM)
+    if (c->bem) FMM_HIP(hipMemsetAsync(M, 0, n * c->esize(), c->stream));
+    else launch_zero_unwritten(c->S, c->Lp, c->esize(), M, c->stream);
     if (c->bem)   // Problem::BEM (multipole_solver.cpp:78-83 -> integration.cpp:295-353)
         launch_p2m_bem(c->f64(), c->src_leaves.as<int>(), (int)c->h_src_leaves.size(), c->S, c->sigS,
                        c->sxyz.as<double>(), c->sq.as<double>(), c->sbd.as<double>(), c->p, c->Lp, M, c->stream);

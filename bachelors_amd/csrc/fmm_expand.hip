@@ -170,6 +170,17 @@ __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, cons
         for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = (T)0;
 }
 
+// The nodes k_p2m does not write (subdivided ones and leaves without points), cleared with
+// 16-byte stores, one wave per node (n4 vectors per node).
+__global__ __launch_bounds__(256) void k_zero_unwritten(const int* __restrict__ first_child,
+                                                        const int* __restrict__ npts, int nnodes, int n4,
+                                                        float4* __restrict__ M) {
+    const int v = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (v >= nnodes || (first_child[v] < 0 && npts[v] > 0)) return;
+    float4* o = M + (size_t)v * n4;
+    for (int i = lane; i < n4; i += 64) o[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // ---------------------------------------------------------------- L2P
 // One wave per useful target leaf; lane (m, g) evaluates column m for the targets g, g+G, ...
 // (G = 64 / p) and the p lanes of a target are summed through LDS.  The local expansion is turned into the complex
@@ -786,6 +797,14 @@ void launch_charges(const int* slots, int n, const double* xyz, const double* w,
 }
 void launch_gather_xyz(const int* slots, int n, const double* xyz, double* out, hipStream_t st) {
     if (n) k_gather_xyz<<<blocks(n), 256, 0, st>>>(slots, n, xyz, out);
+}
+void launch_zero_unwritten(const TreeDev& S, int Lp, size_t esize, void* M, hipStream_t st) {
+    const size_t bytes = (size_t)3 * Lp * esize;
+    if (!S.num_nodes) return;
+    if (bytes % 16) throw Error(1, "expansion rows must be whole 16-byte vectors");
+    k_zero_unwritten<<<(S.num_nodes + 3) / 4, 256, 0, st>>>(S.first_child.as<int>(), S.npts.as<int>(), S.num_nodes,
+                                                            (int)(bytes / 16), (float4*)M);
+    FMM_HIP(hipGetLastError());
 }
 template <int P>
 static void p2m_go(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig,

@@ -8,6 +8,9 @@ void launch_charges(const int* slots, int n, const double* xyz, const double* w,
 void launch_gather_xyz(const int* slots, int n, const double* xyz, double* out, hipStream_t st);
 void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
                 const double* sq, int p, int Lp, void* M, hipStream_t st);
+// clears the expansions (3 Lp elements of esize bytes per node) of every node launch_p2m does not
+// write: subdivided nodes (the M2M adds into them) and leaves without points
+void launch_zero_unwritten(const TreeDev& S, int Lp, size_t esize, void* M, hipStream_t st);
 void launch_l2p(bool f64, const int* leaves, int nleaves, const TreeDev& T, const ScaleTable& sig, const double* txyz,
                 const void* Lexp, int p, int Lp, double* Bfar, hipStream_t st);
 // near-field sources in slot order relative to origin[3] (a device pointer); P0, P1: 4 x R per
