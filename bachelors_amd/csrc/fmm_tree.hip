@@ -902,7 +902,10 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
         uint32_t* rank = S.pos.get<uint32_t>(n + 1);
         double3 rc = make_double3(root_c[0], root_c[1], root_c[2]);
         double3 rh = make_double3(root_h[0], root_h[1], root_h[2]);
-        k_paths<<<blocks(n), 256, 0, st>>>(d_xyz, n, rc, rh, H, depth, key, irr, inroot);
+        // the first attempt needs the top short_depth levels of the paths only; the 64-bit one
+        // below recomputes them to full depth
+        const int short_depth = std::min(depth, SHORT_LEVELS);
+        k_paths<<<blocks(n), 256, 0, st>>>(d_xyz, n, rc, rh, H, short_depth, key, irr, inroot);
         exclusive_scan_u32(inroot, rank, n, S.tmp, st);
         int* cnt2 = S.need.get<int>(4);
         k_count_root<<<1, 1, 0, st>>>(inroot, rank, n, cnt2);
@@ -912,12 +915,10 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
         int* idx = S.val.get<int>(std::max(n0, 1));       // root's points in input order
         int* order = S.order.get<int>(std::max(n0, 1));   // ... sorted by path
         bool ok = false;
-        const int short_depth = std::min(depth, SHORT_LEVELS);
         if (n0 > 0) {
             uint32_t* k32 = S.key.get<uint32_t>(n0);
             uint32_t* k32s = S.key2.get<uint32_t>(n0);
-            k_compact_paths<uint32_t><<<blocks(n), 256, 0, st>>>(key, inroot, rank, n, 3 * (depth - short_depth), k32,
-                                                                 idx);
+            k_compact_paths<uint32_t><<<blocks(n), 256, 0, st>>>(key, inroot, rank, n, 0, k32, idx);
             sort_points_u32(k32, k32s, idx, order, n0, 3 * short_depth, S.tmp, st);
             // node bound of the queued build: 8 n0 / thr children per level
             const size_t per_level = 8 * ((size_t)n0 / thr + 1);
@@ -931,6 +932,8 @@ void build_tree(TreeDev& T, TreeScratch& S, const double* d_xyz, int n, const do
             if (!ok) {
                 uint64_t* kc = S.k64c.get<uint64_t>(n0);
                 uint64_t* kcs = S.k64s.get<uint64_t>(n0);
+                if (depth > short_depth)
+                    k_paths<<<blocks(n), 256, 0, st>>>(d_xyz, n, rc, rh, H, depth, key, irr, inroot);
                 k_compact_paths<uint64_t><<<blocks(n), 256, 0, st>>>(key, inroot, rank, n, 0, kc, idx);
                 sort_pairs_u64(kc, kcs, idx, order, n0, 3 * depth, S.tmp, st);
                 ok = build_from_sorted<uint64_t>(T, S, kcs, n0, depth, thr, max_depth, root_c, st);
