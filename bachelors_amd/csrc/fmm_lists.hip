@@ -237,22 +237,24 @@ __global__ void k_seg_flags_u64(const uint64_t* __restrict__ code, int n, uint32
     flag[i] = (i == 0 || code[i] != code[i - 1]) ? 1u : 0u;
 }
 
-__global__ void k_close_keys(const int2* __restrict__ close, int n, uint64_t* __restrict__ key, int* __restrict__ val) {
+// close pairs keyed (target, source) with the source in the low sbits bits
+__global__ void k_close_keys(const int2* __restrict__ close, int n, int sbits, uint64_t* __restrict__ key,
+                             int* __restrict__ val) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    key[i] = ((uint64_t)(uint32_t)close[i].x << 32) | (uint32_t)close[i].y;
+    key[i] = ((uint64_t)(uint32_t)close[i].x << sbits) | (uint32_t)close[i].y;
     val[i] = close[i].y;
 }
 
 
 // sorted keys back to (target, source[, class code]): far keys carry the target in the low
 // target_bits (class code above), close keys in the high word
-__global__ void k_unpack_pairs(const uint64_t* __restrict__ key, const int* __restrict__ val, int n, int target_high,
+__global__ void k_unpack_pairs(const uint64_t* __restrict__ key, const int* __restrict__ val, int n, int target_shift,
                                int target_bits, int* __restrict__ out_t, int* __restrict__ out_s,
                                uint32_t* __restrict__ out_code) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out_t[i] = target_high ? (int)(uint32_t)(key[i] >> 32) : (int)(uint32_t)(key[i] & ((1ull << target_bits) - 1));
+    out_t[i] = (int)(uint32_t)((key[i] >> target_shift) & ((1ull << target_bits) - 1));
     out_s[i] = val[i];
     if (out_code) out_code[i] = (uint32_t)(key[i] >> target_bits);
 }
@@ -311,13 +313,13 @@ static int bits_for(int n) {   // smallest b with 2^b >= n (at least 1)
 // sort by the low end_bit bits of the 64-bit key and unpack.  The traversal reaches every
 // (source, target) pair at most once (each pair has a unique parent pair), so the sets need no
 // de-duplication, unlike the reference's std::set inserts.
-static void sort_unpack(ListScratch& L, int n, int end_bit, int target_high, int target_bits, int* out_t, int* out_s,
+static void sort_unpack(ListScratch& L, int n, int end_bit, int target_shift, int target_bits, int* out_t, int* out_s,
                         uint32_t* out_code, hipStream_t st) {
     if (n == 0) return;
     uint64_t* k2 = L.key2.get<uint64_t>(n);
     int* v2 = L.val2.get<int>(n);
     sort_pairs_u64(L.key.as<uint64_t>(), k2, L.val.as<int>(), v2, n, end_bit, L.tmp, st);
-    k_unpack_pairs<<<blocks(n), 256, 0, st>>>(k2, v2, n, target_high, target_bits, out_t, out_s, out_code);
+    k_unpack_pairs<<<blocks(n), 256, 0, st>>>(k2, v2, n, target_shift, target_bits, out_t, out_s, out_code);
 }
 
 void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T, double mac_scale, bool classes,
@@ -463,10 +465,12 @@ void build_lists(Lists& out, ListScratch& L, const TreeDev& S, const TreeDev& T,
         {
             uint64_t* key = L.key.get<uint64_t>(std::max(nclose, 1));
             int* val = L.val.get<int>(std::max(nclose, 1));
-            if (nclose > 0) k_close_keys<<<blocks(nclose), 256, 0, st>>>(close, nclose, key, val);
+            // (target, source) in tbits + sbits bits: the radix sort runs over those bits only
+            const int tb = bits_for(T.num_nodes), sb = bits_for(S.num_nodes);
+            if (nclose > 0) k_close_keys<<<blocks(nclose), 256, 0, st>>>(close, nclose, sb, key, val);
             int* ct = out.close_t.get<int>(std::max(nclose, 1));
             int* cs = out.close_s.get<int>(std::max(nclose, 1));
-            sort_unpack(L, nclose, 64, 1, 32, ct, cs, nullptr, st);
+            sort_unpack(L, nclose, tb + sb, sb, tb, ct, cs, nullptr, st);
             out.num_close = nclose;
             int* cb = out.close_begin.get<int>(std::max(T.num_nodes, 1));
             int* ce = out.close_end.get<int>(std::max(T.num_nodes, 1));
