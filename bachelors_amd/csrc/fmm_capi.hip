@@ -643,8 +643,12 @@ void downward_per_pair(fmm_ctx* c) {
 void stage_lists(fmm_ctx* c, hipStream_t ls, hipEvent_t far_ready, const std::function<void()>& after_far = {}) {
     const bool shared = c->cfg.frame == FMM_FRAME_SHARED;
     c->rec_on(E_LISTS0, ls);
-    build_lists(c->lists, c->lscr, c->S, c->T, c->cfg.mac_scale, shared, ls, far_ready, after_far);
-    if (shared) build_pair_index(c->lists.far_t.as<int>(), c->lists.num_far, c->T.num_nodes, c->m2l_ix, ls);
+    // the pair index (a sort of the far pairs by target) is queued as soon as the far pairs are,
+    // ahead of the close lists
+    build_lists(c->lists, c->lscr, c->S, c->T, c->cfg.mac_scale, shared, ls, far_ready, [&] {
+        if (shared) build_pair_index(c->lists.far_t.as<int>(), c->lists.num_far, c->T.num_nodes, c->m2l_ix, ls);
+        if (after_far) after_far();
+    });
     c->rec_on(E_LISTS, ls);
     c->lists_ready = true;
 }
