@@ -187,6 +187,7 @@ struct fmm_ctx {
                j_caller = nullptr;                      // the caller's default-stream work before a _dev read
     long long near_inter = -1;
     bool lists_ready = false;
+    bool leaves_split = false;   // F16X3: the last P2M also wrote its leaves' split M2L columns
 
     bool f64() const { return prec == Precision::F64; }
     // precision of the M2M / L2L passes: F16X3 splits only the M2L, the others run in FP32
@@ -443,15 +444,28 @@ void stage_leaves(fmm_ctx* c) {
     require(c->stage >= ST_TREES, FMM_ERR_STATE, "trees are not built");
     size_t n = (size_t)c->S.num_nodes * 3 * c->Lp;
     void* M = c->M.reserve(n * c->esize());
+    c->leaves_split = false;
     // k_p2m writes its leaves whole, so only the other nodes are cleared (all of them for BEM)
     if (c->bem) FMM_HIP(hipMemsetAsync(M, 0, n * c->esize(), c->stream));
     else launch_zero_unwritten(c->S, c->Lp, c->esize(), M, c->stream);
     if (c->bem)   // Problem::BEM (multipole_solver.cpp:78-83 -> integration.cpp:295-353)
         launch_p2m_bem(c->f64(), c->src_leaves.as<int>(), (int)c->h_src_leaves.size(), c->S, c->sigS,
                        c->sxyz.as<double>(), c->sq.as<double>(), c->sbd.as<double>(), c->p, c->Lp, M, c->stream);
-    else
+    else {
+        // F16X3: the P2M also writes its leaves' columns of the M2L's split multipoles (the split
+        // after the M2M then covers the other nodes only)
+        F16Leaves split;
+        c->leaves_split = c->prec == Precision::F16X3 && c->S.num_nodes > 0;
+        if (c->leaves_split) {
+            const int ncols = 3 * c->S.num_nodes, kp = m2l_f16_kpad(c->L);
+            split.dm = c->dm.as<float>();
+            split.Kp = kp;
+            split.cscale = c->cscale.get<float>(ncols);
+            split.hl = c->Mhl.get<_Float16>((size_t)2 * ncols * kp);
+        }
         launch_p2m(c->f64(), c->src_leaves.as<int>(), (int)c->h_src_leaves.size(), c->S, c->sigS,
-                   c->sxyz.as<double>(), c->sq.as<double>(), c->p, c->Lp, M, c->stream);
+                   c->sxyz.as<double>(), c->sq.as<double>(), c->p, c->Lp, M, c->stream, split);
+    }
     size_t nss = (size_t)std::max(c->S.num_slots, 1), rs = c->f64() ? sizeof(double) : sizeof(float);
     // origin = source root centre (node 0); the P2P kernel reads the same three doubles
     launch_pack_sources(c->f64(), c->S.num_slots, c->S.center.as<double>(), c->sxyz.as<double>(), c->sq.as<double>(),
@@ -727,8 +741,10 @@ void translate_tables(fmm_ctx* c, hipStream_t st) {
 // and hi / lo parts of every source node's multipoles); needs neither the lists nor the tables.
 void split_for_gemm(fmm_ctx* c) {
     const int ncols = 3 * c->S.num_nodes, kp = m2l_f16_kpad(c->L);
+    const bool leaves = c->leaves_split;   // the P2M wrote its leaves' columns
     split_columns(c->M.as<float>(), ncols, c->L, c->Lp, kp, c->dm.as<float>(), c->cscale.get<float>(ncols),
-                  c->Mhl.get<_Float16>((size_t)2 * ncols * kp), c->stream);
+                  c->Mhl.get<_Float16>((size_t)2 * ncols * kp), c->stream,
+                  leaves ? c->S.first_child.as<int>() : nullptr, leaves ? c->S.npts.as<int>() : nullptr);
 }
 
 // Part 2 (stream A, after the tables and the multipoles): the M2L of all classes in one launch

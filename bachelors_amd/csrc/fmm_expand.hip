@@ -86,7 +86,8 @@ __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, cons
                                             const int* __restrict__ begin, const int* __restrict__ end,
                                             const int* __restrict__ level, ScaleTable sig,
                                             const double* __restrict__ sxyz, const double* __restrict__ sq, int Lp,
-                                            T* __restrict__ M) {
+                                            T* __restrict__ M, const float* __restrict__ dm, int Kp,
+                                            float* __restrict__ cscale, _Float16* __restrict__ hl) {
     constexpr int P1 = P + 1, L = P1 * P1, NS = (L + 63) / 64, RS = p2m_row<P>(), RING = p2m_ring<P>();
     __shared__ T ring[RING * RS];   // complex Phi of the points in flight (Re at (l, m), Im at (l, -m))
     __shared__ T F[P1 * P1];        // F[m][l]: the step factor of column m at degree l
@@ -154,20 +155,58 @@ __global__ __launch_bounds__(64) void k_p2m(const int* __restrict__ leaves, cons
         __syncthreads();   // the rows are read before the next round overwrites them
     }
     T* out = M + (size_t)leaf * 3 * Lp;
+    T mv[NS][3];   // the leaf's multipoles as written (zero beyond L)
 #pragma unroll
     for (int r = 0; r < NS; r++) {
         const int k = lane + 64 * r;
-        if (k >= Lp) break;
         if (k < L) {
             const int l = (int)sqrtf((float)k + 0.5f), mm = k - l * l - l;   // k = l^2 + l + mm
             const double s = ((l & 1) ? -1.0 : 1.0) * (mm == 0 ? 1.0 : SQRT2);
-            for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = (T)(acc[r][cc] * s);
+            for (int cc = 0; cc < 3; cc++) mv[r][cc] = (T)(acc[r][cc] * s);
         } else {
-            for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = (T)0;
+            for (int cc = 0; cc < 3; cc++) mv[r][cc] = (T)0;
         }
+        if (k < Lp)
+            for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = mv[r][cc];
     }
     for (int k = lane + 64 * NS; k < Lp; k += 64)   // padding beyond what the loop above covers
         for (int cc = 0; cc < 3; cc++) out[cc * Lp + k] = (T)0;
+    if constexpr (sizeof(T) == 4) {
+        // F16X3: the leaf's three columns of the M2L's B operand as k_split_columns makes them
+        // (the same FP32 operations: dm-scaled coefficients, power-of-2 column scale, FP16 hi and
+        // lo, [column][K step][hi | lo]), so that the split after the M2M covers the other nodes
+        if (hl) {
+            float dmv[NS];
+#pragma unroll
+            for (int r = 0; r < NS; r++) {
+                const int k = lane + 64 * r;
+                dmv[r] = k < L && k < Kp ? dm[k] : 0.0f;
+            }
+            for (int cc = 0; cc < 3; cc++) {
+                float x[NS], mx = 0.0f;
+#pragma unroll
+                for (int r = 0; r < NS; r++) {
+                    x[r] = mv[r][cc] * dmv[r];
+                    mx = fmaxf(mx, fabsf(x[r]));
+                }
+                for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+                const float sc = mx > 0.0f ? exp2f((float)(-(ilogbf(mx) + 1))) : 1.0f;
+                const size_t col = (size_t)leaf * 3 + cc;
+                if (lane == 0) cscale[col] = sc;
+                _Float16* dst = hl + col * 2 * Kp;
+#pragma unroll
+                for (int r = 0; r < NS; r++) {
+                    const int k = lane + 64 * r;
+                    if (k < Kp) {
+                        const float y = x[r] * sc;
+                        const _Float16 h = (_Float16)y;
+                        dst[(k >> 5) * 64 + (k & 31)] = h;
+                        dst[(k >> 5) * 64 + (k & 31) + 32] = (_Float16)(y - (float)h);
+                    }
+                }
+            }
+        }
+    }
 }
 
 // The nodes k_p2m does not write (subdivided ones and leaves without points), cleared with
@@ -808,21 +847,25 @@ void launch_zero_unwritten(const TreeDev& S, int Lp, size_t esize, void* M, hipS
 }
 template <int P>
 static void p2m_go(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig,
-                   const double* sxyz, const double* sq, int Lp, void* M, hipStream_t st) {
+                   const double* sxyz, const double* sq, int Lp, void* M, const F16Leaves& split, hipStream_t st) {
     if (f64)
         k_p2m<double, P><<<nleaves, 64, 0, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
-                                                 S.level.as<int>(), sig, sxyz, sq, Lp, (double*)M);
+                                                 S.level.as<int>(), sig, sxyz, sq, Lp, (double*)M, nullptr, 0,
+                                                 nullptr, nullptr);
     else
         k_p2m<float, P><<<nleaves, 64, 0, st>>>(leaves, S.center.as<double>(), S.begin.as<int>(), S.end.as<int>(),
-                                                S.level.as<int>(), sig, sxyz, sq, Lp, (float*)M);
+                                                S.level.as<int>(), sig, sxyz, sq, Lp, (float*)M, split.dm, split.Kp,
+                                                split.cscale, split.hl);
 }
 void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
-                const double* sq, int p, int Lp, void* M, hipStream_t st) {
+                const double* sq, int p, int Lp, void* M, hipStream_t st, const F16Leaves& split) {
     if (!nleaves) return;
     if (p < 0 || p > MAX_ORDER) throw Error(4, "expansion order above 20");
+    if (split.hl && (f64 || split.Kp % 32 || split.Kp > Lp || !split.dm || !split.cscale))
+        throw Error(4, "P2M: FP16 split of the leaves needs FP32 multipoles and 32 | Kp <= Lp");
     switch (p) {
 #define FMM_P2M_P(PV) \
-    case PV: p2m_go<PV>(f64, leaves, nleaves, S, sig, sxyz, sq, Lp, M, st); break;
+    case PV: p2m_go<PV>(f64, leaves, nleaves, S, sig, sxyz, sq, Lp, M, split, st); break;
         FMM_P2M_P(0) FMM_P2M_P(1) FMM_P2M_P(2) FMM_P2M_P(3) FMM_P2M_P(4) FMM_P2M_P(5) FMM_P2M_P(6) FMM_P2M_P(7)
         FMM_P2M_P(8) FMM_P2M_P(9) FMM_P2M_P(10) FMM_P2M_P(11) FMM_P2M_P(12) FMM_P2M_P(13) FMM_P2M_P(14)
         FMM_P2M_P(15) FMM_P2M_P(16) FMM_P2M_P(17) FMM_P2M_P(18) FMM_P2M_P(19) FMM_P2M_P(20)

@@ -6,8 +6,16 @@ namespace fmm {
 void launch_charges(const int* slots, int n, const double* xyz, const double* w, const double* J, double* sxyz,
                     double* sq, double* sJ, hipStream_t st);
 void launch_gather_xyz(const int* slots, int n, const double* xyz, double* out, hipStream_t st);
+// F16X3: where P2M also writes its leaves' columns of the M2L's split B operand (see
+// split_columns: coefficient scales dm, per-column scales, FP16 hi | lo per K step); hl null: none
+struct F16Leaves {
+    const float* dm = nullptr;
+    int Kp = 0;
+    float* cscale = nullptr;
+    _Float16* hl = nullptr;
+};
 void launch_p2m(bool f64, const int* leaves, int nleaves, const TreeDev& S, const ScaleTable& sig, const double* sxyz,
-                const double* sq, int p, int Lp, void* M, hipStream_t st);
+                const double* sq, int p, int Lp, void* M, hipStream_t st, const F16Leaves& split = F16Leaves());
 // clears the expansions (3 Lp elements of esize bytes per node) of every node launch_p2m does not
 // write: subdivided nodes (the M2M adds into them) and leaves without points
 void launch_zero_unwritten(const TreeDev& S, int Lp, size_t esize, void* M, hipStream_t st);

@@ -760,7 +760,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_vgpr(216))) void k_g
 // columns and keep their lanes' dm in registers.
 __global__ __launch_bounds__(256) void k_split_columns(const float* __restrict__ M, int ncols, int L, int Lp, int Kp,
                                                        const float* __restrict__ dm, float* __restrict__ scale,
-                                                       _Float16* __restrict__ hl) {
+                                                       _Float16* __restrict__ hl,
+                                                       const int* __restrict__ skip_first_child,
+                                                       const int* __restrict__ skip_npts) {
     const int lane = threadIdx.x & 63, i0 = 8 * lane;
     const bool act = i0 < Kp;   // Kp and Lp are multiples of 8, Kp <= Lp
     const int off = (i0 >> 5) * 64 + (i0 & 31);   // K step i0 / 32, chunk within the step
@@ -768,6 +770,10 @@ __global__ __launch_bounds__(256) void k_split_columns(const float* __restrict__
 #pragma unroll
     for (int e = 0; e < 8; e++) d[e] = act && i0 + e < L ? dm[i0 + e] : 0.0f;
     for (int c = blockIdx.x * 4 + (threadIdx.x >> 6); c < ncols; c += gridDim.x * 4) {
+        if (skip_first_child) {   // leaves with points: split by the P2M already
+            const int node = c / 3;
+            if (skip_first_child[node] < 0 && skip_npts[node] > 0) continue;
+        }
         float x[8];
         float mx = 0.0f;
         if (act) {
@@ -797,11 +803,12 @@ __global__ __launch_bounds__(256) void k_split_columns(const float* __restrict__
 }
 
 void split_columns(const float* M, int ncols, int L, int Lp, int Kp, const float* dm, float* scale, _Float16* hl,
-                   hipStream_t st) {
+                   hipStream_t st, const int* skip_first_child, const int* skip_npts) {
     if (ncols <= 0) return;
     if (Kp > 8 * 64 || Kp % 32 || Lp % 8 || Kp > Lp) throw Error(4, "FP16 split: column length out of range");
+    if (!skip_first_child != !skip_npts) throw Error(1, "FP16 split: node flags come in pairs");
     const int grid = std::min((ncols + 3) / 4, 8 * num_cus());
-    k_split_columns<<<grid, 256, 0, st>>>(M, ncols, L, Lp, Kp, dm, scale, hl);
+    k_split_columns<<<grid, 256, 0, st>>>(M, ncols, L, Lp, Kp, dm, scale, hl, skip_first_child, skip_npts);
     FMM_HIP(hipGetLastError());
 }
 

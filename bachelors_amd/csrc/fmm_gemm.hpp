@@ -68,8 +68,10 @@ int m2l_f16_rows(int L);
 // (128-column tiles) that leaves room for a near-field workgroup per CU
 bool m2l_f16x3_persistent(int L);
 int m2l_f16_kpad(int L);
+// skip_first_child / skip_npts (per node, both or neither): columns of leaves with points are
+// left as they are (the P2M writes them, see launch_p2m)
 void split_columns(const float* M, int ncols, int L, int Lp, int Kp, const float* dm, float* scale, _Float16* hl,
-                   hipStream_t st);
+                   hipStream_t st, const int* skip_first_child = nullptr, const int* skip_npts = nullptr);
 void m2l_gemm_f16x3(const _Float16* ops_hi, const _Float16* ops_lo, const float* ops_rinv, int op_base,
                     const GemmTile* d_tiles, int ntiles, int tile_cols, const int* pair_src, const _Float16* src_hl,
                     const float* cscale, float* slab, int L, int* queues, hipStream_t st);
