@@ -738,7 +738,8 @@ void translate_tables(fmm_ctx* c, hipStream_t st) {
 }
 
 // F16X3, stream A after the multipoles: the B side of the FP16 split of the M2L (column scales
-// and hi / lo parts of every source node's multipoles); needs neither the lists nor the tables.
+// and hi / lo parts of the source nodes' multipoles, except the leaves' the P2M already split);
+// needs neither the lists nor the tables.
 void split_for_gemm(fmm_ctx* c) {
     const int ncols = 3 * c->S.num_nodes, kp = m2l_f16_kpad(c->L);
     const bool leaves = c->leaves_split;   // the P2M wrote its leaves' columns
