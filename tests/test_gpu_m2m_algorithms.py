@@ -1,7 +1,9 @@
 """GPU: every M2M algorithm of the reference (M2MAlg, multipole_solver.cpp:102-339) gives the
 multipoles and the field of the Matrices path: NoTranslation (P2M at every node from its points),
 ComplexTranslation / RealTranslation / Layers (the per-node translation formula, no operator
-tables) and Matrices (the MFMA GEMM of per-(level, orientation) operators)."""
+tables) and Matrices (the MFMA GEMM of per-(level, orientation) operators).  In F16X3 the P2M
+also splits its leaves' M2L columns and the split after the M2M covers the nodes the algorithm
+made, so a wrong split of either part shows up in B."""
 import numpy as np
 import pytest
 
@@ -25,7 +27,7 @@ def _run(fm, src, w, tgt, precision, alg):
     return B, M
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("f16x3", 2e-6)])
 def test_every_m2m_algorithm_gives_the_matrices_result(precision, tol):
     import bachelors_amd as fm
     rng = np.random.default_rng(71)
