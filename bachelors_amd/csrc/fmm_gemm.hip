@@ -614,19 +614,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_num_vgpr(216))) void k_g
     // persistent: the grid (a multiple of 8, workgroup b on XCD b mod 8) walks the tiles, each
     // XCD a contiguous eighth of them (one class's operator shared in one L2) taken from that
     // XCD's counter xq[xcd] (zero at launch), so a workgroup that starts late (its CU busy)
-    // takes fewer tiles; a workgroup starts its next tile while its last stores drain.  The
+    // takes fewer tiles, and then from the other XCDs' eighths; a workgroup starts its next tile
+    // while its last stores drain.  The
     // register budget leaves room for one near-field workgroup (k_p2p_queue) per CU.
     const int xcd = blockIdx.x & 7, q = ntiles >> 3, rem = ntiles & 7;
-    const int xbeg = xcd * q + min(xcd, rem), xcnt = q + (xcd < rem ? 1 : 0);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, li = lane & 15;
     const int wr = wave & 1, wc = wave >> 1, rb0 = wr * NB0;
     __shared__ int next_tile;
+    int from = xcd;   // (thread 0) the eighth this workgroup draws from: its XCD's, then the others'
     for (;;) {
-        if (tid == 0) next_tile = atomicAdd(xq + xcd, 1);
+        if (tid == 0) {
+            // once its own eighth is done a workgroup takes tiles left in the others (in turn), so
+            // XCDs that finish early help the rest; each counter hands out each index once
+            int t = -1;
+            for (int s = 0; s < 8 && t < 0; s++) {
+                const int x = (from + s) & 7, cnt = q + (x < rem ? 1 : 0);
+                const int i = atomicAdd(xq + x, 1);
+                if (i < cnt) {
+                    t = x * q + min(x, rem) + i;
+                    from = x;
+                }
+            }
+            next_tile = t;
+        }
         __syncthreads();
         const int it = next_tile;   // read before the K loop's barriers; rewritten only after them
-        if (it >= xcnt) break;
-        const GemmTile tile = tiles[xbeg + it];
+        if (it < 0) break;
+        const GemmTile tile = tiles[it];
         const int op = tile.op - op_base;
         const _Float16* Ahg = ops_hi + (size_t)op * R * Kp;
         const _Float16* Alg = ops_lo + (size_t)op * R * Kp;
